@@ -1,0 +1,197 @@
+"""Benchmark: 3D 7-point FP32 Laplacian on a 512^3 periodic grid (514^3 padded,
+one ghost layer), one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one periodic Laplacian application over the whole grid: the periodic
+ghost fill (a surface-sized launch) followed by the Laplacian kernel.  The
+inputs (s ~ U[-1, 1], seeded) are resident in HBM before timing starts.
+Printed: one JSON line with GLUPS (interior lattice updates per second / 1e9),
+the Laplacian kernel's achieved bandwidth against the 8 TB/s HBM peak (HIP
+events on the launch stream, algorithmic 8 bytes per update), the Grad, Div and
+fused lap+grad+div passes on the same grid, and the CPU restatement timed on a
+bounded slab of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+N = 512
+G = 1
+P = N + 2 * G
+LUPS = N ** 3
+LAP_BYTES = 8          # read s, write lap
+GRAD_BYTES = 16        # read s, write vec3
+DIV_BYTES = 16         # read vec3, write s
+FUSED_BYTES = 36       # read s + vec3, write lap + vec3 + div
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    return p.parse_args()
+
+
+def pmc_traffic():
+    """HBM bytes per Laplacian launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary is present."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "lap3d_512_pmc.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return float(d["lap3d_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds):
+    """The C FP32 restatement (single thread, the serial reference-equivalent
+    CPU path) on slabs of the 512^3 workload: 512 x 512 x 512 planes of the
+    periodic grid are too slow to repeat, so slabs of 32 interior planes (8.4M
+    updates) are timed until about `seconds` of CPU work."""
+    from oracle import c_oracle
+    from oracle.np_oracle import laplacian_weights
+    rng = np.random.default_rng(0)
+    slab = rng.uniform(-1, 1, (32 + 2, P, P, 1)).astype(np.float32)
+    out = np.empty_like(slab)
+    w = laplacian_weights(1.0 / N)
+    lib = c_oracle.lib()
+    res = {}
+    for threads in (1, c_oracle.max_threads()):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            lib.orc_laplacian_f32(slab, out, slab.shape[0], P, P, 1, G, w[0], w[1], w[2],
+                                  np.float32(1.0), threads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > seconds / 2 or (threads == 1 and reps >= 40):
+                break
+        res[threads] = reps * 32 * N * N / el / 1e9
+    return res
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import pde_module_amd as pm
+    from pde_module_amd import _native as NV
+    from pde_module_amd.FDM import functional as F
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator(device=dev).manual_seed(0)
+    s = torch.rand((P, P, P, 1), device=dev, generator=gen) * 2 - 1
+    v = torch.rand((P, P, P, 3), device=dev, generator=gen) * 2 - 1
+    periodic = pm.PeriodicBoundary(G)
+    periodic(s)
+    periodic(v)
+    lap = pm.Laplacian(1, 1.0 / N, G)
+    grad = pm.Grad(1, (P, P, P), 1.0 / N, G)
+    div = pm.Divergence(3, (P, P, P), 1.0 / N, G)
+
+    def step():
+        periodic(s)
+        return lap(s, 1.0)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    glups = LUPS / (ms_per_step * 1e-3) / 1e9
+
+    def event_time(fn, reps):
+        """Average duration of fn's launches, HIP events on the launch stream."""
+        stream = torch.cuda.current_stream(dev)
+        for _ in range(3):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    reps = max(10, args.steps)
+    t_lap = event_time(lambda: lap(s, 1.0), reps)
+    lap_gbs = LAP_BYTES * LUPS / t_lap / 1e9
+    out_l = torch.empty_like(s)
+    out_g = torch.empty_like(v)
+    out_d = torch.empty_like(s)
+    t_grad = event_time(lambda: grad(s, 1.0), reps // 2)
+    t_div = event_time(lambda: div(v, 1.0), reps // 2)
+    t_fused = event_time(lambda: F.fused_lap_grad_div(s, v, out_l, out_g, out_d, 1.0 / N, G),
+                         reps // 2)
+    t_copy = event_time(lambda: NV.call("pde_copy_f32", s.data_ptr(), out_l.data_ptr(),
+                                        s.numel(), NV.stream_of(s)), reps // 2)
+    copy_gbs = 2 * 4 * s.numel() / t_copy / 1e9
+
+    traffic = pmc_traffic()
+    rec = {
+        "metric": "GLUPS & % HBM peak (3D 7-pt FP32); fwd+adjoint vs fwd step time",
+        "value": round(glups, 2),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: s ~ U[-1,1] (torch.Generator seed 0), periodic ghosts",
+        "config": {"workload": "3D 7-point Laplacian, 512^3 periodic grid (514^3 padded, g=1), "
+                               "step = periodic ghost fill + Laplacian",
+                   "grid_padded": [P, P, P], "interior_points": LUPS,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(lap_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "lap3d_march", "avg_launch_us": round(t_lap * 1e6, 2),
+                     "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
+        "extra": {
+            "lap_glups_kernel_only": round(LUPS / t_lap / 1e9, 1),
+            "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1)},
+            "div": {"us": round(t_div * 1e6, 1), "GBs": round(DIV_BYTES * LUPS / t_div / 1e9, 1)},
+            "fused_lap_grad_div": {"us": round(t_fused * 1e6, 1),
+                                   "GBs": round(FUSED_BYTES * LUPS / t_fused / 1e9, 1),
+                                   "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},
+            "copy_GBs": round(copy_gbs, 1),
+        },
+    }
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_seconds)
+        threads = max(cpu)
+        rec["cpu_baseline"] = {"value": round(cpu[1], 4), "unit": "GLUPS", "cores": 1,
+                               "kind": "port",
+                               "sample": "C FP32 restatement of the reference Laplacian "
+                                         "(oracle/pde_oracle.c, -O2 -ffp-contract=off), "
+                                         "32-plane slabs of the 514^3 grid, single thread",
+                               "multicore": {"value": round(cpu[threads], 4), "cores": threads}}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * pde_hip.h — C ABI of libpdehip.so, the MI355X (gfx950) kernels behind
+ * pde_module_amd's uniform-grid finite-difference modules.
+ *
+ * Every entry point:
+ *   - works on caller-owned device pointers (FP32, C-order, AoS);
+ *   - launches asynchronously on `stream` (a hipStream_t passed as void*;
+ *     NULL = the legacy default stream), so a caller may capture it in a graph;
+ *   - never allocates, frees or synchronises;
+ *   - returns PDE_OK (0), a negative PDE_E* argument error, or a positive
+ *     hipError_t from the launch.
+ *
+ * Field layout (reference pm/mesh/mesh_functions.py:108-159): padded extents
+ * (nx, ny, nz), size-1 axes are not grid axes; element n = (i*ny + j)*nz + k
+ * holds a vector of ncomp floats, or an N x D row-major matrix.  Eligible axes
+ * are those of size > 1, ascending (pm/FDM/utils.py:6-27).  Operators write the
+ * interior nodes only (index in [g, P-g) on eligible axes) and leave output
+ * ghosts untouched (pm/stencil/stencil.py:38-56: outputs are wp.empty).
+ *
+ * Arithmetic keeps the reference's operation order with one rounding per + and
+ * per * (the library is built with -ffp-contract=off), so results are
+ * reproducible bit for bit by a CPU restatement in the same order.
+ */
+#ifndef PDE_HIP_H
+#define PDE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDE_OK 0
+#define PDE_EINVAL (-1)   /* null pointer, bad count or unsupported layout */
+#define PDE_ESHAPE (-2)   /* grid too small for the ghost width / stencil  */
+
+/* ---------------------------------------------------------------- operators */
+
+/* Laplacian of a vec(ncomp) field: out[n] = alpha * sum over eligible axes
+ * (ascending) of (w_minus*in[n-e] + w_center*in[n] + w_plus*in[n+e]),
+ * accumulated from 0 term by term.
+ * Replaces pm/FDM/functional/_laplacian.py:5-36 (wp.launch of the kernel built
+ * by pm/FDM/kernel/laplacian.py:7-61), called from Laplacian.forward
+ * (pm/FDM/module/laplacian.py:54-73). */
+int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz,
+                      int32_t ncomp, int32_t ghost, float w_minus, float w_center,
+                      float w_plus, float alpha, void *stream);
+
+/* The same Laplacian through the straightforward one-thread-per-element
+ * kernel only (pde_laplacian_f32 switches 3D scalar fields to an x-marching
+ * kernel).  Same results bit for bit; kept so the two can be compared and
+ * timed side by side.  Replaces the same reference call as pde_laplacian_f32. */
+int pde_laplacian_generic_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                              int64_t nz, int32_t ncomp, int32_t ghost, float w_minus,
+                              float w_center, float w_plus, float alpha, void *stream);
+
+/* Central gradient of a vec(ncomp) field into an ncomp x D matrix per node
+ * (D = number of eligible axes; for ncomp == 1 this is also the vec(D)
+ * layout): out[n][r][t] = alpha * (w_minus*in[n-e_t][r] + w_center*in[n][r] +
+ * w_plus*in[n+e_t][r]).
+ * Replaces pm/FDM/functional/_grad.py:5-36 with the kernel of
+ * pm/FDM/kernel/grad.py:8-102, called from Grad.forward (module/grad.py:68-88). */
+int pde_grad_f32(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz,
+                 int32_t ncomp, int32_t ghost, float w_minus, float w_center,
+                 float w_plus, float alpha, void *stream);
+
+/* Divergence of an nrows x D matrix field (nrows == 1: a vec(D) field) into a
+ * vec(nrows) field: out[n][r] = alpha * sum over axes t, then s in (-1,0,1),
+ * of w_s * in[n + s e_t][r][t].
+ * Replaces pm/FDM/functional/_divergence.py:4-24 with the kernel of
+ * pm/FDM/kernel/divergence.py:7-108.  The reference module launches it twice
+ * per call with the same result (module/divergence.py:98-111); one launch here. */
+int pde_divergence_f32(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz,
+                       int32_t nrows, int32_t ghost, float w_minus, float w_center,
+                       float w_plus, float alpha, void *stream);
+
+/* Fused 3D operator pass over one scalar field s and one vec(3) field v on the
+ * same padded grid: lap = alpha_lap * Laplacian(s), grad = alpha_grad * Grad(s)
+ * (vec3 per node), div = alpha_div * Divergence(v), each bit-identical to the
+ * separate entry points above with the same weights.  No reference
+ * counterpart as a single call: it replaces Laplacian, Grad and Divergence
+ * launched back to back on the same grid (pm/FDM/module/{laplacian,grad,
+ * divergence}.py).  Requires three eligible axes. */
+int pde_fused_lap_grad_div_f32(const float *s, const float *v, float *lap, float *grad,
+                               float *div, int64_t nx, int64_t ny, int64_t nz,
+                               int32_t ghost, float wl_minus, float wl_center,
+                               float wl_plus, float wc_minus, float wc_center,
+                               float wc_plus, float alpha_lap, float alpha_grad,
+                               float alpha_div, void *stream);
+
+/* ------------------------------------------------------------- time stepping */
+
+/* out[i] = cur[i] + dt * k[i] for i < n (n = number of floats, ghosts
+ * included).  out may alias cur or k.
+ * Replaces ForwardEuler.forward (pm/time_step/forwardEuler.py:41-61, kernel
+ * :70-79). */
+int pde_forward_euler_f32(const float *cur, const float *k, float *out, int64_t n,
+                          float dt, void *stream);
+
+/* Plain device copy of n floats (out = in).  Replaces the wp.copy in
+ * GridBoundary.copy_array (pm/FDM/boundary/functional_boundary.py:260-263,
+ * pm/deprecated/gridBoundary.py:193-196); also the bandwidth yardstick. */
+int pde_copy_f32(const float *in, float *out, int64_t n, void *stream);
+
+/* Fused explicit heat step on a vec(ncomp) field (reference loop
+ * examples/2D_Heat_Equation.py:58-62: u2 = BC(u); L = lap(u2, alpha);
+ * u_next = u2 + dt * L).  `u` must already hold u2 (the boundary ops applied
+ * in place, see pde_bc_apply_inplace_f32); writes, for every non-ghost node,
+ * u_next[n] = u2[n] + dt * (alpha * acc(u2, n)) with acc the Laplacian sum,
+ * and copies u2's ghost values into u_next's ghosts.  u_next must not alias u. */
+int pde_heat_step_f32(const float *u, float *u_next, int64_t nx, int64_t ny, int64_t nz,
+                      int32_t ncomp, int32_t ghost, float w_minus, float w_center,
+                      float w_plus, float alpha, float dt, void *stream);
+
+/* Fused u/v wave step on a scalar field (reference loop
+ * examples/wave2d/standingWave.py:72-76 with a per-node coefficient):
+ *   v_next[n] = v[n] + dt * (c2[n] * (alpha * acc(u2, n)))
+ *   u_next[n] = u2[n] + dt * (first_step ? v[n] : v_next[n])
+ * for non-ghost nodes; `u` holds u2 (boundary already applied).  c2 == NULL
+ * means a uniform coefficient of 1 (then the scale is alpha alone).  v_next
+ * may alias v; u_next must not alias u. */
+int pde_wave_step_f32(const float *u, const float *v, const float *c2, float *u_next,
+                      float *v_next, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                      float w_minus, float w_center, float w_plus, float alpha, float dt,
+                      int32_t first_step, void *stream);
+
+/* ------------------------------------------------------------------ boundary */
+
+/* One resolved boundary write.  v = value, or min(t/tmax, 1) * value when
+ * tmax > 0 (t read from device memory at launch).  kinds:
+ *   PDE_BC_SET:     out[dst] = v
+ *   PDE_BC_COPY:    out[dst] = in[src]
+ *   PDE_BC_GHOST_D: out[dst] = 2*v - in[src]            (Dirichlet ghost)
+ *   PDE_BC_GHOST_N: out[dst] = coef*v + in[src]         (Neumann ghost, coef =
+ *                                                       -sign(s)*2*dx) */
+#define PDE_BC_SET 0
+#define PDE_BC_COPY 1
+#define PDE_BC_GHOST_D 2
+#define PDE_BC_GHOST_N 3
+
+typedef struct pde_bc_op {
+    int64_t dst;
+    int64_t src;
+    int32_t kind;
+    float value;
+    float coef;
+    float tmax;
+} pde_bc_op; /* 32 bytes */
+
+/* out = in (n floats; in and out must not alias), then every op applied with
+ * all reads from `in`.  The host resolves
+ * a GridBoundary's groups into `ops` so that each dst appears once; the result
+ * equals the reference's copy + sequential group launches.
+ * Replaces GridBoundary.forward of pm/FDM/boundary/functional_boundary.py:265-294
+ * (kernel :356-400) and of pm/deprecated/gridBoundary.py:218-246 (kernel
+ * :267-302).  t may be NULL when no op has tmax > 0. */
+int pde_bc_apply_f32(const float *in, float *out, int64_t n, const pde_bc_op *ops,
+                     int64_t nops, const float *t, void *stream);
+
+/* The same ops applied in place to u: every op's value is first gathered from
+ * the unmodified u into staging[nops], then scattered, so reads still see the
+ * pre-boundary field.  staging holds nops floats of caller-owned device
+ * memory.  No reference counterpart as one call: it is the copy-free form of
+ * GridBoundary used by the fused steps. */
+int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t nops, const float *t,
+                             float *staging, void *stream);
+
+/* Periodic ghost fill, in place: for each eligible axis ascending, ghost layer
+ * i < g takes interior layer i + E and ghost layer i >= P - g takes i - E
+ * (E = P - 2g), over the full extent of the other axes.  No reference
+ * counterpart (the reference has no periodic boundary); it supplies the
+ * periodic grids the 3D operator benchmark runs on. */
+int pde_periodic_fill_f32(float *u, int64_t nx, int64_t ny, int64_t nz, int32_t ncomp,
+                          int32_t ghost, void *stream);
+
+/* ------------------------------------------------------------------- library */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pde_version(void);
+
+/* Human-readable name of a status code returned by any entry point. */
+const char *pde_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDE_HIP_H */

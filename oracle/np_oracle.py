@@ -1,0 +1,287 @@
+"""ORACLE — test infrastructure only (tests/, smoke() and bench.py's cpu_baseline).
+
+NumPy restatement of PDE_Module's uniform-grid FDM hot path, parametrised on the
+floating type:
+
+* ``dtype=np.float32`` — a second, independent FP32 restatement.  It keeps the
+  reference's operation order (NumPy rounds every + and * separately), so it must
+  agree bit for bit with the C restatement in ``pde_oracle.c``; the two are
+  written differently (vectorised slices here, scalar loops there), so an
+  indexing mistake in either shows up as a mismatch.
+* ``dtype=np.float64`` — the FP64 oracle used to report accumulated error.
+
+Fields are arrays of padded shape ``(X, Y, Z, C)`` (vector) or ``(X, Y, Z, N, D)``
+(matrix), C-order, components innermost (reference
+``pm/mesh/mesh_functions.py:108-159``).  Operator outputs have NaN ghosts here, so
+any path that reads an uninitialised ghost shows up as NaN in the interior.
+
+Pinned by tests/test_oracle_*.py: closed forms (quadratics, discrete
+eigenfunctions, whole-loop heat and standing-wave solutions), scipy brute force
+(sparse Kronecker sums, ``ndimage.correlate``), ``numpy.gradient``, summation by
+parts, and the boundary identities (``(ghost + adj) / 2 == v`` etc.).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+# --------------------------------------------------------------------------- geometry
+def eligible_axes(shape3):
+    """Axes with size > 1, ascending (reference ``pm/FDM/utils.py:24-27``)."""
+    return tuple(a for a, s in enumerate(shape3) if s > 1)
+
+
+def interior(shape3, g, axis=None, shift=0):
+    """Slices of the interior nodes, optionally shifted by ``shift`` along ``axis``
+    (reference ``pm/FDM/kernel/laplacian.py:54-57``: node = tid + g on eligible
+    axes)."""
+    out = []
+    for a, p in enumerate(shape3):
+        lo, hi = (g, p - g) if p > 1 else (0, p)
+        if a == axis:
+            lo, hi = lo + shift, hi + shift
+        out.append(slice(lo, hi))
+    return tuple(out)
+
+
+def interior_mask(shape3, g):
+    m = np.zeros(shape3, dtype=bool)
+    m[interior(shape3, g)] = True
+    return m
+
+
+# --------------------------------------------------------------------------- weights
+def laplacian_weights(dx, dtype=np.float32):
+    """Reference ``pm/FDM/module/laplacian.py:32-35``: FP64 values cast to the
+    field's float type."""
+    return tuple(dtype(v) for v in (1.0 / dx**2, -2.0 / dx**2, 1.0 / dx**2))
+
+
+def central_weights(dx, dtype=np.float32):
+    """Reference ``pm/FDM/module/grad.py:37-39`` and ``divergence.py:36-39``."""
+    return tuple(dtype(v) for v in (-1.0 / (2 * dx), 0.0, 1.0 / (2 * dx)))
+
+
+# --------------------------------------------------------------------------- operators
+def laplacian(u, w, alpha, g):
+    """Reference ``pm/FDM/kernel/laplacian.py:34-59``; u is (X, Y, Z, C)."""
+    dt = u.dtype.type
+    shape3 = u.shape[:3]
+    out = np.full_like(u, np.nan)
+    acc = np.zeros_like(u[interior(shape3, g)])
+    for a in eligible_axes(shape3):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * u[interior(shape3, g, a, s)]
+    out[interior(shape3, g)] = dt(alpha) * acc
+    return out
+
+
+def grad(u, w, alpha, g, force_matrix=False):
+    """Reference ``pm/FDM/kernel/grad.py:78-100``.  u is (X, Y, Z, N); returns
+    (X, Y, Z, D) when N == 1 and not force_matrix, else (X, Y, Z, N, D)."""
+    dt = u.dtype.type
+    shape3 = u.shape[:3]
+    axes = eligible_axes(shape3)
+    N, D = u.shape[3], len(axes)
+    out = np.full(shape3 + (N, D), np.nan, dtype=u.dtype)
+    inner = interior(shape3, g)
+    for t, a in enumerate(axes):
+        acc = np.zeros_like(u[inner])
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * u[interior(shape3, g, a, s)]
+        out[inner + (slice(None), t)] = dt(alpha) * acc
+    if N == 1 and not force_matrix:
+        return out.reshape(shape3 + (D,))
+    return out
+
+
+def divergence(u, w, alpha, g, matrix_in=False):
+    """Reference ``pm/FDM/kernel/divergence.py:85-106``.  Vector input
+    (X, Y, Z, D) -> (X, Y, Z, 1); matrix input (X, Y, Z, N, D) -> (X, Y, Z, N)."""
+    dt = u.dtype.type
+    shape3 = u.shape[:3]
+    axes = eligible_axes(shape3)
+    m = u if matrix_in else u.reshape(shape3 + (1, u.shape[3]))
+    rows = m.shape[3]
+    out = np.full(shape3 + (rows,), np.nan, dtype=u.dtype)
+    inner = interior(shape3, g)
+    acc = np.zeros(m[inner].shape[:-1], dtype=u.dtype)
+    for t, a in enumerate(axes):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * m[interior(shape3, g, a, s) + (slice(None), t)]
+    out[inner] = dt(alpha) * acc
+    return out
+
+
+def forward_euler(cur, k, dt_step):
+    """Reference ``pm/time_step/forwardEuler.py:79``: cur + dt * k everywhere."""
+    return cur + cur.dtype.type(dt_step) * k
+
+
+# --------------------------------------------------------------------------- boundary groups
+def no_ghost_shape(shape3, g):
+    """Reference ``ExplicitUniformGridStencil.py:134-148``."""
+    return tuple(p - 2 * g if p > 1 else p for p in shape3)
+
+
+def legacy_boundary_layout(shape3, g):
+    """Boundary node list, named groups and interior directions of the legacy
+    ``GridBoundary`` (reference ``pm/deprecated/gridBoundary.py:60-116``).
+
+    Returns (ijk (nb, 3) int32 sorted lexicographically, groups dict name -> ids,
+    interior (nb, 3) int32)."""
+    ng = no_ghost_shape(shape3, g)
+    parts = []
+    for i in range(3):
+        if ng[i] == 1:
+            continue
+        for lim in (0, ng[i] - 1):
+            side = list(ng)
+            side[i] = 1
+            idx = np.moveaxis(np.indices(side, dtype=np.int32), 0, -1).reshape(-1, 3)
+            idx[:, i] = lim
+            for ax in range(3):
+                if ng[ax] != 1:
+                    idx[:, ax] += g
+            parts.append(idx)
+    ijk = np.unique(np.concatenate(parts, axis=0), axis=0).astype(np.int32)
+    ids = np.arange(len(ijk), dtype=np.int32)
+    groups = {"ALL": ids}
+    for axis, name in enumerate("XYZ"):
+        if ng[axis] == 1:
+            continue
+        for lim, parity in ((g, "-"), (ng[axis] - 1 + g, "+")):
+            groups[parity + name] = ids[ijk[:, axis] == lim]
+    inter = np.zeros((len(ijk), 3), dtype=np.int32)
+    for axis, name in enumerate("XYZ"):
+        if ng[axis] == 1:
+            continue
+        for parity, sign in (("-", 1), ("+", -1)):
+            inter[groups[parity + name], axis] = sign
+    return ijk, groups, inter
+
+
+def functional_groups(shape3, g):
+    """Named face groups of the functional ``GridBoundary`` as flat C-order node
+    ids (reference ``pm/FDM/boundary/functional_boundary.py:111-137``)."""
+    flat = np.arange(int(np.prod(shape3)), dtype=np.int32).reshape(shape3)
+    groups, parts = {}, []
+    for i, name in enumerate("XYZ"):
+        if shape3[i] <= 1:
+            continue
+        for prefix, sign in (("-", -1), ("+", 1)):
+            sl = [slice(None)] * 3
+            sl[i] = g if sign == -1 else -(g + 1)
+            for x, p in enumerate(shape3):
+                if p > 1 and x != i:
+                    sl[x] = slice(g, -g) if g > 0 else slice(None)
+            groups[prefix + name] = flat[tuple(sl)].ravel()
+            parts.append(groups[prefix + name])
+    groups["ALL"] = np.unique(np.concatenate(parts)).astype(np.int32)
+    return groups
+
+
+# --------------------------------------------------------------------------- boundary apply
+def bc_legacy(u, ijk, inter, btype, bvalue, dx):
+    """Legacy table apply (reference ``pm/deprecated/gridBoundary.py:267-302``).
+    btype/bvalue are (nb, C).  Writes of one launch never collide, so applying
+    them vectorised per axis is the same as the reference's threads."""
+    dt = u.dtype.type
+    X, Y, Z, C = u.shape
+    sn = np.array([Y * Z, Z, 1], dtype=np.int64)
+    out = u.copy()
+    fo, fi = out.reshape(-1, C), u.reshape(-1, C)
+    node = ijk.astype(np.int64) @ sn
+    for a in range(3):
+        b = np.nonzero(inter[:, a] != 0)[0]
+        s = inter[b, a].astype(np.int64)
+        ghost, adj = node[b] - s * sn[a], node[b] + s * sn[a]
+        k = np.where(s > 0, dt(-2.0), dt(2.0)).astype(u.dtype) * dt(dx)
+        for c in range(C):
+            ty, v = btype[b, c], bvalue[b, c].astype(u.dtype)
+            d = ty == 1
+            fo[node[b][d], c] = v[d]
+            fo[ghost[d], c] = dt(2.0) * v[d] - fi[adj[d], c]
+            n = ty == 2
+            fo[ghost[n], c] = k[n] * v[n] + fi[adj[n], c]
+    return out
+
+
+def bc_functional(u, g, groups, dx, t=0.0):
+    """Functional apply (reference ``pm/FDM/boundary/functional_boundary.py:260-400``).
+
+    ``groups`` is a list of dicts with keys ids (flat node ids), type (1 Dirichlet,
+    2 Neumann), comps (selected components), const, tmax (None for constant)."""
+    dt = u.dtype.type
+    X, Y, Z, C = u.shape
+    shape3 = (X, Y, Z)
+    sn = np.array([Y * Z, Z, 1], dtype=np.int64)
+    out = u.copy()
+    fo, fi = out.reshape(-1, C), u.reshape(-1, C)
+    for grp in groups:
+        ids = np.asarray(grp["ids"], dtype=np.int64)
+        comps = list(grp["comps"])
+        v = dt(grp["const"])
+        if grp.get("tmax") is not None:
+            ramp = dt(t) / dt(grp["tmax"])
+            ramp = min(ramp, dt(1.0))
+            v = dt(ramp) * dt(grp["const"])
+        val = fi[ids].copy()
+        val[:, comps] = v
+        if grp["type"] == 1:
+            fo[ids] = val
+        idx = np.stack([ids // (Y * Z), (ids // Z) % Y, ids % Z], axis=1)
+        for a in eligible_axes(shape3):
+            s = np.where(idx[:, a] == g, 1, np.where(idx[:, a] == shape3[a] - 1 - g, -1, 0))
+            sel = s != 0
+            q, sq = ids[sel], s[sel]
+            ghost, adj = q - sq * sn[a], q + sq * sn[a]
+            k = np.where(sq > 0, dt(-2.0), dt(2.0)).astype(u.dtype) * dt(dx)
+            for c in comps:
+                vc = val[sel, c]
+                if grp["type"] == 1:
+                    fo[ghost, c] = dt(2.0) * vc - fi[adj, c]
+                else:
+                    fo[ghost, c] = k * vc + fi[adj, c]
+    return out
+
+
+def periodic_fill(u, g):
+    """In-place periodic ghost fill, axis by axis (no counterpart in the
+    reference)."""
+    shape3 = u.shape[:3]
+    for a in eligible_axes(shape3):
+        E = shape3[a] - 2 * g
+        for i in range(g):
+            dst = [slice(None)] * 3
+            src = [slice(None)] * 3
+            dst[a], src[a] = i, i + E
+            u[tuple(dst)] = u[tuple(src)]
+            dst[a], src[a] = shape3[a] - g + i, g + i
+            u[tuple(dst)] = u[tuple(src)]
+    return u
+
+
+# --------------------------------------------------------------------------- composed steps
+def heat_step(u, w, alpha, g, dt_step, bc):
+    """One step of the reference heat loop (``examples/2D_Heat_Equation.py:58-62``):
+    u2 = BC(u); L = lap(u2, alpha); u_next = u2 + dt * L.  ``bc`` maps u -> u2.
+    Ghosts of u_next are NaN (the reference leaves them uninitialised)."""
+    u2 = bc(u)
+    lap = laplacian(u2, w, alpha, g)
+    return forward_euler(u2, lap, dt_step)
+
+
+def wave_step(u, v, w, c2, g, dt_step, bc, first):
+    """One step of the reference u/v wave loop (``examples/wave2d/standingWave.py:72-76``)
+    with the aliasing it relies on: from the second step on v is updated in place
+    before u reads it (symplectic Euler); the first step reads the old v.
+    ``c2`` is a scalar or a per-node (X, Y, Z, 1) array multiplying the Laplacian."""
+    dt = u.dtype.type
+    u2 = bc(u)
+    lap = laplacian(u2, w, dt(1.0) if np.ndim(c2) else c2, g)
+    if np.ndim(c2):
+        lap = c2 * lap
+    v_next = forward_euler(v, lap, dt_step)
+    u_next = forward_euler(u2, v if first else v_next, dt_step)
+    return u_next, v_next

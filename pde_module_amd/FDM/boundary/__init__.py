@@ -1,0 +1,7 @@
+from .flags import DIRICHLET, VON_NEUMANN, NO_BC, Boundary_Types
+from .boundary_functions import get_constant_func, get_ramp_func
+from .functional_boundary import GridBoundary
+from .periodic import PeriodicBoundary
+
+__all__ = ["GridBoundary", "PeriodicBoundary", "DIRICHLET", "VON_NEUMANN", "NO_BC",
+           "Boundary_Types", "get_constant_func", "get_ramp_func"]

@@ -1,0 +1,112 @@
+"""Table-driven boundary base used by the legacy GridBoundary (reference
+``pm/FDM/boundary/boundary.py:11-222``).
+
+Every boundary (node, component) entry has a type and a value.  Types start as
+DIRICHLET and values as 0 (``boundary.py:59-65``), so an entry never set is a
+Dirichlet-0 boundary.  ``set_BC`` overwrites entries of a group or of explicit
+entry ids.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ...utils import get_unique_key
+from ..module.explicitUniformGridStencil import ExplicitUniformGridStencil
+from .flags import Boundary_Types
+
+
+class Boundary(ExplicitUniformGridStencil):
+    boundary_type: np.ndarray
+    boundary_value: np.ndarray
+
+    def __init__(self, field: torch.Tensor, dx: float, ghost_cells: int) -> None:
+        if field.dim() != 4:
+            raise AssertionError("input must be vector type")
+        inputs = int(field.shape[3])
+        super().__init__(inputs, inputs, dx)
+        if not (type(ghost_cells) is int and ghost_cells > 0):
+            raise AssertionError("ghost_cells must be a positive int")
+        self.ghost_cells = ghost_cells
+        self.grid_shape = tuple(int(s) for s in field.shape[:3])
+        self.grid_shape_without_ghost = self.grid_shape_with_no_ghost_cells(self.grid_shape,
+                                                                            ghost_cells)
+        self.dimension = self.calculate_dimension_from_grid_shape(self.grid_shape)
+        self.groups: dict[str, np.ndarray] = {}
+        self.func_groups: dict = {}
+
+    def define_boundary_value_and_type_arrays(self, indices: np.ndarray) -> None:
+        shape = (len(indices),) + self.input_dtype_shape
+        self.boundary_ids = np.arange(len(indices))
+        self.boundary_value = np.zeros(shape, dtype=np.float32)
+        self.boundary_type = np.ones_like(self.boundary_value, dtype=np.int8)
+        self.groups["ALL"] = self.boundary_ids
+
+    def _check_output_ids(self, output_ids):
+        if output_ids is None:
+            return slice(None)
+        if isinstance(output_ids, (int, np.integer)):
+            if not 0 <= output_ids < self.inputs[0]:
+                raise AssertionError("output id out of range")
+            return int(output_ids)
+        if isinstance(output_ids, (list, tuple, np.ndarray)):
+            output_ids = np.array(output_ids, dtype=np.int32)
+            if not np.all((0 <= output_ids) & (output_ids < self.inputs[0])):
+                raise AssertionError("output ids out of range")
+            return output_ids
+        raise TypeError(
+            f"Valid Types are: int|np.ndarray|list|tuple|None got {type(output_ids)} instead")
+
+    def set_BC(self, face_ids, value: float, boundary_type: int, outputs_ids) -> None:
+        if isinstance(face_ids, str):
+            if face_ids not in self.groups:
+                raise AssertionError(f"unknown group {face_ids!r}")
+            face_ids = self.groups[face_ids]
+        else:
+            if not isinstance(face_ids, (np.ndarray, list, tuple, int)):
+                raise AssertionError("face ids must be a group name or entry ids")
+            face_ids = np.array(face_ids, dtype=np.int32)
+        outputs_ids = self._check_output_ids(outputs_ids)
+        if not isinstance(value, float):
+            raise AssertionError("legacy boundary values must be float")
+        # NumPy indexing exactly as the reference (``boundary.py:111-113``): an
+        # id array with a component array pairs them element-wise.
+        self.boundary_type[face_ids, outputs_ids] = boundary_type
+        self.boundary_value[face_ids, outputs_ids] = value
+        self.table = None
+
+    def set_func_BC(self, face_ids, func, boundary_type, outputs_id) -> None:
+        raise NotImplementedError(
+            "per-node boundary functions are not supported by the GPU boundary tables; "
+            "use constant values (legacy) or get_ramp_func with the functional GridBoundary")
+
+    def dirichlet_BC(self, group, value, outputs_ids=None) -> None:
+        if isinstance(value, (float, int)) and not isinstance(value, bool):
+            self.set_BC(group, float(value), Boundary_Types.DIRICHLET, outputs_ids)
+        else:
+            self.set_func_BC(group, value, Boundary_Types.DIRICHLET, outputs_ids)
+
+    def vonNeumann_BC(self, group, value, outputs_ids=None) -> None:
+        self.set_BC(group, value, Boundary_Types.VON_NEUMANN, outputs_ids)
+
+    def no_slip(self, group) -> None:
+        if self.inputs[0] != self.dimension:
+            raise AssertionError(
+                "Valid only when input_dtype is vector with same length equal to dimension of field")
+        self.dirichlet_BC(group, 0.0)
+
+    def impermeable(self, group: str) -> None:
+        if self.inputs[0] != self.dimension:
+            raise AssertionError(
+                "Valid only when input_dtype is vector with same length equal to dimension of field")
+        if group not in self.groups or group not in {"-X", "+X", "-Y", "+Y", "-Z", "+Z"}:
+            raise AssertionError("{'-X','+X','-Y','+Y','-Z','+Z'} are valid groups")
+        axis = "XYZ".index(group[-1])
+        self.vonNeumann_BC(group, 0.0)
+        self.dirichlet_BC(group, 0.0, axis)
+
+    def check_boundary_types(self) -> bool:
+        return not np.any(self.boundary_type == Boundary_Types.NO_BC)
+
+
+__all__ = ["Boundary", "get_unique_key"]

@@ -1,0 +1,53 @@
+"""Device side of a resolved boundary: the op table uploaded once at setup,
+the 1-element time array and the staging buffer for in-place application."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ... import _native as N
+
+
+class DeviceBoundaryTable:
+    def __init__(self, ops: np.ndarray, device: torch.device):
+        self.nops = int(len(ops))
+        self.needs_t = bool(self.nops and np.any(ops["tmax"] > 0))
+        raw = np.ascontiguousarray(ops).view(np.uint8) if self.nops else np.zeros(32, np.uint8)
+        self.ops = torch.from_numpy(raw.copy()).to(device)
+        self.staging = torch.empty(max(self.nops, 1), dtype=torch.float32, device=device)
+        self.host_ops = ops
+        # Largest scalar index any op touches; checked against the field size at
+        # every call so a bad table raises on the host instead of faulting.
+        if self.nops:
+            src = ops["src"][ops["kind"] != N.BC_SET]
+            self.max_index = int(max(ops["dst"].max(), src.max() if len(src) else 0))
+            self.min_index = int(min(ops["dst"].min(), src.min() if len(src) else 0))
+        else:
+            self.max_index, self.min_index = -1, 0
+
+    def _check_bounds(self, numel: int) -> None:
+        if self.min_index < 0 or self.max_index >= numel:
+            raise IndexError(
+                f"boundary table touches scalar index {self.min_index}..{self.max_index} "
+                f"outside a field of {numel} values (node ids or shifted groups leave the grid)")
+
+    def apply(self, x: torch.Tensor, out: torch.Tensor, t: torch.Tensor | None) -> torch.Tensor:
+        """out = copy(x) with the boundary writes (reads all from x)."""
+        N.check_field(x, "input_array")
+        N.check_field(out, "output_array")
+        if out.shape != x.shape:
+            raise AssertionError("boundary output must match the input's shape")
+        self._check_bounds(x.numel())
+        N.call("pde_bc_apply_f32", N.dev_ptr(x), N.dev_ptr(out), x.numel(), self.ops.data_ptr(),
+               self.nops, t.data_ptr() if t is not None else None, N.stream_of(x))
+        return out
+
+    def apply_inplace(self, u: torch.Tensor, t: torch.Tensor | None) -> torch.Tensor:
+        """The same writes applied to u itself (values gathered from the
+        unmodified u first)."""
+        N.check_field(u, "field")
+        self._check_bounds(u.numel())
+        N.call("pde_bc_apply_inplace_f32", N.dev_ptr(u), self.ops.data_ptr(), self.nops,
+               t.data_ptr() if t is not None else None, self.staging.data_ptr(),
+               N.stream_of(u))
+        return u

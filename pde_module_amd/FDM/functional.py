@@ -1,0 +1,87 @@
+"""Functional tier: one launch over the interior of a padded field (reference
+``pm/FDM/functional/_laplacian.py:5-36``, ``_grad.py:5-36``,
+``_divergence.py:4-24``; same argument order, including divergence's
+``ghost_cells`` before ``output_array``).
+
+Each call launches asynchronously on torch's current stream of the input's
+device and returns ``output_array``.  ``device`` is accepted for signature
+compatibility; the launch device is the tensors' device.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from .kernel import StencilKernel
+
+
+def _grid(x: torch.Tensor):
+    if x.dim() < 4:
+        raise ValueError("fields are (X, Y, Z, C) or (X, Y, Z, N, D) tensors")
+    return tuple(int(s) for s in x.shape[:3])
+
+
+def _check(kernel: StencilKernel, x: torch.Tensor, out: torch.Tensor, op: str):
+    if kernel.op != op:
+        raise TypeError(f"expected a {op} kernel, got {kernel.op}")
+    N.check_field(x, "input_array")
+    N.check_field(out, "output_array")
+    if tuple(x.shape[3:]) != kernel.in_shape:
+        raise AssertionError(f"input element shape {tuple(x.shape[3:])} != {kernel.in_shape}")
+    if tuple(out.shape) != tuple(x.shape[:3]) + kernel.out_shape:
+        raise AssertionError(f"output shape {tuple(out.shape)} does not match the kernel")
+    if out.device != x.device:
+        raise ValueError("input and output must be on the same device")
+
+
+def laplacian(kernel, input_array, alpha, output_array, ghost_cells, device=None):
+    _check(kernel, input_array, output_array, "laplacian")
+    X, Y, Z = _grid(input_array)
+    wm, wc, wp = kernel.weights
+    N.call("pde_laplacian_f32", N.dev_ptr(input_array), N.dev_ptr(output_array), X, Y, Z,
+           int(kernel.in_shape[0]), int(ghost_cells), wm, wc, wp, np.float32(alpha),
+           N.stream_of(input_array))
+    return output_array
+
+
+def grad(kernel, input_array, alpha, output_array, ghost_cells, device=None):
+    _check(kernel, input_array, output_array, "grad")
+    X, Y, Z = _grid(input_array)
+    wm, wc, wp = kernel.weights
+    N.call("pde_grad_f32", N.dev_ptr(input_array), N.dev_ptr(output_array), X, Y, Z,
+           int(kernel.in_shape[0]), int(ghost_cells), wm, wc, wp, np.float32(alpha),
+           N.stream_of(input_array))
+    return output_array
+
+
+def divergence(kernel, input_array, alpha, ghost_cells, output_array, device=None):
+    _check(kernel, input_array, output_array, "divergence")
+    X, Y, Z = _grid(input_array)
+    wm, wc, wp = kernel.weights
+    rows = kernel.in_shape[0] if len(kernel.in_shape) == 2 else 1
+    N.call("pde_divergence_f32", N.dev_ptr(input_array), N.dev_ptr(output_array), X, Y, Z,
+           int(rows), int(ghost_cells), wm, wc, wp, np.float32(alpha), N.stream_of(input_array))
+    return output_array
+
+
+def fused_lap_grad_div(s, v, lap_out, grad_out, div_out, dx, ghost_cells,
+                       alpha_lap=1.0, alpha_grad=1.0, alpha_div=1.0):
+    """lap(s), grad(s) and div(v) of a scalar and a vec3 field on one 3D grid in
+    a single HBM pass; each output equals the corresponding module's."""
+    for name, t in (("s", s), ("v", v), ("lap", lap_out), ("grad", grad_out), ("div", div_out)):
+        N.check_field(t, name)
+    X, Y, Z = _grid(s)
+    g3 = (X, Y, Z)
+    if (tuple(s.shape) != g3 + (1,) or tuple(v.shape) != g3 + (3,) or
+            tuple(lap_out.shape) != g3 + (1,) or tuple(grad_out.shape) != g3 + (3,) or
+            tuple(div_out.shape) != g3 + (1,)):
+        raise AssertionError("fused pass needs s, lap, div: (X,Y,Z,1) and v, grad: (X,Y,Z,3)")
+    from .module.laplacian import laplacian_weights
+    from .module.grad import central_weights
+    wl, wc = laplacian_weights(dx), central_weights(dx)
+    N.call("pde_fused_lap_grad_div_f32", N.dev_ptr(s), N.dev_ptr(v), N.dev_ptr(lap_out),
+           N.dev_ptr(grad_out), N.dev_ptr(div_out), X, Y, Z, int(ghost_cells), *wl, *wc,
+           np.float32(alpha_lap), np.float32(alpha_grad), np.float32(alpha_div),
+           N.stream_of(s))
+    return lap_out, grad_out, div_out

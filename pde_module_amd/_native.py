@@ -1,0 +1,100 @@
+"""ctypes binding of libpdehip.so (declared in include/pde_hip.h).
+
+There is no CPU or PyTorch fallback: if the library is missing or no GPU is
+visible, the first operator call raises.  Build the library with
+``python -m pde_module_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpdehip.so")
+
+_P, _I64, _I32, _F = C.c_void_p, C.c_int64, C.c_int32, C.c_float
+
+# name -> argtypes, in the order of include/pde_hip.h
+SIGNATURES = {
+    "pde_laplacian_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_laplacian_generic_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_grad_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_divergence_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_fused_lap_grad_div_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32,
+                                   _F, _F, _F, _F, _F, _F, _F, _F, _F, _P],
+    "pde_forward_euler_f32": [_P, _P, _P, _I64, _F, _P],
+    "pde_copy_f32": [_P, _P, _I64, _P],
+    "pde_heat_step_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _F, _P],
+    "pde_wave_step_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
+                          _I32, _P],
+    "pde_bc_apply_f32": [_P, _P, _I64, _P, _I64, _P, _P],
+    "pde_bc_apply_inplace_f32": [_P, _P, _I64, _P, _P, _P],
+    "pde_periodic_fill_f32": [_P, _I64, _I64, _I64, _I32, _I32, _P],
+    "pde_version": [],
+    "pde_status_string": [C.c_int],
+}
+
+# Must match struct pde_bc_op in include/pde_hip.h (32 bytes).
+BC_OP_DTYPE = np.dtype([("dst", "<i8"), ("src", "<i8"), ("kind", "<i4"),
+                        ("value", "<f4"), ("coef", "<f4"), ("tmax", "<f4")])
+BC_SET, BC_COPY, BC_GHOST_D, BC_GHOST_N = 0, 1, 2, 3
+assert BC_OP_DTYPE.itemsize == 32
+
+
+class PDEError(RuntimeError):
+    """A libpdehip entry point returned a non-zero status."""
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and bind every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: the HIP kernels are not built "
+            "(run `python -m pde_module_amd.build`); there is no fallback path")
+    lib = C.CDLL(path)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = C.c_int
+    lib.pde_status_string.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point and raise PDEError on a non-zero status."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        msg = lib.pde_status_string(st).decode()
+        raise PDEError(f"{name} failed: {msg} (status {st})")
+
+
+def stream_of(t: torch.Tensor) -> int:
+    """Raw hipStream_t of torch's current stream on t's device."""
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def dev_ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def check_field(t: torch.Tensor, name: str = "field") -> None:
+    """Fields handed to the kernels: contiguous FP32 on a GPU."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must live on a GPU (got device {t.device}); "
+                           "the HIP kernels have no CPU path")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")

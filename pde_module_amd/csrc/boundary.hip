@@ -1,0 +1,118 @@
+// Ghost-cell boundary conditions from a host-resolved op table.
+//
+// The reference applies a GridBoundary as copy(in -> out) followed by one
+// launch per boundary group (pm/FDM/boundary/functional_boundary.py:278-291)
+// or one table launch over (boundary node, component)
+// (pm/deprecated/gridBoundary.py:231-242).  Every write reads only `in`, so
+// the final value of each written scalar is the last group's formula applied
+// to `in`; the host resolves that into one op per destination and a single
+// launch here reproduces either semantics exactly.
+#include "pde_common.hpp"
+
+namespace pde {
+
+__device__ __forceinline__ float op_value(const pde_bc_op &op, const float *t)
+{
+    if (op.tmax > 0.0f) {
+        // wp.min(t / T_max, 1) * const  (boundary_functions.py:85-89).  A null
+        // t is the reference's default time array, zeros(1)
+        // (functional_boundary.py:237).
+        const float tv = t ? t[0] : 0.0f;
+        float ramp = tv / op.tmax;
+        ramp = ramp < 1.0f ? ramp : 1.0f;
+        return ramp * op.value;
+    }
+    return op.value;
+}
+
+__device__ __forceinline__ float op_eval(const pde_bc_op &op, const float *__restrict__ in,
+                                         const float *t)
+{
+    const float v = op_value(op, t);
+    switch (op.kind) {
+    case PDE_BC_SET: return v;
+    case PDE_BC_COPY: return in[op.src];
+    case PDE_BC_GHOST_D: return 2.0f * v - in[op.src];
+    default: return op.coef * v + in[op.src];
+    }
+}
+
+__global__ __launch_bounds__(256) void bc_apply_kernel(const float *__restrict__ in,
+                                                       float *__restrict__ out,
+                                                       const pde_bc_op *__restrict__ ops,
+                                                       int64_t nops, const float *t)
+{
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= nops) return;
+    const pde_bc_op op = ops[m];
+    out[op.dst] = op_eval(op, in, t);
+}
+
+__global__ __launch_bounds__(256) void bc_gather_kernel(const float *__restrict__ u,
+                                                        const pde_bc_op *__restrict__ ops,
+                                                        int64_t nops, const float *t,
+                                                        float *__restrict__ staging)
+{
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= nops) return;
+    staging[m] = op_eval(ops[m], u, t);
+}
+
+__global__ __launch_bounds__(256) void bc_scatter_kernel(float *__restrict__ u,
+                                                         const pde_bc_op *__restrict__ ops,
+                                                         int64_t nops,
+                                                         const float *__restrict__ staging)
+{
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= nops) return;
+    u[ops[m].dst] = staging[m];
+}
+
+// Small tables (one workgroup's worth) gather and scatter in one launch with a
+// workgroup barrier between the phases.
+__global__ __launch_bounds__(1024) void bc_inplace_small_kernel(float *__restrict__ u,
+                                                                const pde_bc_op *__restrict__ ops,
+                                                                int64_t nops, const float *t)
+{
+    __shared__ float vals[4096];
+    for (int64_t m = threadIdx.x; m < nops; m += 1024) vals[m] = op_eval(ops[m], u, t);
+    __syncthreads();
+    for (int64_t m = threadIdx.x; m < nops; m += 1024) u[ops[m].dst] = vals[m];
+}
+
+}  // namespace pde
+
+using namespace pde;
+
+extern "C" int pde_copy_f32(const float *in, float *out, int64_t n, void *stream);
+
+extern "C" int pde_bc_apply_f32(const float *in, float *out, int64_t n, const pde_bc_op *ops,
+                                int64_t nops, const float *t, void *stream)
+{
+    if (!in || !out || n < 0 || nops < 0 || (nops > 0 && !ops) || in == out) return PDE_EINVAL;
+    int st = pde_copy_f32(in, out, n, stream);
+    if (st) return st;
+    if (nops == 0) return PDE_OK;
+    hipLaunchKernelGGL(bc_apply_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), in, out, ops, nops, t);
+    return launch_status();
+}
+
+extern "C" int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t nops,
+                                        const float *t, float *staging, void *stream)
+{
+    if (!u || nops < 0 || (nops > 0 && !ops)) return PDE_EINVAL;
+    if (nops == 0) return PDE_OK;
+    hipStream_t s = as_stream(stream);
+    if (nops <= 4096) {
+        hipLaunchKernelGGL(bc_inplace_small_kernel, dim3(1), dim3(1024), 0, s, u, ops, nops, t);
+        return launch_status();
+    }
+    if (!staging) return PDE_EINVAL;
+    const dim3 grid((unsigned)((nops + 255) / 256)), block(256);
+    hipLaunchKernelGGL(bc_gather_kernel, grid, block, 0, s, u, ops, nops, t, staging);
+    int st = launch_status();
+    if (st) return st;
+    hipLaunchKernelGGL(bc_scatter_kernel, grid, block, 0, s, u, ops, nops, staging);
+    return launch_status();
+}

@@ -1,0 +1,485 @@
+// Stencil operators: Laplacian, Grad, Divergence, the fused 3D operator pass
+// and the periodic ghost fill.
+//
+// Every kernel keeps the reference's accumulation order (acc starts at 0, axes
+// ascending, s = -1, 0, +1) and the library is compiled with
+// -ffp-contract=off, so each result is the same FP32 value a serial CPU
+// restatement of pm/FDM/kernel/{laplacian,grad,divergence}.py produces.
+#include "pde_common.hpp"
+
+namespace pde {
+
+// ---------------------------------------------------------------- Laplacian
+
+// Element geometry of one interior "row" family: a thread owns one element
+// r of the flattened (fastest axis x components) interior row; rows are
+// indexed by (i0, i1).  Strides are in floats.
+struct RowGeom {
+    int64_t base;  // float offset of the first interior element
+    int64_t R;     // interior elements per row (E[2] * comps)
+    int64_t E0, E1;
+    int64_t S0, S1, S2;  // float strides of canonical axes 0, 1, 2
+};
+
+inline RowGeom row_geom(const Geom &G, int64_t comps)
+{
+    RowGeom r;
+    r.S0 = G.S[0] * comps;
+    r.S1 = G.S[1] * comps;
+    r.S2 = comps;
+    r.base = G.G[0] * r.S0 + G.G[1] * r.S1 + G.G[2] * r.S2;
+    r.R = G.E[2] * comps;
+    r.E0 = G.E[0];
+    r.E1 = G.E[1];
+    return r;
+}
+
+// Generic Laplacian: any ncomp, 0..3 eligible axes.  Canonical axis 0 is a
+// grid axis only for NDIM == 3, axis 1 for NDIM >= 2, axis 2 for NDIM >= 1.
+template <int NDIM>
+__global__ __launch_bounds__(256) void lap_generic(const float *__restrict__ in,
+                                                   float *__restrict__ out, RowGeom g,
+                                                   float wm, float wc, float wp, float alpha)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= g.R) return;
+    for (int64_t i0 = blockIdx.z; i0 < g.E0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < g.E1; i1 += gridDim.y) {
+            const int64_t e = g.base + i0 * g.S0 + i1 * g.S1 + r;
+            const float c = in[e];
+            float acc = 0.0f;
+            if (NDIM >= 3) {
+                acc += wm * in[e - g.S0];
+                acc += wc * c;
+                acc += wp * in[e + g.S0];
+            }
+            if (NDIM >= 2) {
+                acc += wm * in[e - g.S1];
+                acc += wc * c;
+                acc += wp * in[e + g.S1];
+            }
+            if (NDIM >= 1) {
+                acc += wm * in[e - g.S2];
+                acc += wc * c;
+                acc += wp * in[e + g.S2];
+            }
+            out[e] = alpha * acc;
+        }
+}
+
+// 3D scalar Laplacian, x-marching.  A block owns a 64 (z) x BY (y) column
+// tile and walks NXC planes along the slowest axis, keeping the x-1 / x / x+1
+// centre values in registers so each plane is fetched from HBM once; y and z
+// neighbours of the current plane come from the loads of the adjacent lanes
+// and rows (L1/L2 hits).  Tiles are numbered z-fastest and XCD-remapped so
+// neighbouring tiles share an L2.
+struct March3 {
+    int64_t SX, SY;        // plane and row pitch (floats)
+    int64_t E0, E1, E2;    // interior extents
+    int64_t g;
+    uint32_t tz, ty;       // tile counts along z and y
+};
+
+template <int BY, int NXC>
+__global__ __launch_bounds__(64 * BY) void lap3d_march(const float *__restrict__ in,
+                                                      float *__restrict__ out, March3 m,
+                                                      float wm, float wc, float wp,
+                                                      float alpha)
+{
+    uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t zt = b % m.tz;
+    b /= m.tz;
+    const uint32_t yt = b % m.ty;
+    const uint32_t xt = b / m.ty;
+    const int64_t k = (int64_t)zt * 64 + threadIdx.x;
+    const int64_t j = (int64_t)yt * BY + threadIdx.y;
+    if (k >= m.E2 || j >= m.E1) return;
+    const int64_t x0 = (int64_t)xt * NXC;
+    int64_t nx = m.E0 - x0;
+    if (nx > NXC) nx = NXC;
+    int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k + m.g);
+    float xm = in[e - m.SX];
+    float xc = in[e];
+#pragma unroll 4
+    for (int64_t x = 0; x < nx; ++x, e += m.SX) {
+        const float xp = in[e + m.SX];
+        const float ym = in[e - m.SY];
+        const float yp = in[e + m.SY];
+        const float zm = in[e - 1];
+        const float zp = in[e + 1];
+        float acc = 0.0f;
+        acc += wm * xm;
+        acc += wc * xc;
+        acc += wp * xp;
+        acc += wm * ym;
+        acc += wc * xc;
+        acc += wp * yp;
+        acc += wm * zm;
+        acc += wc * xc;
+        acc += wp * zp;
+        out[e] = alpha * acc;
+        xm = xc;
+        xc = xp;
+    }
+}
+
+// ------------------------------------------------------------------ Grad
+
+// Thread per (node, component r) of an interior row; writes the D slots of
+// row r of the node's N x D output matrix.
+template <int NDIM>
+__global__ __launch_bounds__(256) void grad_generic(const float *__restrict__ in,
+                                                    float *__restrict__ out, RowGeom g,
+                                                    float wm, float wc, float wp, float alpha)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= g.R) return;
+    for (int64_t i0 = blockIdx.z; i0 < g.E0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < g.E1; i1 += gridDim.y) {
+            const int64_t e = g.base + i0 * g.S0 + i1 * g.S1 + r;
+            const float c = in[e];
+            float *o = out + e * NDIM;
+            int t = 0;
+            if (NDIM >= 3) {
+                float acc = 0.0f;
+                acc += wm * in[e - g.S0];
+                acc += wc * c;
+                acc += wp * in[e + g.S0];
+                o[t++] = alpha * acc;
+            }
+            if (NDIM >= 2) {
+                float acc = 0.0f;
+                acc += wm * in[e - g.S1];
+                acc += wc * c;
+                acc += wp * in[e + g.S1];
+                o[t++] = alpha * acc;
+            }
+            if (NDIM >= 1) {
+                float acc = 0.0f;
+                acc += wm * in[e - g.S2];
+                acc += wc * c;
+                acc += wp * in[e + g.S2];
+                o[t] = alpha * acc;
+            }
+        }
+}
+
+// ------------------------------------------------------------- Divergence
+
+// Thread per (node, row r).  Input element (node, r, t) sits at float
+// (node*rows + r)*D + t; `g` is the row geometry of the *output* (comps =
+// rows), so the input offset of output element e is e*D (+ t).
+template <int NDIM>
+__global__ __launch_bounds__(256) void div_generic(const float *__restrict__ in,
+                                                   float *__restrict__ out, RowGeom g,
+                                                   float wm, float wc, float wp, float alpha)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= g.R) return;
+    const int64_t D = NDIM;
+    for (int64_t i0 = blockIdx.z; i0 < g.E0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < g.E1; i1 += gridDim.y) {
+            const int64_t e = g.base + i0 * g.S0 + i1 * g.S1 + r;
+            const float *p = in + e * D;
+            float acc = 0.0f;
+            int t = 0;
+            if (NDIM >= 3) {
+                acc += wm * p[t - g.S0 * D];
+                acc += wc * p[t];
+                acc += wp * p[t + g.S0 * D];
+                ++t;
+            }
+            if (NDIM >= 2) {
+                acc += wm * p[t - g.S1 * D];
+                acc += wc * p[t];
+                acc += wp * p[t + g.S1 * D];
+                ++t;
+            }
+            if (NDIM >= 1) {
+                acc += wm * p[t - g.S2 * D];
+                acc += wc * p[t];
+                acc += wp * p[t + g.S2 * D];
+            }
+            out[e] = alpha * acc;
+        }
+}
+
+// ------------------------------------------------------- fused lap/grad/div
+
+// One x-marching pass over a scalar s and a vec3 v: lap(s), grad(s), div(v).
+// s's x-1 / x / x+1 and v.x's x-1 / x+1 ride in registers.
+template <int BY, int NXC>
+__global__ __launch_bounds__(64 * BY) void fused3d_march(
+    const float *__restrict__ s, const float *__restrict__ v, float *__restrict__ lap,
+    float *__restrict__ grad, float *__restrict__ div, March3 m, float lm, float lc, float lp,
+    float cm, float cc, float cp, float al, float ag, float ad)
+{
+    uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t zt = b % m.tz;
+    b /= m.tz;
+    const uint32_t yt = b % m.ty;
+    const uint32_t xt = b / m.ty;
+    const int64_t k = (int64_t)zt * 64 + threadIdx.x;
+    const int64_t j = (int64_t)yt * BY + threadIdx.y;
+    if (k >= m.E2 || j >= m.E1) return;
+    const int64_t x0 = (int64_t)xt * NXC;
+    int64_t nx = m.E0 - x0;
+    if (nx > NXC) nx = NXC;
+    int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k + m.g);
+    float sm = s[e - m.SX], sc = s[e];
+    float vxm = v[3 * (e - m.SX)], vxc = v[3 * e];
+#pragma unroll 2
+    for (int64_t x = 0; x < nx; ++x, e += m.SX) {
+        const float sp = s[e + m.SX];
+        const float sym = s[e - m.SY], syp = s[e + m.SY];
+        const float szm = s[e - 1], szp = s[e + 1];
+        const float vxp = v[3 * (e + m.SX)];
+        const float vym = v[3 * (e - m.SY) + 1], vyc = v[3 * e + 1], vyp = v[3 * (e + m.SY) + 1];
+        const float vzm = v[3 * (e - 1) + 2], vzc = v[3 * e + 2], vzp = v[3 * (e + 1) + 2];
+        float acc = 0.0f;
+        acc += lm * sm;
+        acc += lc * sc;
+        acc += lp * sp;
+        acc += lm * sym;
+        acc += lc * sc;
+        acc += lp * syp;
+        acc += lm * szm;
+        acc += lc * sc;
+        acc += lp * szp;
+        lap[e] = al * acc;
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        gx += cm * sm;
+        gx += cc * sc;
+        gx += cp * sp;
+        gy += cm * sym;
+        gy += cc * sc;
+        gy += cp * syp;
+        gz += cm * szm;
+        gz += cc * sc;
+        gz += cp * szp;
+        grad[3 * e] = ag * gx;
+        grad[3 * e + 1] = ag * gy;
+        grad[3 * e + 2] = ag * gz;
+        float d = 0.0f;
+        d += cm * vxm;
+        d += cc * vxc;
+        d += cp * vxp;
+        d += cm * vym;
+        d += cc * vyc;
+        d += cp * vyp;
+        d += cm * vzm;
+        d += cc * vzc;
+        d += cp * vzp;
+        div[e] = ad * d;
+        sm = sc;
+        sc = sp;
+        vxm = vxc;
+        vxc = vxp;
+    }
+}
+
+// --------------------------------------------------------- periodic fill
+
+// Fills the 2g ghost layers of canonical axis A from the opposite interior
+// layers, over the full extent of the other two axes.  Thread per
+// (ghost-slab node, component).
+template <int A>
+__global__ __launch_bounds__(256) void periodic_axis(float *__restrict__ u, Geom G, int g,
+                                                     int C, int64_t count)
+{
+    const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (tid >= count) return;
+    const int c = (int)(tid % C);
+    int64_t q = tid / C;
+    // slab coordinates: the ghost index along A (0..2g), then the other two
+    // axes in ascending order.
+    int64_t idx[3];
+    const int64_t n_other1 = G.P[A == 2 ? 1 : 2];
+    const int o0 = (A == 0) ? 1 : 0;
+    const int o1 = (A == 2) ? 1 : 2;
+    idx[o1] = q % n_other1;
+    q /= n_other1;
+    idx[o0] = q % G.P[o0];
+    q /= G.P[o0];
+    const int64_t l = q;  // 0 .. 2g-1
+    const int64_t E = G.P[A] - 2 * (int64_t)g;
+    int64_t dst_a, src_a;
+    if (l < g) {
+        dst_a = l;
+        src_a = l + E;
+    } else {
+        dst_a = G.P[A] - 2 * (int64_t)g + l;  // P - g + (l - g)
+        src_a = dst_a - E;
+    }
+    idx[A] = dst_a;
+    const int64_t dn = idx[0] * G.S[0] + idx[1] * G.S[1] + idx[2];
+    idx[A] = src_a;
+    const int64_t sn = idx[0] * G.S[0] + idx[1] * G.S[1] + idx[2];
+    u[dn * C + c] = u[sn * C + c];
+}
+
+// ------------------------------------------------------------ dispatchers
+
+static dim3 row_grid(const RowGeom &rg)
+{
+    return dim3(clamp_grid((rg.R + 255) / 256, 0x7fffffff), clamp_grid(rg.E1, 65535),
+                clamp_grid(rg.E0, 65535));
+}
+
+}  // namespace pde
+
+using namespace pde;
+
+extern "C" int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                                 int64_t nz, int32_t ncomp, int32_t ghost, float wm,
+                                 float wc, float wp, float alpha, void *stream)
+{
+    if (!in || !out || ncomp < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    hipStream_t s = as_stream(stream);
+    if (G.ndim == 3 && ncomp == 1) {
+        constexpr int BY = 4, NXC = 64;
+        March3 m;
+        m.SX = G.S[0];
+        m.SY = G.S[1];
+        m.E0 = G.E[0];
+        m.E1 = G.E[1];
+        m.E2 = G.E[2];
+        m.g = ghost;
+        m.tz = (uint32_t)((G.E[2] + 63) / 64);
+        m.ty = (uint32_t)((G.E[1] + BY - 1) / BY);
+        const int64_t tx = (G.E[0] + NXC - 1) / NXC;
+        const int64_t nb = (int64_t)m.tz * m.ty * tx;
+        if (nb > 0x7fffffff) return PDE_ESHAPE;
+        hipLaunchKernelGGL((lap3d_march<BY, NXC>), dim3((unsigned)nb), dim3(64, BY), 0, s, in,
+                           out, m, wm, wc, wp, alpha);
+        return launch_status();
+    }
+    RowGeom rg = row_geom(G, ncomp);
+    dim3 grid = row_grid(rg), block(256);
+    switch (G.ndim) {
+    case 0: hipLaunchKernelGGL(lap_generic<0>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 1: hipLaunchKernelGGL(lap_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 2: hipLaunchKernelGGL(lap_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    default: hipLaunchKernelGGL(lap_generic<3>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    }
+    return launch_status();
+}
+
+// Generic (non-marching) Laplacian, exported for tests and benchmarks that
+// compare the marching kernel with the straightforward one.
+extern "C" int pde_laplacian_generic_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                                         int64_t nz, int32_t ncomp, int32_t ghost, float wm,
+                                         float wc, float wp, float alpha, void *stream)
+{
+    if (!in || !out || ncomp < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    hipStream_t s = as_stream(stream);
+    RowGeom rg = row_geom(G, ncomp);
+    dim3 grid = row_grid(rg), block(256);
+    switch (G.ndim) {
+    case 0: hipLaunchKernelGGL(lap_generic<0>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 1: hipLaunchKernelGGL(lap_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 2: hipLaunchKernelGGL(lap_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    default: hipLaunchKernelGGL(lap_generic<3>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    }
+    return launch_status();
+}
+
+extern "C" int pde_grad_f32(const float *in, float *out, int64_t nx, int64_t ny, int64_t nz,
+                            int32_t ncomp, int32_t ghost, float wm, float wc, float wp,
+                            float alpha, void *stream)
+{
+    if (!in || !out || ncomp < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    hipStream_t s = as_stream(stream);
+    RowGeom rg = row_geom(G, ncomp);
+    dim3 grid = row_grid(rg), block(256);
+    switch (G.ndim) {
+    case 1: hipLaunchKernelGGL(grad_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 2: hipLaunchKernelGGL(grad_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    default: hipLaunchKernelGGL(grad_generic<3>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    }
+    return launch_status();
+}
+
+extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                                  int64_t nz, int32_t nrows, int32_t ghost, float wm,
+                                  float wc, float wp, float alpha, void *stream)
+{
+    if (!in || !out || nrows < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    hipStream_t s = as_stream(stream);
+    RowGeom rg = row_geom(G, nrows);
+    dim3 grid = row_grid(rg), block(256);
+    switch (G.ndim) {
+    case 1: hipLaunchKernelGGL(div_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    case 2: hipLaunchKernelGGL(div_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    default: hipLaunchKernelGGL(div_generic<3>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    }
+    return launch_status();
+}
+
+extern "C" int pde_fused_lap_grad_div_f32(const float *s_, const float *v, float *lap,
+                                          float *grad, float *div, int64_t nx, int64_t ny,
+                                          int64_t nz, int32_t ghost, float lm, float lc,
+                                          float lp, float cm, float cc, float cp, float al,
+                                          float ag, float ad, void *stream)
+{
+    if (!s_ || !v || !lap || !grad || !div) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 3) return PDE_EINVAL;
+    constexpr int BY = 4, NXC = 64;
+    March3 m;
+    m.SX = G.S[0];
+    m.SY = G.S[1];
+    m.E0 = G.E[0];
+    m.E1 = G.E[1];
+    m.E2 = G.E[2];
+    m.g = ghost;
+    m.tz = (uint32_t)((G.E[2] + 63) / 64);
+    m.ty = (uint32_t)((G.E[1] + BY - 1) / BY);
+    const int64_t tx = (G.E[0] + NXC - 1) / NXC;
+    const int64_t nb = (int64_t)m.tz * m.ty * tx;
+    if (nb > 0x7fffffff) return PDE_ESHAPE;
+    hipLaunchKernelGGL((fused3d_march<BY, NXC>), dim3((unsigned)nb), dim3(64, BY), 0,
+                       as_stream(stream), s_, v, lap, grad, div, m, lm, lc, lp, cm, cc, cp, al,
+                       ag, ad);
+    return launch_status();
+}
+
+extern "C" int pde_periodic_fill_f32(float *u, int64_t nx, int64_t ny, int64_t nz,
+                                     int32_t ncomp, int32_t ghost, void *stream)
+{
+    if (!u || ncomp < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    for (int A = 3 - G.ndim; A < 3; ++A)
+        if (G.E[A] < ghost) return PDE_ESHAPE;  // a ghost's source must be interior
+    hipStream_t s = as_stream(stream);
+    for (int A = 3 - G.ndim; A < 3; ++A) {
+        int64_t other = G.nodes / G.P[A];
+        int64_t count = 2 * (int64_t)ghost * other * ncomp;
+        if (count == 0) continue;
+        dim3 grid((unsigned)((count + 255) / 256)), block(256);
+        if (A == 0) hipLaunchKernelGGL(periodic_axis<0>, grid, block, 0, s, u, G, ghost, ncomp, count);
+        else if (A == 1) hipLaunchKernelGGL(periodic_axis<1>, grid, block, 0, s, u, G, ghost, ncomp, count);
+        else hipLaunchKernelGGL(periodic_axis<2>, grid, block, 0, s, u, G, ghost, ncomp, count);
+        st = launch_status();
+        if (st) return st;
+    }
+    return PDE_OK;
+}

@@ -1,0 +1,76 @@
+// Shared host/device helpers for libpdehip.so (gfx950).
+//
+// Canonical geometry: axes of size 1 are not grid axes, and because their
+// extent is 1 dropping them leaves every flat index unchanged.  The host
+// therefore folds (nx, ny, nz) into up to three *eligible* extents P[0..2],
+// right-aligned (P[0] = 1 for a 2D grid, P[0] = P[1] = 1 for 1D), so one
+// kernel family covers 1D/2D/3D: canonical axis 2 is always the fastest
+// (node stride 1), axis 1 has node stride P[2], axis 0 node stride P[1]*P[2].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pde_hip.h"
+
+namespace pde {
+
+struct Geom {
+    int ndim;       // number of eligible axes (0..3)
+    int64_t P[3];   // canonical padded extents, right-aligned
+    int64_t E[3];   // interior extents (P - 2g on eligible axes, 1 otherwise)
+    int64_t G[3];   // interior start (g on eligible axes, 0 otherwise)
+    int64_t S[3];   // node strides (S[2] = 1)
+    int64_t nodes;  // P[0]*P[1]*P[2]
+};
+
+// Returns PDE_OK, PDE_EINVAL or PDE_ESHAPE.
+inline int make_geom(int64_t nx, int64_t ny, int64_t nz, int g, Geom *G)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || g < 0) return PDE_EINVAL;
+    int64_t in[3] = {nx, ny, nz};
+    int64_t el[3];
+    int nd = 0;
+    for (int a = 0; a < 3; ++a)
+        if (in[a] > 1) el[nd++] = in[a];
+    G->ndim = nd;
+    for (int a = 0; a < 3; ++a) {
+        int src = a - (3 - nd);
+        G->P[a] = src >= 0 ? el[src] : 1;
+        bool e = src >= 0;
+        G->G[a] = e ? g : 0;
+        G->E[a] = e ? G->P[a] - 2 * (int64_t)g : 1;
+        if (e && (g < 1 || G->E[a] < 1)) return PDE_ESHAPE;
+    }
+    G->S[2] = 1;
+    G->S[1] = G->P[2];
+    G->S[0] = G->P[1] * G->P[2];
+    G->nodes = G->P[0] * G->P[1] * G->P[2];
+    return PDE_OK;
+}
+
+inline int launch_status()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PDE_OK : (int)e;
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned clamp_grid(int64_t n, int64_t cap)
+{
+    return (unsigned)(n < 1 ? 1 : (n > cap ? cap : n));
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X, each with
+// its own L2.  Remapping the hardware block id so that runs of consecutive
+// logical tiles land on one XCD keeps tiles that share halo rows in the same
+// L2.  Identity when the grid is not a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblocks)
+{
+    if (nblocks & 7u) return b;
+    const uint32_t per = nblocks >> 3;
+    return (b & 7u) * per + (b >> 3);
+}
+
+}  // namespace pde

@@ -1,0 +1,101 @@
+"""Fused explicit steps: boundary + Laplacian + update in one pass over HBM.
+
+The reference runs these loops as separate modules (``examples/2D_Heat_Equation.py:58-62``:
+BC copy, Laplacian, ForwardEuler; ``examples/wave2d/standingWave.py:72-76``: the
+same plus a second Euler), moving about 28 bytes per lattice update for a scalar
+heat step.  Here the boundary writes go into the state in place (a surface-sized
+launch) and one kernel reads the state once and writes the next one: 8 bytes per
+update for heat, 20 for the u/v wave step.  Every non-ghost value is the same
+FP32 number the module-by-module loop gives.
+
+The steppers own two ping-pong buffers.  The first call copies the caller's
+state into them; later calls are expected to pass back the tensor the previous
+call returned (``u = step(u)``), exactly like the reference loop's
+``u = u_next``.  The state passed in is modified (its boundary is applied).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from ..FDM.module.laplacian import laplacian_weights
+
+
+class HeatStep:
+    """``u_next = u2 + dt * (alpha * lap(u2))`` with ``u2 = boundary(u)``.
+
+    ``boundary`` is a GridBoundary (functional or legacy) built for the field."""
+
+    def __init__(self, boundary, dx: float, ghost_cells: int, alpha: float, dt: float) -> None:
+        self.boundary = boundary
+        self.weights = laplacian_weights(dx)
+        self.ghost_cells = ghost_cells
+        self.alpha = np.float32(alpha)
+        self.dt = np.float32(dt)
+        self._bufs = None
+
+    def _own(self, u: torch.Tensor) -> torch.Tensor:
+        if self._bufs is None or self._bufs[0].shape != u.shape or self._bufs[0].device != u.device:
+            self._bufs = (torch.empty_like(u), torch.empty_like(u))
+        a, b = self._bufs
+        if u.data_ptr() in (a.data_ptr(), b.data_ptr()):
+            return u
+        a.copy_(u)
+        return a
+
+    def __call__(self, u: torch.Tensor, t=None) -> torch.Tensor:
+        N.check_field(u, "u")
+        u = self._own(u)
+        nxt = self._bufs[1] if u.data_ptr() == self._bufs[0].data_ptr() else self._bufs[0]
+        self.boundary.apply_inplace(u, t)
+        X, Y, Z = (int(s) for s in u.shape[:3])
+        wm, wc, wp = self.weights
+        N.call("pde_heat_step_f32", N.dev_ptr(u), N.dev_ptr(nxt), X, Y, Z, int(u.shape[3]),
+               self.ghost_cells, wm, wc, wp, self.alpha, self.dt, N.stream_of(u))
+        return nxt
+
+
+class WaveStep:
+    """u/v wave step with the reference loop's aliasing:
+    ``v_next = v + dt * (c2 * lap(u2))``, ``u_next = u2 + dt * v'`` where v' is
+    the old v on the first call and v_next afterwards (symplectic Euler).
+    ``c2`` is a number (used as the Laplacian's alpha) or a per-node (X, Y, Z, 1)
+    tensor multiplying ``lap(u2)`` computed with alpha = 1."""
+
+    def __init__(self, boundary, dx: float, ghost_cells: int, c2, dt: float) -> None:
+        self.boundary = boundary
+        self.weights = laplacian_weights(dx)
+        self.ghost_cells = ghost_cells
+        self.dt = np.float32(dt)
+        if isinstance(c2, torch.Tensor):
+            N.check_field(c2, "c2")
+            self.c2, self.alpha = c2, np.float32(1.0)
+        else:
+            self.c2, self.alpha = None, np.float32(c2)
+        self.first = True
+        self._u = None
+        self._v = None
+
+    def __call__(self, u: torch.Tensor, v: torch.Tensor, t=None):
+        N.check_field(u, "u")
+        N.check_field(v, "v")
+        if self._u is None or self._u[0].shape != u.shape:
+            self._u = (torch.empty_like(u), torch.empty_like(u))
+            self._v = torch.empty_like(v)
+        a, b = self._u
+        if u.data_ptr() not in (a.data_ptr(), b.data_ptr()):
+            a.copy_(u)
+            u = a
+        if v.data_ptr() != self._v.data_ptr():
+            self._v.copy_(v)
+        nxt = b if u.data_ptr() == a.data_ptr() else a
+        self.boundary.apply_inplace(u, t)
+        X, Y, Z = (int(s) for s in u.shape[:3])
+        wm, wc, wp = self.weights
+        c2p = N.dev_ptr(self.c2) if self.c2 is not None else None
+        N.call("pde_wave_step_f32", N.dev_ptr(u), N.dev_ptr(self._v), c2p, N.dev_ptr(nxt),
+               N.dev_ptr(self._v), X, Y, Z, self.ghost_cells, wm, wc, wp, self.alpha, self.dt,
+               1 if self.first else 0, N.stream_of(u))
+        self.first = False
+        return nxt, self._v

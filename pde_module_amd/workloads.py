@@ -1,0 +1,59 @@
+"""Seeded synthetic inputs of the benchmark workloads (NumPy
+``default_rng(seed)``, generated in FP64 and rounded to FP32).
+
+* heat_2d_config:   256 x 256 nodes (+1 ghost ring), dx = 1/255, alpha = 0.01,
+                    dt = 0.2 dx^2 / alpha, 8 Gaussian bumps, Dirichlet 0 walls.
+* operator_3d_fields: periodic scalar and vec3 fields, i.i.d. U[-1, 1].
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def gaussian_bumps(rng, coords_x, coords_y, n_bumps=8):
+    """sum_b A_b exp(-|x - c_b|^2 / (2 w_b^2)) with centres U[0.2, 0.8]^2,
+    widths U[0.02, 0.08], amplitudes U[0.5, 1], drawn in that order."""
+    centres = rng.uniform(0.2, 0.8, size=(n_bumps, 2))
+    widths = rng.uniform(0.02, 0.08, size=n_bumps)
+    amps = rng.uniform(0.5, 1.0, size=n_bumps)
+    X, Y = np.meshgrid(coords_x, coords_y, indexing="ij")
+    u = np.zeros_like(X)
+    for (cx, cy), w, a in zip(centres, widths, amps):
+        u += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * w * w))
+    return u
+
+
+@dataclass
+class Heat2D:
+    n: int
+    g: int
+    dx: float
+    alpha: float
+    dt: float
+    u0: np.ndarray  # (n+2g, n+2g, 1, 1) float32
+
+
+def heat_2d_config(seed=0, n=256, alpha=0.01, ghost_cells=1):
+    dx = 1.0 / (n - 1)
+    dt = 0.2 * dx * dx / alpha
+    g = ghost_cells
+    coords = np.arange(-g, n + g, dtype=np.float64) * dx
+    rng = np.random.default_rng(seed)
+    u = gaussian_bumps(rng, coords, coords)
+    u0 = u.astype(np.float32).reshape(n + 2 * g, n + 2 * g, 1, 1)
+    return Heat2D(n, g, dx, alpha, dt, u0)
+
+
+def operator_3d_fields(seed=0, n=512, ghost_cells=1):
+    """Interior-only draws (n^3 scalar, n^3 x 3 vector) padded by ghost_cells;
+    ghosts are left 0 for the caller to fill periodically."""
+    rng = np.random.default_rng(seed)
+    g = ghost_cells
+    P = n + 2 * g
+    s = np.zeros((P, P, P, 1), np.float32)
+    v = np.zeros((P, P, P, 3), np.float32)
+    s[g:P - g, g:P - g, g:P - g, 0] = rng.uniform(-1, 1, size=(n, n, n))
+    v[g:P - g, g:P - g, g:P - g, :] = rng.uniform(-1, 1, size=(n, n, n, 3))
+    return s, v

@@ -1,0 +1,79 @@
+"""The C-ABI library loads and exports every entry point include/pde_hip.h
+declares; the op-record layout matches the header (CPU only: no compute)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pde_module_amd._native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+           if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|char\s*\*|const char\s*\*|void)\s*\*?\s*(pde_\w+)\s*\(",
+                             text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_the_bound_entry_points():
+    names = declared_functions()
+    assert names, "no declarations found"
+    assert names == set(N.SIGNATURES), names ^ set(N.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert declared_functions() <= exported
+
+
+def test_library_has_gfx950_code_only():
+    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", N.LIB_PATH], capture_output=True, text=True)
+    if out.returncode != 0:
+        pytest.skip("roc-obj-ls unavailable")
+    targets = set(re.findall(r"gfx\d+\w*", out.stdout))
+    assert targets == {"gfx950"}, targets
+
+
+def test_version_and_status_strings():
+    lib = N.load()
+    assert lib.pde_version() >= 100
+    assert lib.pde_status_string(0) == b"ok"
+    assert b"invalid" in lib.pde_status_string(-1)
+
+
+def test_bc_op_record_layout():
+    text = open(os.path.join(ROOT, "include", "pde_hip.h")).read()
+    body = re.search(r"typedef struct pde_bc_op \{(.*?)\} pde_bc_op;", text, flags=re.S).group(1)
+    fields = re.findall(r"(int64_t|int32_t|float)\s+(\w+);", body)
+    assert [f for _, f in fields] == list(N.BC_OP_DTYPE.names)
+    sizes = {"int64_t": 8, "int32_t": 4, "float": 4}
+    assert sum(sizes[t] for t, _ in fields) == N.BC_OP_DTYPE.itemsize == 32
+    kinds = dict(re.findall(r"#define (PDE_BC_\w+) (\d+)", text))
+    assert int(kinds["PDE_BC_SET"]) == N.BC_SET and int(kinds["PDE_BC_GHOST_N"]) == N.BC_GHOST_N
+
+
+def test_argument_errors_need_no_gpu():
+    """Entry points validate arguments before touching the device."""
+    lib = N.load()
+    assert lib.pde_laplacian_f32(None, None, 8, 8, 8, 1, 1, 1.0, -2.0, 1.0, 1.0, None) == -1
+    fake = ctypes.c_void_p(16)  # never dereferenced: the shape check fails first
+    assert lib.pde_laplacian_f32(fake, fake, 2, 8, 8, 1, 1, 1.0, -2.0, 1.0, 1.0, None) == -2
+    assert lib.pde_bc_apply_f32(fake, fake, 10, None, 5, None, None) == -1
+    assert lib.pde_wave_step_f32(fake, fake, None, fake, fake, 8, 8, 1, 1, 1.0, -2.0, 1.0,
+                                 1.0, 0.1, 0, None) == -1
