@@ -169,7 +169,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(lap_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "lap3d_march", "avg_launch_us": round(t_lap * 1e6, 2),
+                     "kernel": "lap3d_tile", "avg_launch_us": round(t_lap * 1e6, 2),
                      "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
         "extra": {
             "lap_glups_kernel_only": round(LUPS / t_lap / 1e9, 1),

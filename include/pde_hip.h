@@ -173,6 +173,64 @@ int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t nops, const
 int pde_periodic_fill_f32(float *u, int64_t nx, int64_t ny, int64_t nz, int32_t ncomp,
                           int32_t ghost, void *stream);
 
+/* ------------------------------------------------------------------ adjoints
+ *
+ * Transposes for reverse-mode differentiation.  None has a reference
+ * counterpart (PDE_Module has no autodiff: no wp.Tape anywhere); they are the
+ * exact transposes of the forward entry points above, each forward operator
+ * viewed as a linear map from the whole padded field to its interior nodes.
+ * Every one writes the whole padded input-gradient field and never reads the
+ * ghost values of the incoming gradient.  Input and output must not alias. */
+
+/* gin[m] = alpha * sum over axes (ascending), s = -1, 0, +1, of
+ * w_s * gout[m - s e] for each interior node m - s e (transpose of
+ * pde_laplacian_f32). */
+int pde_laplacian_adjoint_f32(const float *gout, float *gin, int64_t nx, int64_t ny,
+                              int64_t nz, int32_t ncomp, int32_t ghost, float w_minus,
+                              float w_center, float w_plus, float alpha, void *stream);
+
+/* Transpose of pde_grad_f32: gout holds ncomp x D per node, gin ncomp per node. */
+int pde_grad_adjoint_f32(const float *gout, float *gin, int64_t nx, int64_t ny, int64_t nz,
+                         int32_t ncomp, int32_t ghost, float w_minus, float w_center,
+                         float w_plus, float alpha, void *stream);
+
+/* Transpose of pde_divergence_f32: gout holds nrows per node, gin nrows x D. */
+int pde_divergence_adjoint_f32(const float *gout, float *gin, int64_t nx, int64_t ny,
+                               int64_t nz, int32_t nrows, int32_t ghost, float w_minus,
+                               float w_center, float w_plus, float alpha, void *stream);
+
+/* Transpose of pde_heat_step_f32 (as a map u2 -> u_next):
+ * gu2 = g + dt * (alpha * laplacian_transpose_sum(g)). */
+int pde_heat_step_adjoint_f32(const float *g, float *gu2, int64_t nx, int64_t ny, int64_t nz,
+                              int32_t ncomp, int32_t ghost, float w_minus, float w_center,
+                              float w_plus, float alpha, float dt, void *stream);
+
+/* Transpose of pde_wave_step_f32 in (u2, v) and its derivative in c2.  With
+ * gvt = gvn + dt*gun (gvn when first_step) on interior nodes:
+ *   gv  = gvt (gvn + dt*gun when first_step) on interior nodes, gvn on ghosts;
+ *   gu2 = gun + dt * (alpha * laplacian_transpose_sum(c2 * gvt));
+ *   gc2 += dt * (gvt * (alpha * acc(u2))) on interior nodes (skipped when
+ *   gc2 is NULL; u2 is the boundary-applied state the forward step read).
+ * c2 == NULL means the uniform coefficient 1. */
+int pde_wave_step_adjoint_f32(const float *gun, const float *gvn, const float *c2,
+                              const float *u2, float *gu2, float *gv, float *gc2, int64_t nx,
+                              int64_t ny, int64_t nz, int32_t ghost, float w_minus,
+                              float w_center, float w_plus, float alpha, float dt,
+                              int32_t first_step, void *stream);
+
+/* Transpose of the linear part of pde_bc_apply_f32: gin = gout, then
+ * gin[zero_idx[i]] = 0 for every written scalar, then for each distinct
+ * source src_idx[i] (flag bit 0 set when it is itself written):
+ * gin[src] = (flag ? 0 : gout[src]) + sum over q in [off[i], off[i+1]) of
+ * csign[q] * gout[cdst[q]].  The host builds these arrays from the op table. */
+int pde_bc_adjoint_f32(const float *gout, float *gin, int64_t n, const int64_t *zero_idx,
+                       int64_t nzero, const int64_t *src_idx, const int32_t *src_flags,
+                       const int64_t *off, const int64_t *cdst, const float *csign,
+                       int64_t nsrc, void *stream);
+
+/* out[i] = scale * in[i] (the dt * g factor of ForwardEuler's transpose). */
+int pde_scale_f32(const float *in, float *out, int64_t n, float scale, void *stream);
+
 /* ------------------------------------------------------------------- library */
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -285,3 +285,145 @@ def wave_step(u, v, w, c2, g, dt_step, bc, first):
     v_next = forward_euler(v, lap, dt_step)
     u_next = forward_euler(u2, v if first else v_next, dt_step)
     return u_next, v_next
+
+
+# --------------------------------------------------------------------------- adjoints
+# Transposes of the operators above, seen as linear maps from the whole padded
+# field to its interior nodes (output ghosts are not part of the range).  Each
+# is written as a gather in a fixed order: for every padded node m, over axes
+# ascending and s = -1, 0, +1, the term w_s * g[m - s e_a] of every interior
+# node m - s e_a that reads m.  Pinned by dot-product tests against the
+# forward functions (tests/test_oracle_adjoints.py).
+def _shift(a, axis, s):
+    """out[m] = a[m - s e_axis], zero where m - s e_axis leaves the array."""
+    out = np.zeros_like(a)
+    n = a.shape[axis]
+    src = [slice(None)] * a.ndim
+    dst = [slice(None)] * a.ndim
+    if s > 0:
+        dst[axis], src[axis] = slice(s, n), slice(0, n - s)
+    elif s < 0:
+        dst[axis], src[axis] = slice(0, n + s), slice(-s, n)
+    out[tuple(dst)] = a[tuple(src)]
+    return out
+
+
+def _interior_only(g, g_cells):
+    """g with everything outside the interior nodes set to 0 (the transposes
+    never read output ghosts)."""
+    out = np.zeros_like(g)
+    inner = interior(g.shape[:3], g_cells)
+    out[inner] = g[inner]
+    return out
+
+
+def laplacian_adjoint(gout, w, alpha, g):
+    dt = gout.dtype.type
+    gz = _interior_only(gout, g)
+    acc = np.zeros_like(gz)
+    for a in eligible_axes(gz.shape[:3]):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * _shift(gz, a, s)
+    return dt(alpha) * acc
+
+
+def grad_adjoint(gout, w, alpha, g):
+    """gout (X,Y,Z,N,D) or (X,Y,Z,D) for a scalar input; returns (X,Y,Z,N)."""
+    dt = gout.dtype.type
+    shape3 = gout.shape[:3]
+    axes = eligible_axes(shape3)
+    m = gout if gout.ndim == 5 else gout.reshape(shape3 + (1, len(axes)))
+    gz = _interior_only(m, g)
+    acc = np.zeros(shape3 + (m.shape[3],), dtype=gout.dtype)
+    for t, a in enumerate(axes):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * _shift(gz[..., t], a, s)
+    return dt(alpha) * acc
+
+
+def divergence_adjoint(gout, w, alpha, g, D):
+    """gout (X,Y,Z,rows); returns (X,Y,Z,rows,D) (rows == 1: reshape to a vec D
+    field for the vector form)."""
+    dt = gout.dtype.type
+    shape3 = gout.shape[:3]
+    axes = eligible_axes(shape3)
+    gz = _interior_only(gout, g)
+    out = np.zeros(shape3 + (gout.shape[3], D), dtype=gout.dtype)
+    for t, a in enumerate(axes):
+        acc = np.zeros_like(gz)
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * _shift(gz, a, s)
+        out[..., t] = dt(alpha) * acc
+    return out
+
+
+def boundary_linear_adjoint(gout, writes):
+    """Transpose of the linear part of a boundary apply.  ``writes`` lists the
+    final (dst, src, sign) of every written scalar (sign 0 for a constant
+    write): out[dst] = sign * in[src] + const, out = in elsewhere."""
+    fo = gout.reshape(-1)
+    gin = fo.copy()
+    dsts = np.array([d for d, _, _ in writes], dtype=np.int64)
+    gin[dsts] = 0
+    for d, s, sign in writes:
+        if sign != 0:
+            gin[s] = gin[s] + gout.dtype.type(sign) * fo[d]
+    return gin.reshape(gout.shape)
+
+
+def heat_step_adjoint(gnext, w, alpha, g, dt_step):
+    """Transpose of u2 -> u_next = u2 + dt * (alpha * lap(u2)) (interior) /
+    u2 (ghosts): g_u2 = g + dt * (alpha * accT), accT the Laplacian transpose
+    sum of g over interior nodes."""
+    dt = gnext.dtype.type
+    gz = _interior_only(gnext, g)
+    acc = np.zeros_like(gz)
+    for a in eligible_axes(gz.shape[:3]):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * _shift(gz, a, s)
+    return gnext + dt(dt_step) * (dt(alpha) * acc)
+
+
+def _acc_full(u, w, g):
+    """Laplacian sum (no alpha) on interior nodes, 0 elsewhere."""
+    dt = u.dtype.type
+    shape3 = u.shape[:3]
+    out = np.zeros_like(u)
+    acc = np.zeros_like(u[interior(shape3, g)])
+    for a in eligible_axes(shape3):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * u[interior(shape3, g, a, s)]
+    out[interior(shape3, g)] = acc
+    return out
+
+
+def wave_step_core(u2, v, c2, w, alpha, g, dt_step, first):
+    """The fused u/v wave update on a boundary-applied u2 (no boundary call):
+    interior vn = v + dt*(c2*(alpha*acc)), un = u2 + dt*(v if first else vn);
+    ghosts vn = v, un = u2.  c2 None means 1."""
+    dt = u2.dtype.type
+    M = interior_mask(u2.shape[:3], g)[..., None]
+    rhs = dt(alpha) * _acc_full(u2, w, g)
+    if c2 is not None:
+        rhs = c2 * rhs
+    vn = np.where(M, v + dt(dt_step) * rhs, v)
+    un = np.where(M, u2 + dt(dt_step) * (v if first else vn), u2)
+    return un, vn
+
+
+def wave_step_adjoint(gun, gvn, c2, u2, w, alpha, g, dt_step, first):
+    """Transpose of wave_step_core in (u2, v) and its derivative in c2.
+    Returns (gu2, gv, dc2) with dc2 the increment added to the c2 gradient."""
+    dt = gun.dtype.type
+    M = interior_mask(gun.shape[:3], g)[..., None]
+    d = dt(dt_step)
+    gvt = np.where(M, gvn if first else gvn + d * gun, gvn)
+    h = np.where(M, gvt if c2 is None else c2 * gvt, dt(0))
+    acc = np.zeros_like(h)
+    for a in eligible_axes(h.shape[:3]):
+        for s, ws in zip((-1, 0, 1), w):
+            acc = acc + dt(ws) * _shift(h, a, s)
+    gu2 = gun + d * (dt(alpha) * acc)
+    gv = np.where(M, (gvn + d * gun) if first else gvt, gvn)
+    dc2 = np.where(M, d * (gvt * (dt(alpha) * _acc_full(u2, w, g))), dt(0))
+    return gu2, gv, dc2

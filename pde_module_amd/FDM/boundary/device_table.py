@@ -16,6 +16,8 @@ class DeviceBoundaryTable:
         self.ops = torch.from_numpy(raw.copy()).to(device)
         self.staging = torch.empty(max(self.nops, 1), dtype=torch.float32, device=device)
         self.host_ops = ops
+        self.device = device
+        self._adj = None
         # Largest scalar index any op touches; checked against the field size at
         # every call so a bad table raises on the host instead of faulting.
         if self.nops:
@@ -51,3 +53,24 @@ class DeviceBoundaryTable:
                t.data_ptr() if t is not None else None, self.staging.data_ptr(),
                N.stream_of(u))
         return u
+
+    def adjoint(self, gout: torch.Tensor, gin: torch.Tensor) -> torch.Tensor:
+        """gin = transpose of the boundary's linear part applied to gout."""
+        N.check_field(gout, "gout")
+        N.check_field(gin, "gin")
+        self._check_bounds(gout.numel())
+        if self._adj is None:
+            from .tables import transpose_table
+            T = transpose_table(self.host_ops) if self.nops else None
+            if T is None:
+                self._adj = (None, 0, None, None, None, None, None, 0)
+            else:
+                dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+                self._adj = (dev(T["zero_idx"]), len(T["zero_idx"]), dev(T["src_idx"]),
+                             dev(T["flags"]), dev(T["off"]), dev(T["cdst"]), dev(T["csign"]),
+                             len(T["src_idx"]))
+        z, nz, s, f, o, cd, cs, ns = self._adj
+        p = lambda t: t.data_ptr() if t is not None else None
+        N.call("pde_bc_adjoint_f32", N.dev_ptr(gout), N.dev_ptr(gin), gout.numel(), p(z), nz,
+               p(s), p(f), p(o), p(cd), p(cs), ns, N.stream_of(gout))
+        return gin

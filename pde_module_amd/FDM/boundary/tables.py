@@ -140,3 +140,22 @@ def legacy_ops(grid_shape, ncomp, ijk, interior, btype, bvalue, dx32):
             W.add(ghost[n] * ncomp + c, adj[n] * ncomp + c, BC_GHOST_N, v[n], coef[n], 0.0,
                   key[n] + 1)
     return W.resolve()
+
+
+def transpose_table(ops):
+    """Arrays of pde_bc_adjoint_f32 for a resolved op table: every written
+    scalar is zeroed, then each distinct source gathers sign * g[dst] over the
+    ops that read it (sorted by dst), plus g[src] itself unless it is written.
+    Signs: COPY +1, Neumann ghost +1, Dirichlet ghost -1; SET reads nothing."""
+    zero_idx = np.ascontiguousarray(ops["dst"], dtype=np.int64)
+    reads = ops[ops["kind"] != BC_SET]
+    sign = np.where(reads["kind"] == BC_GHOST_D, np.float32(-1.0), np.float32(1.0)).astype(np.float32)
+    order = np.lexsort((reads["dst"], reads["src"]))
+    src_sorted = reads["src"][order]
+    cdst = np.ascontiguousarray(reads["dst"][order], dtype=np.int64)
+    csign = np.ascontiguousarray(sign[order])
+    src_idx, start = np.unique(src_sorted, return_index=True)
+    off = np.append(start, len(src_sorted)).astype(np.int64)
+    flags = np.isin(src_idx, zero_idx).astype(np.int32)
+    return dict(zero_idx=zero_idx, src_idx=np.ascontiguousarray(src_idx, dtype=np.int64),
+                flags=flags, off=off, cdst=cdst, csign=csign)

@@ -32,6 +32,14 @@ SIGNATURES = {
     "pde_bc_apply_f32": [_P, _P, _I64, _P, _I64, _P, _P],
     "pde_bc_apply_inplace_f32": [_P, _P, _I64, _P, _P, _P],
     "pde_periodic_fill_f32": [_P, _I64, _I64, _I64, _I32, _I32, _P],
+    "pde_laplacian_adjoint_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_grad_adjoint_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_divergence_adjoint_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_heat_step_adjoint_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _F, _P],
+    "pde_wave_step_adjoint_f32": [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F,
+                                  _F, _F, _F, _I32, _P],
+    "pde_bc_adjoint_f32": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
+    "pde_scale_f32": [_P, _P, _I64, _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],
 }

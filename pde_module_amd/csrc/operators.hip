@@ -67,59 +67,150 @@ __global__ __launch_bounds__(256) void lap_generic(const float *__restrict__ in,
         }
 }
 
-// 3D scalar Laplacian, x-marching.  A block owns a 64 (z) x BY (y) column
-// tile and walks NXC planes along the slowest axis, keeping the x-1 / x / x+1
-// centre values in registers so each plane is fetched from HBM once; y and z
-// neighbours of the current plane come from the loads of the adjacent lanes
-// and rows (L1/L2 hits).  Tiles are numbered z-fastest and XCD-remapped so
-// neighbouring tiles share an L2.
-struct March3 {
+// 3D scalar Laplacian, x-marching row tiles.
+//
+// A block owns TB rows (y) x 512 points (z) and walks NXC planes along the
+// slowest axis x.  Each lane holds 4 consecutive z points, so every global
+// access is a 16-byte load or store (unaligned by the ghost offset, which the
+// hardware serves); the x-1 / x / x+1 values stay in registers, so each plane
+// is read from memory once; the y neighbours of a row come from the rows of
+// the same block through LDS (only the tile's top and bottom halo rows are
+// loaded), and the z neighbours from the adjacent lanes by shuffles (lane 0 and
+// lane 63 of a wave load their outer one).  Tiles are numbered z-fastest, then
+// y, then x, and XCD-remapped so that neighbouring tiles share an L2.
+//
+// Lanes whose 4 points run past the interior use scalar loads and stores;
+// lanes and rows entirely outside it still take part in the barriers.
+struct Tile3 {
     int64_t SX, SY;        // plane and row pitch (floats)
     int64_t E0, E1, E2;    // interior extents
     int64_t g;
     uint32_t tz, ty;       // tile counts along z and y
 };
 
-template <int BY, int NXC>
-__global__ __launch_bounds__(64 * BY) void lap3d_march(const float *__restrict__ in,
-                                                      float *__restrict__ out, March3 m,
-                                                      float wm, float wc, float wp,
-                                                      float alpha)
+constexpr int LAP_TB = 8;      // rows per block
+constexpr int LAP_NXC = 32;    // planes per block
+constexpr int LAP_TZ = 512;    // z points per block (128 lanes x 4)
+
+typedef float pde_f4 __attribute__((ext_vector_type(4)));
+typedef float pde_f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ pde_f4 ld4(const float *p)
 {
+    const pde_f4u v = *reinterpret_cast<const pde_f4u *>(p);
+    return pde_f4{v.x, v.y, v.z, v.w};
+}
+
+__device__ __forceinline__ void st4(float *p, pde_f4 v)
+{
+    *reinterpret_cast<pde_f4u *>(p) = pde_f4u{v.x, v.y, v.z, v.w};
+}
+
+// Loads the n (1..3) valid points of a partial lane; the rest read as 0.
+__device__ __forceinline__ pde_f4 ld4_part(const float *p, int n)
+{
+    pde_f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    v.x = p[0];
+    if (n > 1) v.y = p[1];
+    if (n > 2) v.z = p[2];
+    return v;
+}
+
+__device__ __forceinline__ float lap7(float xm, float xc, float xp, float ym, float yp, float zm,
+                                      float zp, float wm, float wc, float wp, float alpha)
+{
+    float acc = 0.0f;
+    acc += wm * xm;
+    acc += wc * xc;
+    acc += wp * xp;
+    acc += wm * ym;
+    acc += wc * xc;
+    acc += wp * yp;
+    acc += wm * zm;
+    acc += wc * xc;
+    acc += wp * zp;
+    return alpha * acc;
+}
+
+__global__ __launch_bounds__(128 * LAP_TB) void lap3d_tile(const float *__restrict__ in,
+                                                          float *__restrict__ out, Tile3 m,
+                                                          float wm, float wc, float wp,
+                                                          float alpha)
+{
+    __shared__ pde_f4 rows[2][LAP_TB][128];
     uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t zt = b % m.tz;
     b /= m.tz;
     const uint32_t yt = b % m.ty;
     const uint32_t xt = b / m.ty;
-    const int64_t k = (int64_t)zt * 64 + threadIdx.x;
-    const int64_t j = (int64_t)yt * BY + threadIdx.y;
-    if (k >= m.E2 || j >= m.E1) return;
-    const int64_t x0 = (int64_t)xt * NXC;
+    const int q = threadIdx.x;          // lane within the 128-lane row group
+    const int w = threadIdx.y;          // row within the tile
+    const int lane = q & 63;
+    const int64_t k0 = (int64_t)zt * LAP_TZ + 4 * q;        // first interior z of the lane
+    const int64_t nk = m.E2 - k0;                            // valid points from k0
+    const int mode = nk >= 4 ? 2 : (nk > 0 ? 1 : 0);         // 2 full, 1 partial, 0 idle
+    const int np = (int)(nk < 4 ? (nk > 0 ? nk : 0) : 4);
+    const int64_t j = (int64_t)yt * LAP_TB + w;
+    int64_t rows_here = m.E1 - (int64_t)yt * LAP_TB;
+    if (rows_here > LAP_TB) rows_here = LAP_TB;
+    const bool row_ok = w < rows_here;
+    const bool act = row_ok && mode != 0;
+    const int64_t x0 = (int64_t)xt * LAP_NXC;
     int64_t nx = m.E0 - x0;
-    if (nx > NXC) nx = NXC;
-    int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k + m.g);
-    float xm = in[e - m.SX];
-    float xc = in[e];
-#pragma unroll 4
+    if (nx > LAP_NXC) nx = LAP_NXC;
+    int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k0 + m.g);
+    pde_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    pde_f4 xm = zero, xc = zero;
+    if (act) {
+        xm = mode == 2 ? ld4(in + e - m.SX) : ld4_part(in + e - m.SX, np);
+        xc = mode == 2 ? ld4(in + e) : ld4_part(in + e, np);
+    }
+    int buf = 0;
     for (int64_t x = 0; x < nx; ++x, e += m.SX) {
-        const float xp = in[e + m.SX];
-        const float ym = in[e - m.SY];
-        const float yp = in[e + m.SY];
-        const float zm = in[e - 1];
-        const float zp = in[e + 1];
-        float acc = 0.0f;
-        acc += wm * xm;
-        acc += wc * xc;
-        acc += wp * xp;
-        acc += wm * ym;
-        acc += wc * xc;
-        acc += wp * yp;
-        acc += wm * zm;
-        acc += wc * xc;
-        acc += wp * zp;
-        out[e] = alpha * acc;
+        pde_f4 xp = zero, ylo = zero, yhi = zero;
+        if (act) {
+            xp = mode == 2 ? ld4(in + e + m.SX) : ld4_part(in + e + m.SX, np);
+            if (w == 0) ylo = mode == 2 ? ld4(in + e - m.SY) : ld4_part(in + e - m.SY, np);
+            if (w == rows_here - 1)
+                yhi = mode == 2 ? ld4(in + e + m.SY) : ld4_part(in + e + m.SY, np);
+        }
+        rows[buf][w][q] = xc;
+        __syncthreads();
+        if (act) {
+            if (w > 0) ylo = rows[buf][w - 1][q];
+            if (w < rows_here - 1) yhi = rows[buf][w + 1][q];
+        }
+        float zl = __shfl_up(xc.w, 1, 64);
+        float zr = __shfl_down(xc.x, 1, 64);
+        if (act) {
+            if (lane == 0) zl = in[e - 1];
+            if (mode == 2) {
+                if (lane == 63 || nk == 4) zr = in[e + 4];
+                pde_f4 o;
+                o.x = lap7(xm.x, xc.x, xp.x, ylo.x, yhi.x, zl, xc.y, wm, wc, wp, alpha);
+                o.y = lap7(xm.y, xc.y, xp.y, ylo.y, yhi.y, xc.x, xc.z, wm, wc, wp, alpha);
+                o.z = lap7(xm.z, xc.z, xp.z, ylo.z, yhi.z, xc.y, xc.w, wm, wc, wp, alpha);
+                o.w = lap7(xm.w, xc.w, xp.w, ylo.w, yhi.w, xc.z, zr, wm, wc, wp, alpha);
+                st4(out + e, o);
+            } else {
+                // partial lane: points 0 .. np-1, right neighbour of the last
+                // one is the ghost column after the interior
+                const float c[4] = {xc.x, xc.y, xc.z, xc.w};
+                const float a[4] = {xm.x, xm.y, xm.z, xm.w};
+                const float p[4] = {xp.x, xp.y, xp.z, xp.w};
+                const float lo[4] = {ylo.x, ylo.y, ylo.z, ylo.w};
+                const float hi[4] = {yhi.x, yhi.y, yhi.z, yhi.w};
+                const float zlast = in[e + np];
+                for (int t = 0; t < np; ++t) {
+                    const float zm = t == 0 ? zl : c[t > 0 ? t - 1 : 0];
+                    const float zp = t == np - 1 ? zlast : c[t + 1 < 4 ? t + 1 : 3];
+                    out[e + t] = lap7(a[t], c[t], p[t], lo[t], hi[t], zm, zp, wm, wc, wp, alpha);
+                }
+            }
+        }
         xm = xc;
         xc = xp;
+        buf ^= 1;
     }
 }
 
@@ -205,6 +296,13 @@ __global__ __launch_bounds__(256) void div_generic(const float *__restrict__ in,
 }
 
 // ------------------------------------------------------- fused lap/grad/div
+
+struct March3 {
+    int64_t SX, SY;        // plane and row pitch (floats)
+    int64_t E0, E1, E2;    // interior extents
+    int64_t g;
+    uint32_t tz, ty;       // tile counts along z and y
+};
 
 // One x-marching pass over a scalar s and a vec3 v: lap(s), grad(s), div(v).
 // s's x-1 / x / x+1 and v.x's x-1 / x+1 ride in registers.
@@ -340,21 +438,20 @@ extern "C" int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_
     if (st) return st;
     hipStream_t s = as_stream(stream);
     if (G.ndim == 3 && ncomp == 1) {
-        constexpr int BY = 4, NXC = 64;
-        March3 m;
+        Tile3 m;
         m.SX = G.S[0];
         m.SY = G.S[1];
         m.E0 = G.E[0];
         m.E1 = G.E[1];
         m.E2 = G.E[2];
         m.g = ghost;
-        m.tz = (uint32_t)((G.E[2] + 63) / 64);
-        m.ty = (uint32_t)((G.E[1] + BY - 1) / BY);
-        const int64_t tx = (G.E[0] + NXC - 1) / NXC;
+        m.tz = (uint32_t)((G.E[2] + LAP_TZ - 1) / LAP_TZ);
+        m.ty = (uint32_t)((G.E[1] + LAP_TB - 1) / LAP_TB);
+        const int64_t tx = (G.E[0] + LAP_NXC - 1) / LAP_NXC;
         const int64_t nb = (int64_t)m.tz * m.ty * tx;
         if (nb > 0x7fffffff) return PDE_ESHAPE;
-        hipLaunchKernelGGL((lap3d_march<BY, NXC>), dim3((unsigned)nb), dim3(64, BY), 0, s, in,
-                           out, m, wm, wc, wp, alpha);
+        hipLaunchKernelGGL(lap3d_tile, dim3((unsigned)nb), dim3(128, LAP_TB), 0, s, in, out, m,
+                           wm, wc, wp, alpha);
         return launch_status();
     }
     RowGeom rg = row_geom(G, ncomp);

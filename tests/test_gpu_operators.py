@@ -84,8 +84,12 @@ def test_output_ghosts_untouched(dev):
     assert np.all(np.isnan(o[~m])) and not np.any(np.isnan(o[m]))
 
 
-def test_marching_kernel_equals_generic_kernel(dev):
-    shape = (131, 67, 200)
+@pytest.mark.parametrize("shape", [(131, 67, 200), (20, 19, 1030), (5, 3, 4), (34, 9, 517),
+                                   (70, 33, 515), (3, 3, 3), (9, 17, 1)])
+def test_tiled_kernel_equals_generic_kernel(dev, shape):
+    """The 3D tiled Laplacian (partial lanes, partial row tiles, several z
+    tiles, short x chunks) against the one-thread-per-point kernel."""
+    print(f"tiled vs generic {shape}", flush=True)
     u = torch.rand(shape + (1,), device=dev) * 2 - 1
     a = torch.zeros_like(u)
     b = torch.zeros_like(u)
