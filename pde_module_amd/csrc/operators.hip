@@ -70,14 +70,14 @@ __global__ __launch_bounds__(256) void lap_generic(const float *__restrict__ in,
 // 3D scalar Laplacian, x-marching row tiles.
 //
 // A block owns TB rows (y) x 512 points (z) and walks NXC planes along the
-// slowest axis x.  Each lane holds 4 consecutive z points, so every global
-// access is a 16-byte load or store (unaligned by the ghost offset, which the
-// hardware serves); the x-1 / x / x+1 values stay in registers, so each plane
-// is read from memory once; the y neighbours of a row come from the rows of
-// the same block through LDS (only the tile's top and bottom halo rows are
-// loaded), and the z neighbours from the adjacent lanes by shuffles (lane 0 and
-// lane 63 of a wave load their outer one).  Tiles are numbered z-fastest, then
-// y, then x, and XCD-remapped so that neighbouring tiles share an L2.
+// slowest axis x, two planes per step.  Each lane holds 4 consecutive z
+// points, so every global access is a 16-byte load or store (unaligned by the
+// ghost offset, which the hardware serves); planes x-1 .. x+2 stay in
+// registers, so each plane is read from memory once; the y neighbours of a row
+// come from the rows of the same block through LDS (only the tile's top and
+// bottom halo rows are loaded), and the z neighbours from the adjacent lanes by
+// shuffles (lane 0 and lane 63 of a wave load their outer one).  Tiles are
+// numbered z-fastest, then y, then x.
 //
 // Lanes whose 4 points run past the interior use scalar loads and stores;
 // lanes and rows entirely outside it still take part in the barriers.
@@ -89,7 +89,7 @@ struct Tile3 {
 };
 
 constexpr int LAP_TB = 8;      // rows per block
-constexpr int LAP_NXC = 32;    // planes per block
+constexpr int LAP_NXC = 64;    // planes per block
 constexpr int LAP_TZ = 512;    // z points per block (128 lanes x 4)
 
 typedef float pde_f4 __attribute__((ext_vector_type(4)));
@@ -116,6 +116,12 @@ __device__ __forceinline__ pde_f4 ld4_part(const float *p, int n)
     return v;
 }
 
+// mode 2: a full lane, 1: a partial lane with np points, 0: idle.
+__device__ __forceinline__ pde_f4 ld4_mode(const float *p, int mode, int np)
+{
+    return mode == 2 ? ld4(p) : ld4_part(p, np);
+}
+
 __device__ __forceinline__ float lap7(float xm, float xc, float xp, float ym, float yp, float zm,
                                       float zp, float wm, float wc, float wp, float alpha)
 {
@@ -132,13 +138,47 @@ __device__ __forceinline__ float lap7(float xm, float xc, float xp, float ym, fl
     return alpha * acc;
 }
 
+// One output plane of a lane: centre cc with x neighbours cm / cp, y
+// neighbours lo / hi, z neighbours from the shuffles (zl, zr).
+__device__ __forceinline__ void lap_lane_plane(float *__restrict__ out, const float *__restrict__ in,
+                                               int64_t ec, int mode, int np, int64_t nk, int lane,
+                                               pde_f4 cm, pde_f4 cc, pde_f4 cp, pde_f4 lo,
+                                               pde_f4 hi, float zl, float zr, float wm, float wc,
+                                               float wp, float alpha)
+{
+    if (lane == 0) zl = in[ec - 1];
+    if (mode == 2) {
+        if (lane == 63 || nk == 4) zr = in[ec + 4];
+        pde_f4 o;
+        o.x = lap7(cm.x, cc.x, cp.x, lo.x, hi.x, zl, cc.y, wm, wc, wp, alpha);
+        o.y = lap7(cm.y, cc.y, cp.y, lo.y, hi.y, cc.x, cc.z, wm, wc, wp, alpha);
+        o.z = lap7(cm.z, cc.z, cp.z, lo.z, hi.z, cc.y, cc.w, wm, wc, wp, alpha);
+        o.w = lap7(cm.w, cc.w, cp.w, lo.w, hi.w, cc.z, zr, wm, wc, wp, alpha);
+        st4(out + ec, o);
+        return;
+    }
+    // partial lane: points 0 .. np-1; the right neighbour of the last one is
+    // the ghost column after the interior
+    const float c[4] = {cc.x, cc.y, cc.z, cc.w};
+    const float a[4] = {cm.x, cm.y, cm.z, cm.w};
+    const float p[4] = {cp.x, cp.y, cp.z, cp.w};
+    const float l[4] = {lo.x, lo.y, lo.z, lo.w};
+    const float h[4] = {hi.x, hi.y, hi.z, hi.w};
+    const float zlast = in[ec + np];
+    for (int t = 0; t < np; ++t) {
+        const float zm = t == 0 ? zl : c[t > 0 ? t - 1 : 0];
+        const float zp = t == np - 1 ? zlast : c[t + 1 < 4 ? t + 1 : 3];
+        out[ec + t] = lap7(a[t], c[t], p[t], l[t], h[t], zm, zp, wm, wc, wp, alpha);
+    }
+}
+
 __global__ __launch_bounds__(128 * LAP_TB) void lap3d_tile(const float *__restrict__ in,
                                                           float *__restrict__ out, Tile3 m,
                                                           float wm, float wc, float wp,
                                                           float alpha)
 {
-    __shared__ pde_f4 rows[2][LAP_TB][128];
-    uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    __shared__ pde_f4 rows[2][2][LAP_TB][128];
+    uint32_t b = blockIdx.x;
     const uint32_t zt = b % m.tz;
     b /= m.tz;
     const uint32_t yt = b % m.ty;
@@ -148,68 +188,65 @@ __global__ __launch_bounds__(128 * LAP_TB) void lap3d_tile(const float *__restri
     const int lane = q & 63;
     const int64_t k0 = (int64_t)zt * LAP_TZ + 4 * q;        // first interior z of the lane
     const int64_t nk = m.E2 - k0;                            // valid points from k0
-    const int mode = nk >= 4 ? 2 : (nk > 0 ? 1 : 0);         // 2 full, 1 partial, 0 idle
+    const int mode = nk >= 4 ? 2 : (nk > 0 ? 1 : 0);
     const int np = (int)(nk < 4 ? (nk > 0 ? nk : 0) : 4);
     const int64_t j = (int64_t)yt * LAP_TB + w;
     int64_t rows_here = m.E1 - (int64_t)yt * LAP_TB;
     if (rows_here > LAP_TB) rows_here = LAP_TB;
-    const bool row_ok = w < rows_here;
-    const bool act = row_ok && mode != 0;
+    const bool act = w < rows_here && mode != 0;
+    const bool top = w == 0, bot = w == rows_here - 1;
+    const int64_t hoff = top ? -m.SY : m.SY;                 // halo row of an edge row
     const int64_t x0 = (int64_t)xt * LAP_NXC;
     int64_t nx = m.E0 - x0;
     if (nx > LAP_NXC) nx = LAP_NXC;
     int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k0 + m.g);
-    pde_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const pde_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     pde_f4 xm = zero, xc = zero;
     if (act) {
-        xm = mode == 2 ? ld4(in + e - m.SX) : ld4_part(in + e - m.SX, np);
-        xc = mode == 2 ? ld4(in + e) : ld4_part(in + e, np);
+        xm = ld4_mode(in + e - m.SX, mode, np);
+        xc = ld4_mode(in + e, mode, np);
     }
     int buf = 0;
-    for (int64_t x = 0; x < nx; ++x, e += m.SX) {
-        pde_f4 xp = zero, ylo = zero, yhi = zero;
+    for (int64_t x = 0; x < nx; x += 2, e += 2 * m.SX) {
+        const bool two = x + 1 < nx;     // uniform over the block
+        pde_f4 p1 = zero, p2 = zero, h0 = zero, h1 = zero, h0b = zero, h1b = zero;
         if (act) {
-            xp = mode == 2 ? ld4(in + e + m.SX) : ld4_part(in + e + m.SX, np);
-            if (w == 0) ylo = mode == 2 ? ld4(in + e - m.SY) : ld4_part(in + e - m.SY, np);
-            if (w == rows_here - 1)
-                yhi = mode == 2 ? ld4(in + e + m.SY) : ld4_part(in + e + m.SY, np);
-        }
-        rows[buf][w][q] = xc;
-        __syncthreads();
-        if (act) {
-            if (w > 0) ylo = rows[buf][w - 1][q];
-            if (w < rows_here - 1) yhi = rows[buf][w + 1][q];
-        }
-        float zl = __shfl_up(xc.w, 1, 64);
-        float zr = __shfl_down(xc.x, 1, 64);
-        if (act) {
-            if (lane == 0) zl = in[e - 1];
-            if (mode == 2) {
-                if (lane == 63 || nk == 4) zr = in[e + 4];
-                pde_f4 o;
-                o.x = lap7(xm.x, xc.x, xp.x, ylo.x, yhi.x, zl, xc.y, wm, wc, wp, alpha);
-                o.y = lap7(xm.y, xc.y, xp.y, ylo.y, yhi.y, xc.x, xc.z, wm, wc, wp, alpha);
-                o.z = lap7(xm.z, xc.z, xp.z, ylo.z, yhi.z, xc.y, xc.w, wm, wc, wp, alpha);
-                o.w = lap7(xm.w, xc.w, xp.w, ylo.w, yhi.w, xc.z, zr, wm, wc, wp, alpha);
-                st4(out + e, o);
-            } else {
-                // partial lane: points 0 .. np-1, right neighbour of the last
-                // one is the ghost column after the interior
-                const float c[4] = {xc.x, xc.y, xc.z, xc.w};
-                const float a[4] = {xm.x, xm.y, xm.z, xm.w};
-                const float p[4] = {xp.x, xp.y, xp.z, xp.w};
-                const float lo[4] = {ylo.x, ylo.y, ylo.z, ylo.w};
-                const float hi[4] = {yhi.x, yhi.y, yhi.z, yhi.w};
-                const float zlast = in[e + np];
-                for (int t = 0; t < np; ++t) {
-                    const float zm = t == 0 ? zl : c[t > 0 ? t - 1 : 0];
-                    const float zp = t == np - 1 ? zlast : c[t + 1 < 4 ? t + 1 : 3];
-                    out[e + t] = lap7(a[t], c[t], p[t], lo[t], hi[t], zm, zp, wm, wc, wp, alpha);
-                }
+            p1 = ld4_mode(in + e + m.SX, mode, np);
+            if (two) p2 = ld4_mode(in + e + 2 * m.SX, mode, np);
+            if (top || bot) {
+                h0 = ld4_mode(in + e + hoff, mode, np);
+                if (two) h1 = ld4_mode(in + e + m.SX + hoff, mode, np);
+            }
+            if (top && bot) {            // a one-row tile needs both halos
+                h0b = ld4_mode(in + e + m.SY, mode, np);
+                if (two) h1b = ld4_mode(in + e + m.SX + m.SY, mode, np);
             }
         }
-        xm = xc;
-        xc = xp;
+        rows[buf][0][w][q] = xc;
+        rows[buf][1][w][q] = p1;
+        __syncthreads();
+        {
+            float zl = __shfl_up(xc.w, 1, 64);
+            float zr = __shfl_down(xc.x, 1, 64);
+            if (act) {
+                const pde_f4 lo = top ? h0 : rows[buf][0][w > 0 ? w - 1 : 0][q];
+                const pde_f4 hi = bot ? (top ? h0b : h0) : rows[buf][0][w + 1 < LAP_TB ? w + 1 : w][q];
+                lap_lane_plane(out, in, e, mode, np, nk, lane, xm, xc, p1, lo, hi, zl, zr, wm,
+                               wc, wp, alpha);
+            }
+        }
+        if (two) {
+            float zl = __shfl_up(p1.w, 1, 64);
+            float zr = __shfl_down(p1.x, 1, 64);
+            if (act) {
+                const pde_f4 lo = top ? h1 : rows[buf][1][w > 0 ? w - 1 : 0][q];
+                const pde_f4 hi = bot ? (top ? h1b : h1) : rows[buf][1][w + 1 < LAP_TB ? w + 1 : w][q];
+                lap_lane_plane(out, in, e + m.SX, mode, np, nk, lane, xc, p1, p2, lo, hi, zl,
+                               zr, wm, wc, wp, alpha);
+            }
+        }
+        xm = p1;
+        xc = p2;
         buf ^= 1;
     }
 }
@@ -378,42 +415,60 @@ __global__ __launch_bounds__(64 * BY) void fused3d_march(
 
 // --------------------------------------------------------- periodic fill
 
-// Fills the 2g ghost layers of canonical axis A from the opposite interior
-// layers, over the full extent of the other two axes.  Thread per
-// (ghost-slab node, component).
-template <int A>
-__global__ __launch_bounds__(256) void periodic_axis(float *__restrict__ u, Geom G, int g,
-                                                     int C, int64_t count)
+// Ghost layer number l (0 .. 2g-1) along an axis of padded extent P and
+// interior extent E: for l < g the ghost index l takes index l + E; for l >= g
+// the ghost index P - 2g + l takes index P - 2g + l - E.  With E >= g every
+// source is an interior index, so one launch never reads what it writes.
+__device__ __forceinline__ void ghost_pair(int64_t l, int64_t g, int64_t P, int64_t E,
+                                           int64_t *dst, int64_t *src)
 {
-    const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (tid >= count) return;
-    const int c = (int)(tid % C);
-    int64_t q = tid / C;
-    // slab coordinates: the ghost index along A (0..2g), then the other two
-    // axes in ascending order.
-    int64_t idx[3];
-    const int64_t n_other1 = G.P[A == 2 ? 1 : 2];
-    const int o0 = (A == 0) ? 1 : 0;
-    const int o1 = (A == 2) ? 1 : 2;
-    idx[o1] = q % n_other1;
-    q /= n_other1;
-    idx[o0] = q % G.P[o0];
-    q /= G.P[o0];
-    const int64_t l = q;  // 0 .. 2g-1
-    const int64_t E = G.P[A] - 2 * (int64_t)g;
-    int64_t dst_a, src_a;
     if (l < g) {
-        dst_a = l;
-        src_a = l + E;
+        *dst = l;
+        *src = l + E;
     } else {
-        dst_a = G.P[A] - 2 * (int64_t)g + l;  // P - g + (l - g)
-        src_a = dst_a - E;
+        *dst = P - 2 * g + l;
+        *src = *dst - E;
     }
-    idx[A] = dst_a;
-    const int64_t dn = idx[0] * G.S[0] + idx[1] * G.S[1] + idx[2];
-    idx[A] = src_a;
-    const int64_t sn = idx[0] * G.S[0] + idx[1] * G.S[1] + idx[2];
-    u[dn * C + c] = u[sn * C + c];
+}
+
+// Axis 0 (planes): thread per float of a ghost plane; coalesced.
+__global__ __launch_bounds__(256) void periodic_planes(float *__restrict__ u, int64_t plane,
+                                                       int64_t g, int64_t P, int64_t E)
+{
+    const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (f >= plane) return;
+    int64_t d, s;
+    ghost_pair(blockIdx.y, g, P, E, &d, &s);
+    u[d * plane + f] = u[s * plane + f];
+}
+
+// Axis 1 (rows): thread per float of a ghost row, grid.y = ghost layer,
+// grid.z strides over planes; coalesced.
+__global__ __launch_bounds__(256) void periodic_rows(float *__restrict__ u, int64_t row,
+                                                     int64_t nplanes, int64_t g, int64_t P,
+                                                     int64_t E)
+{
+    const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (f >= row) return;
+    int64_t d, s;
+    ghost_pair(blockIdx.y, g, P, E, &d, &s);
+    const int64_t plane = P * row;
+    for (int64_t i = blockIdx.z; i < nplanes; i += gridDim.z)
+        u[i * plane + d * row + f] = u[i * plane + s * row + f];
+}
+
+// Axis 2 (the contiguous axis): thread per row, copying its 2g ghost nodes.
+__global__ __launch_bounds__(256) void periodic_cols(float *__restrict__ u, int64_t nrows,
+                                                     int C, int g, int64_t P, int64_t E)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= nrows) return;
+    float *row = u + r * P * C;
+    for (int l = 0; l < 2 * g; ++l) {
+        int64_t d, s;
+        ghost_pair(l, g, P, E, &d, &s);
+        for (int c = 0; c < C; ++c) row[d * C + c] = row[s * C + c];
+    }
 }
 
 // ------------------------------------------------------------ dispatchers
@@ -567,14 +622,25 @@ extern "C" int pde_periodic_fill_f32(float *u, int64_t nx, int64_t ny, int64_t n
     for (int A = 3 - G.ndim; A < 3; ++A)
         if (G.E[A] < ghost) return PDE_ESHAPE;  // a ghost's source must be interior
     hipStream_t s = as_stream(stream);
+    const int64_t C = ncomp;
     for (int A = 3 - G.ndim; A < 3; ++A) {
-        int64_t other = G.nodes / G.P[A];
-        int64_t count = 2 * (int64_t)ghost * other * ncomp;
-        if (count == 0) continue;
-        dim3 grid((unsigned)((count + 255) / 256)), block(256);
-        if (A == 0) hipLaunchKernelGGL(periodic_axis<0>, grid, block, 0, s, u, G, ghost, ncomp, count);
-        else if (A == 1) hipLaunchKernelGGL(periodic_axis<1>, grid, block, 0, s, u, G, ghost, ncomp, count);
-        else hipLaunchKernelGGL(periodic_axis<2>, grid, block, 0, s, u, G, ghost, ncomp, count);
+        const int64_t P = G.P[A], E = G.E[A];
+        const unsigned layers = (unsigned)(2 * ghost);
+        if (A == 0) {
+            const int64_t plane = G.S[0] * C;
+            hipLaunchKernelGGL(periodic_planes, dim3((unsigned)((plane + 255) / 256), layers),
+                               dim3(256), 0, s, u, plane, (int64_t)ghost, P, E);
+        } else if (A == 1) {
+            const int64_t row = G.S[1] * C;
+            hipLaunchKernelGGL(periodic_rows,
+                               dim3((unsigned)((row + 255) / 256), layers,
+                                    clamp_grid(G.P[0], 65535)),
+                               dim3(256), 0, s, u, row, G.P[0], (int64_t)ghost, P, E);
+        } else {
+            const int64_t nrows = G.P[0] * G.P[1];
+            hipLaunchKernelGGL(periodic_cols, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0,
+                               s, u, nrows, (int)C, (int)ghost, P, E);
+        }
         st = launch_status();
         if (st) return st;
     }
