@@ -124,6 +124,18 @@ int pde_wave_step_f32(const float *u, const float *v, const float *c2, float *u_
                       float w_minus, float w_center, float w_plus, float alpha, float dt,
                       int32_t first_step, void *stream);
 
+/* out[i] = field[idx[i]] for i < n (receiver samples of a time loop).  No
+ * reference counterpart: the reference's examples read fields back with
+ * .numpy() (e.g. examples/2D_Heat_Equation.py:64-65). */
+int pde_gather_f32(const float *field, const int64_t *idx, float *out, int64_t n, void *stream);
+
+/* field[idx[i]] += scale * vals[i] for i < n; the indices must be distinct
+ * (point sources of a time loop, and the receiver transpose).  No reference
+ * counterpart (the reference's loops add forcing with Warp array operators,
+ * examples/wave2d/Interference.py:143). */
+int pde_scatter_add_f32(float *field, const int64_t *idx, const float *vals, int64_t n,
+                        float scale, void *stream);
+
 /* ------------------------------------------------------------------ boundary */
 
 /* One resolved boundary write.  v = value, or min(t/tmax, 1) * value when

@@ -157,6 +157,21 @@ class GridBoundary(ExplicitUniformGridStencil):
     def forward(self, input_array, t=None, params=None, **kwargs):
         return self.table.apply(input_array, self.output_array, self.t)
 
+    def _ensure_table(self, u: torch.Tensor) -> None:
+        self._check_input(u)
+        if self.table is None or self.table.device != u.device:
+            self.table = DeviceBoundaryTable(self.resolved_ops(), u.device)
+
+    def _time_tensor(self, u: torch.Tensor, t=None):
+        """The 1-element device time array the ramp values read."""
+        if isinstance(t, torch.Tensor):
+            return t
+        if not hasattr(self, "t") or self.t.device != u.device:
+            self.t = torch.zeros(1, dtype=torch.float32, device=u.device)
+        if t is not None:
+            self.t.fill_(float(np.float32(t)))
+        return self.t
+
     def apply_inplace(self, u: torch.Tensor, t=None) -> torch.Tensor:
         """Apply the boundary to ``u`` itself (no copy): the form the fused time
         steps use.  Same values as ``__call__`` on every written scalar."""

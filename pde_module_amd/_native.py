@@ -40,6 +40,8 @@ SIGNATURES = {
                                   _F, _F, _F, _I32, _P],
     "pde_bc_adjoint_f32": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
+    "pde_gather_f32": [_P, _P, _P, _I64, _P],
+    "pde_scatter_add_f32": [_P, _P, _P, _I64, _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],
 }

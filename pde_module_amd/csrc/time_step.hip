@@ -236,3 +236,47 @@ extern "C" const char *pde_status_string(int status)
     if (status > 0) return hipGetErrorString((hipError_t)status);
     return "unknown status";
 }
+
+// ------------------------------------------------------------ point samples
+// Receivers and point sources of a time loop: gather field values at a list
+// of flat indices, and add scaled values at a list of distinct flat indices.
+namespace pde {
+
+__global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ f,
+                                                     const int64_t *__restrict__ idx,
+                                                     float *__restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = f[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void scatter_add_kernel(float *__restrict__ f,
+                                                          const int64_t *__restrict__ idx,
+                                                          const float *__restrict__ vals,
+                                                          int64_t n, float scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) f[idx[i]] = f[idx[i]] + scale * vals[i];
+}
+
+}  // namespace pde
+
+extern "C" int pde_gather_f32(const float *field, const int64_t *idx, float *out, int64_t n,
+                              void *stream)
+{
+    if (!field || !out || n < 0 || (n > 0 && !idx)) return PDE_EINVAL;
+    if (n == 0) return PDE_OK;
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), field, idx, out, n);
+    return launch_status();
+}
+
+extern "C" int pde_scatter_add_f32(float *field, const int64_t *idx, const float *vals, int64_t n,
+                                   float scale, void *stream)
+{
+    if (!field || n < 0 || (n > 0 && (!idx || !vals))) return PDE_EINVAL;
+    if (n == 0) return PDE_OK;
+    hipLaunchKernelGGL(scatter_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), field, idx, vals, n, scale);
+    return launch_status();
+}

@@ -103,6 +103,14 @@ class GridBoundary(Boundary):
     def forward(self, input_array, t=0.0, params=None, **kwargs):
         return self.table.apply(input_array, self.output_array, None)
 
+    def _ensure_table(self, u: torch.Tensor) -> None:
+        self._check_input(u)
+        if self.table is None or self.table.device != u.device:
+            self.table = DeviceBoundaryTable(self.resolved_ops(), u.device)
+
+    def _time_tensor(self, u: torch.Tensor, t=None):
+        return None  # legacy tables hold constants only
+
     def apply_inplace(self, u: torch.Tensor, t=0.0) -> torch.Tensor:
         """Apply the boundary to u itself (no copy), as the fused steps do."""
         self._check_input(u)

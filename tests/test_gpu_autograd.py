@@ -1,0 +1,153 @@
+"""GPU reverse-mode checks: operator autograd against the transpose kernels and
+dot products; checkpointed heat and wave rollouts against torch autograd of the
+FP64 restatement (<= 1e-4 relative gradient error), and
+checkpoint-interval invariance (bit for bit)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import torch_fp64 as TF
+import pde_module_amd.autograd as AD
+from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+from pde_module_amd.rollout import HeatRollout, WaveRollout
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).norm() / b.norm())
+
+
+def _dot_check(fx, y, x, gx):
+    """<f(x), y> == <x, f^T(y)> up to FP32 rounding of the terms."""
+    a = fx.detach().double() * y.double()
+    b = x.detach().double() * gx.double()
+    lhs, rhs = float(a.sum()), float(b.sum())
+    scale = float(a.abs().sum()) + float(b.abs().sum())
+    assert abs(lhs - rhs) <= 2e-6 * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("shape", [(12, 11, 1), (8, 9, 10)])
+def test_operator_autograd_dot_products(dev, shape):
+    torch.manual_seed(0)
+    x = torch.randn(shape + (2,), device=dev, requires_grad=True)
+    y = torch.randn(shape + (2,), device=dev)
+    lx = AD.laplacian(x, 0.1, 1, 0.5)
+    gx, = torch.autograd.grad((lx * y).sum(), x)
+    _dot_check(lx, y, x, gx)
+    G = AD.grad(x, 0.1, 1, 1.5)
+    D = len([s for s in shape if s > 1])
+    z = torch.randn(shape + (2, D), device=dev)
+    gg, = torch.autograd.grad((G * z).sum(), x)
+    _dot_check(G, z, x, gg)
+    m = torch.randn(shape + (2, D), device=dev, requires_grad=True)
+    dv = AD.divergence(m, 0.1, 1, -0.7)
+    q = torch.randn(shape + (2,), device=dev)
+    gm, = torch.autograd.grad((dv * q).sum(), m)
+    _dot_check(dv, q, m, gm)
+
+
+def test_boundary_and_euler_autograd(dev):
+    torch.manual_seed(1)
+    shape = (10, 9, 1)
+    u = torch.randn(shape + (1,), device=dev, requires_grad=True)
+    bc = LegacyGridBoundary(u.detach(), 0.1, 1)
+    k = torch.randn(shape + (1,), device=dev, requires_grad=True)
+    y = torch.randn(shape + (1,), device=dev)
+    out = AD.forward_euler(AD.boundary(u, bc), k, 0.3)
+    gu, gk = torch.autograd.grad((out * y).sum(), (u, k))
+    assert torch.allclose(gk, 0.3 * y)
+    ref = u.detach().cpu().double().requires_grad_(True)
+    oref = TF.bc_dirichlet0(ref, 1) + 0.3 * k.detach().cpu().double()
+    gref, = torch.autograd.grad((oref * y.cpu().double()).sum(), ref)
+    assert rel(gu, gref) < 1e-6
+
+
+def _heat_setup(dev, n=40):
+    rng = np.random.default_rng(3)
+    shape = (n + 2, n + 2, 1)
+    u0 = torch.from_numpy(rng.uniform(-1, 1, shape + (1,)).astype(np.float32)).to(dev)
+    dx = 1.0 / (n - 1)
+    alpha = 0.05
+    return shape, u0, dx, alpha, 0.2 * dx * dx / alpha
+
+
+def test_heat_rollout_gradient(dev):
+    shape, u0, dx, alpha, dt = _heat_setup(dev)
+    steps = 60
+    y = torch.randn(shape + (1,), device=dev, generator=torch.Generator(device=dev).manual_seed(5))
+    grads = []
+    for K in (1, 7, steps):
+        bc = LegacyGridBoundary(u0, dx, 1)
+        ro = HeatRollout(bc, dx, 1, alpha, dt, steps, checkpoint_every=K)
+        x = u0.clone().requires_grad_(True)
+        uN = ro(x)
+        g, = torch.autograd.grad((uN * y).sum(), x)
+        grads.append(g)
+    for g in grads[1:]:
+        assert torch.equal(g.view(torch.int32), grads[0].view(torch.int32))
+    x64 = u0.cpu().double().requires_grad_(True)
+    uN64 = TF.heat_rollout(x64, dx, 1, alpha, dt, steps)
+    m = TF.interior_mask(x64, 1).double()
+    g64, = torch.autograd.grad((uN64 * m * y.cpu().double()).sum(), x64)
+    # the GPU loss also sees ghost values of u_N (copied from u2); compare the
+    # gradient of the interior-weighted loss
+    bc = LegacyGridBoundary(u0, dx, 1)
+    ro = HeatRollout(bc, dx, 1, alpha, dt, steps, checkpoint_every=10)
+    x = u0.clone().requires_grad_(True)
+    g32, = torch.autograd.grad((ro(x) * m.float().to(dev) * y).sum(), x)
+    assert rel(g32, g64) < 1e-4
+
+
+def _wave_setup(dev, n=32, steps=120):
+    rng = np.random.default_rng(4)
+    shape = (n + 2, n + 2, 1)
+    dx = 1.0 / (n - 1)
+    c2 = (1 + 0.3 * rng.uniform(-1, 1, shape + (1,))).astype(np.float32)
+    dt = 0.5 * dx / np.sqrt(c2.max())
+    u0 = np.zeros(shape + (1,), np.float32)
+    v0 = np.zeros(shape + (1,), np.float32)
+    src = (shape[0] // 2) * shape[1] + shape[1] // 2
+    f0 = 0.1 / dx            # wavelength 10 cells; source peak at t0 = 15 dx
+    t = np.arange(steps) * dt
+    t0 = 1.5 / f0
+    a = (np.pi * f0 * (t - t0)) ** 2
+    wav = (100.0 * (1 - 2 * a) * np.exp(-a)).astype(np.float32)[:, None]
+    # receivers on the row next to the +X wall (the wall row itself is held at 0)
+    rec = np.array([(n - 1) * shape[1] + j for j in range(2, n, 6)], dtype=np.int64)
+    return shape, dx, dt, c2, u0, v0, np.array([src], np.int64), wav, rec
+
+
+def test_wave_rollout_gradient(dev):
+    steps = 120
+    shape, dx, dt, c2, u0, v0, src, wav, rec = _wave_setup(dev, steps=steps)
+    T = lambda a: torch.from_numpy(a).to(dev)
+    d = torch.randn(steps, len(rec), device=dev, generator=torch.Generator(device=dev).manual_seed(6)) * 1e-2
+    grads = []
+    for K in (7, 120):
+        bc = LegacyGridBoundary(T(u0), dx, 1)
+        ro = WaveRollout(bc, dx, 1, float(dt), steps, checkpoint_every=K, src_idx=T(src),
+                         src_wavelet=T(wav), rec_idx=T(rec))
+        c = T(c2).requires_grad_(True)
+        x = T(u0).requires_grad_(True)
+        _, _, r = ro(c, x, T(v0))
+        J = ((r - d) ** 2).sum()
+        gc, gx = torch.autograd.grad(J, (c, x))
+        grads.append((gc, gx, float(J)))
+    assert torch.equal(grads[0][0].view(torch.int32), grads[1][0].view(torch.int32))
+    assert torch.equal(grads[0][1].view(torch.int32), grads[1][1].view(torch.int32))
+    c64 = torch.from_numpy(c2).double().requires_grad_(True)
+    x64 = torch.from_numpy(u0).double().requires_grad_(True)
+    _, _, r64 = TF.wave_rollout(c64, x64, torch.from_numpy(v0).double(), dx, 1, float(dt), steps,
+                                src_idx=torch.from_numpy(src), src_wavelet=torch.from_numpy(wav).double(),
+                                rec_idx=torch.from_numpy(rec))
+    J64 = ((r64 - d.cpu().double()) ** 2).sum()
+    gc64, gx64 = torch.autograd.grad(J64, (c64, x64))
+    print(f"wave rollout: J32={grads[0][2]:.6e} J64={float(J64):.6e} "
+          f"rel err dJ/dc2={rel(grads[0][0], gc64):.2e} dJ/du0={rel(grads[0][1], gx64):.2e}", flush=True)
+    assert float(gc64.abs().max()) > 0 and float(r64.abs().max()) > 1e-3   # signal reached
+    assert abs(grads[0][2] - float(J64)) <= 1e-4 * abs(float(J64))
+    assert rel(grads[0][0], gc64) < 1e-4
+    assert rel(grads[0][1], gx64) < 1e-4
