@@ -124,6 +124,30 @@ int pde_wave_step_f32(const float *u, const float *v, const float *c2, float *u_
                       float w_minus, float w_center, float w_plus, float alpha, float dt,
                       int32_t first_step, void *stream);
 
+/* pde_wave_step_f32 with first_step = (*step == 0), read from device memory
+ * (a step counter advanced by pde_counter_add), so one captured graph of a loop
+ * segment serves every segment.  No reference counterpart (graph-capturable
+ * form of the step above). */
+int pde_wave_step_ctr_f32(const float *u, const float *v, const float *c2, float *u_next,
+                          float *v_next, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                          float w_minus, float w_center, float w_plus, float alpha, float dt,
+                          const int64_t *step, void *stream);
+
+/* out[step * stride + i] = field[idx[i]] for i < n, step read from device
+ * memory (receiver records of a captured loop).  No reference counterpart. */
+int pde_gather_step_f32(const float *field, const int64_t *idx, float *out, int64_t n,
+                        int64_t stride, const int64_t *step, void *stream);
+
+/* field[idx[i]] += scale * vals[step * stride + i] for i < n (distinct
+ * indices), step read from device memory (point sources of a captured loop).
+ * No reference counterpart. */
+int pde_scatter_add_step_f32(float *field, const int64_t *idx, const float *vals, int64_t n,
+                             int64_t stride, float scale, const int64_t *step, void *stream);
+
+/* *counter += delta on the device (advances the step counter inside a
+ * captured loop).  No reference counterpart. */
+int pde_counter_add(int64_t *counter, int64_t delta, void *stream);
+
 /* out[i] = field[idx[i]] for i < n (receiver samples of a time loop).  No
  * reference counterpart: the reference's examples read fields back with
  * .numpy() (e.g. examples/2D_Heat_Equation.py:64-65). */
@@ -230,6 +254,14 @@ int pde_wave_step_adjoint_f32(const float *gun, const float *gvn, const float *c
                               float w_center, float w_plus, float alpha, float dt,
                               int32_t first_step, void *stream);
 
+/* pde_wave_step_adjoint_f32 with first_step = (*step == 0) read from device
+ * memory (graph-capturable form; see pde_wave_step_ctr_f32). */
+int pde_wave_step_adjoint_ctr_f32(const float *gun, const float *gvn, const float *c2,
+                                  const float *u2, float *gu2, float *gv, float *gc2, int64_t nx,
+                                  int64_t ny, int64_t nz, int32_t ghost, float w_minus,
+                                  float w_center, float w_plus, float alpha, float dt,
+                                  const int64_t *step, void *stream);
+
 /* Transpose of the linear part of pde_bc_apply_f32: gin = gout, then
  * gin[zero_idx[i]] = 0 for every written scalar, then for each distinct
  * source src_idx[i] (flag bit 0 set when it is itself written):
@@ -239,6 +271,14 @@ int pde_bc_adjoint_f32(const float *gout, float *gin, int64_t n, const int64_t *
                        int64_t nzero, const int64_t *src_idx, const int32_t *src_flags,
                        const int64_t *off, const int64_t *cdst, const float *csign,
                        int64_t nsrc, void *stream);
+
+/* The same transpose applied to g in place: the new source values are first
+ * gathered into staging[nsrc] from the untouched g, then every written scalar
+ * is zeroed, then the staged values are stored.  No full-field copy. */
+int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int64_t nzero,
+                               const int64_t *src_idx, const int32_t *src_flags,
+                               const int64_t *off, const int64_t *cdst, const float *csign,
+                               int64_t nsrc, float *staging, void *stream);
 
 /* out[i] = scale * in[i] (the dt * g factor of ForwardEuler's transpose). */
 int pde_scale_f32(const float *in, float *out, int64_t n, float scale, void *stream);

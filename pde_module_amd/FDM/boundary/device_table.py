@@ -54,23 +54,43 @@ class DeviceBoundaryTable:
                N.stream_of(u))
         return u
 
+    def prepare_adjoint(self) -> None:
+        """Uploads the transpose arrays (once); call before capturing a graph
+        that applies the transpose."""
+        if self._adj is not None:
+            return
+        if not self.nops:
+            self._adj = (None, 0, None, None, None, None, None, 0)
+            return
+        from .tables import transpose_table
+        T = transpose_table(self.host_ops)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        self._adj = (dev(T["zero_idx"]), len(T["zero_idx"]), dev(T["src_idx"]), dev(T["flags"]),
+                     dev(T["off"]), dev(T["cdst"]), dev(T["csign"]), len(T["src_idx"]))
+        self._adj_staging = torch.empty(max(len(T["src_idx"]), 1), dtype=torch.float32,
+                                        device=self.device)
+
     def adjoint(self, gout: torch.Tensor, gin: torch.Tensor) -> torch.Tensor:
         """gin = transpose of the boundary's linear part applied to gout."""
         N.check_field(gout, "gout")
         N.check_field(gin, "gin")
         self._check_bounds(gout.numel())
-        if self._adj is None:
-            from .tables import transpose_table
-            T = transpose_table(self.host_ops) if self.nops else None
-            if T is None:
-                self._adj = (None, 0, None, None, None, None, None, 0)
-            else:
-                dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
-                self._adj = (dev(T["zero_idx"]), len(T["zero_idx"]), dev(T["src_idx"]),
-                             dev(T["flags"]), dev(T["off"]), dev(T["cdst"]), dev(T["csign"]),
-                             len(T["src_idx"]))
+        self.prepare_adjoint()
         z, nz, s, f, o, cd, cs, ns = self._adj
         p = lambda t: t.data_ptr() if t is not None else None
         N.call("pde_bc_adjoint_f32", N.dev_ptr(gout), N.dev_ptr(gin), gout.numel(), p(z), nz,
                p(s), p(f), p(o), p(cd), p(cs), ns, N.stream_of(gout))
         return gin
+
+    def adjoint_inplace(self, g: torch.Tensor) -> torch.Tensor:
+        """g <- transpose of the boundary's linear part applied to g (no copy)."""
+        N.check_field(g, "g")
+        self._check_bounds(g.numel())
+        self.prepare_adjoint()
+        z, nz, s, f, o, cd, cs, ns = self._adj
+        if nz == 0 and ns == 0:
+            return g
+        p = lambda t: t.data_ptr() if t is not None else None
+        N.call("pde_bc_adjoint_inplace_f32", N.dev_ptr(g), p(z), nz, p(s), p(f), p(o), p(cd),
+               p(cs), ns, self._adj_staging.data_ptr(), N.stream_of(g))
+        return g

@@ -40,7 +40,15 @@ SIGNATURES = {
                                   _F, _F, _F, _I32, _P],
     "pde_bc_adjoint_f32": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
+    "pde_bc_adjoint_inplace_f32": [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P],
     "pde_gather_f32": [_P, _P, _P, _I64, _P],
+    "pde_wave_step_ctr_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
+                              _P, _P],
+    "pde_wave_step_adjoint_ctr_f32": [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F,
+                                      _F, _F, _F, _F, _P, _P],
+    "pde_gather_step_f32": [_P, _P, _P, _I64, _I64, _P, _P],
+    "pde_scatter_add_step_f32": [_P, _P, _P, _I64, _I64, _F, _P, _P],
+    "pde_counter_add": [_P, _I64, _P],
     "pde_scatter_add_f32": [_P, _P, _P, _I64, _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],

@@ -199,8 +199,9 @@ __global__ __launch_bounds__(256) void wave_adj_kernel(
     const float *__restrict__ gun, const float *__restrict__ gvn, const float *__restrict__ c2,
     const float *__restrict__ u2, float *__restrict__ gu2, float *__restrict__ gv,
     float *__restrict__ gc2, AdjGeom g, float wm, float wc, float wp, float alpha, float dt,
-    int first)
+    int first, const int64_t *__restrict__ step)
 {
+    if (step) first = (*step == 0);
     PDE_ADJ_LOOP_BEGIN
     auto gvt_at = [&](int64_t x) {
         return first ? gvn[x] : gvn[x] + dt * gun[x];
@@ -264,6 +265,30 @@ __global__ __launch_bounds__(256) void bc_adj_gather_kernel(
     float acc = (flags[i] & 1) ? 0.0f : gout[s];
     for (int64_t q = off[i]; q < off[i + 1]; ++q) acc += csign[q] * gout[cdst[q]];
     gin[s] = acc;
+}
+
+// In-place form: gather the new source values into staging (reading the
+// untouched g), then zero every dst, then write the staged values.
+__global__ __launch_bounds__(256) void bc_adj_gather_stage_kernel(
+    const float *__restrict__ g, const int64_t *__restrict__ src,
+    const int32_t *__restrict__ flags, const int64_t *__restrict__ off,
+    const int64_t *__restrict__ cdst, const float *__restrict__ csign, int64_t nsrc,
+    float *__restrict__ staging)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nsrc) return;
+    float acc = (flags[i] & 1) ? 0.0f : g[src[i]];
+    for (int64_t q = off[i]; q < off[i + 1]; ++q) acc += csign[q] * g[cdst[q]];
+    staging[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void bc_adj_write_kernel(float *__restrict__ g,
+                                                           const int64_t *__restrict__ src,
+                                                           const float *__restrict__ staging,
+                                                           int64_t nsrc)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nsrc) g[src[i]] = staging[i];
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(const float *in, float *out, int64_t n, float s)
@@ -343,17 +368,19 @@ extern "C" int pde_heat_step_adjoint_f32(const float *gn, float *gu2, int64_t nx
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
-    AdjGeom g = adj_geom(G, ncomp);
     hipStream_t s = as_stream(stream);
+    st = step2d_launch(STEP2D_HEAT_ADJ, G, ncomp, gn, nullptr, nullptr, nullptr, gu2, nullptr,
+                       nullptr, wm, wc, wp, alpha, dt, 0, nullptr, ghost, s);
+    if (st != STEP2D_NA) return st;
+    AdjGeom g = adj_geom(G, ncomp);
     PDE_ADJ_DISPATCH(heat_adj_kernel, G.ndim, adj_grid(g), gn, gu2, g, wm, wc, wp, alpha, dt)
     return launch_status();
 }
 
-extern "C" int pde_wave_step_adjoint_f32(const float *gun, const float *gvn, const float *c2,
-                                         const float *u2, float *gu2, float *gv, float *gc2,
-                                         int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
-                                         float wm, float wc, float wp, float alpha, float dt,
-                                         int32_t first_step, void *stream)
+static int wave_adj_impl(const float *gun, const float *gvn, const float *c2, const float *u2,
+                         float *gu2, float *gv, float *gc2, int64_t nx, int64_t ny, int64_t nz,
+                         int32_t ghost, float wm, float wc, float wp, float alpha, float dt,
+                         int32_t first_step, const int64_t *step, void *stream)
 {
     if (!gun || !gvn || !gu2 || !gv) return PDE_EINVAL;
     if (gc2 && !u2) return PDE_EINVAL;
@@ -367,12 +394,36 @@ extern "C" int pde_wave_step_adjoint_f32(const float *gun, const float *gvn, con
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
-    AdjGeom g = adj_geom(G, 1);
     hipStream_t s = as_stream(stream);
     const int f = first_step ? 1 : 0;
+    st = step2d_launch(STEP2D_WAVE_ADJ, G, 1, gun, gvn, c2, u2, gu2, gv, gc2, wm, wc, wp, alpha,
+                       dt, f, step, ghost, s);
+    if (st != STEP2D_NA) return st;
+    AdjGeom g = adj_geom(G, 1);
     PDE_ADJ_DISPATCH(wave_adj_kernel, G.ndim, adj_grid(g), gun, gvn, c2, u2, gu2, gv, gc2, g, wm,
-                     wc, wp, alpha, dt, f)
+                     wc, wp, alpha, dt, f, step)
     return launch_status();
+}
+
+extern "C" int pde_wave_step_adjoint_f32(const float *gun, const float *gvn, const float *c2,
+                                         const float *u2, float *gu2, float *gv, float *gc2,
+                                         int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                         float wm, float wc, float wp, float alpha, float dt,
+                                         int32_t first_step, void *stream)
+{
+    return wave_adj_impl(gun, gvn, c2, u2, gu2, gv, gc2, nx, ny, nz, ghost, wm, wc, wp, alpha, dt,
+                         first_step, nullptr, stream);
+}
+
+extern "C" int pde_wave_step_adjoint_ctr_f32(const float *gun, const float *gvn, const float *c2,
+                                             const float *u2, float *gu2, float *gv, float *gc2,
+                                             int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                             float wm, float wc, float wp, float alpha, float dt,
+                                             const int64_t *step, void *stream)
+{
+    if (!step) return PDE_EINVAL;
+    return wave_adj_impl(gun, gvn, c2, u2, gu2, gv, gc2, nx, ny, nz, ghost, wm, wc, wp, alpha, dt,
+                         0, step, stream);
 }
 
 extern "C" int pde_copy_f32(const float *in, float *out, int64_t n, void *stream);
@@ -398,6 +449,38 @@ extern "C" int pde_bc_adjoint_f32(const float *gout, float *gin, int64_t n,
     if (nsrc > 0) {
         hipLaunchKernelGGL(bc_adj_gather_kernel, dim3((unsigned)((nsrc + 255) / 256)), dim3(256),
                            0, s, gout, gin, src_idx, src_flags, off, cdst, csign, nsrc);
+        st = launch_status();
+    }
+    return st;
+}
+
+extern "C" int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int64_t nzero,
+                                          const int64_t *src_idx, const int32_t *src_flags,
+                                          const int64_t *off, const int64_t *cdst,
+                                          const float *csign, int64_t nsrc, float *staging,
+                                          void *stream)
+{
+    if (!g || nzero < 0 || nsrc < 0) return PDE_EINVAL;
+    if ((nzero > 0 && !zero_idx) ||
+        (nsrc > 0 && (!src_idx || !src_flags || !off || !cdst || !csign || !staging)))
+        return PDE_EINVAL;
+    hipStream_t s = as_stream(stream);
+    int st = PDE_OK;
+    if (nsrc > 0) {
+        hipLaunchKernelGGL(bc_adj_gather_stage_kernel, dim3((unsigned)((nsrc + 255) / 256)),
+                           dim3(256), 0, s, g, src_idx, src_flags, off, cdst, csign, nsrc, staging);
+        st = launch_status();
+        if (st) return st;
+    }
+    if (nzero > 0) {
+        hipLaunchKernelGGL(bc_adj_zero_kernel, dim3((unsigned)((nzero + 255) / 256)), dim3(256), 0,
+                           s, g, zero_idx, nzero);
+        st = launch_status();
+        if (st) return st;
+    }
+    if (nsrc > 0) {
+        hipLaunchKernelGGL(bc_adj_write_kernel, dim3((unsigned)((nsrc + 255) / 256)), dim3(256),
+                           0, s, g, src_idx, staging, nsrc);
         st = launch_status();
     }
     return st;

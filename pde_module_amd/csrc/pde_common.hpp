@@ -73,4 +73,17 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblocks)
     return (b & 7u) * per + (b >> 3);
 }
 
+// 2D scalar row-tile kernels of the fused steps and their transposes
+// (steps2d.hip).  a/b/c2/u2 are inputs, o1/o2/gc2 outputs:
+//   HEAT:     a = u2, o1 = u_next
+//   WAVE:     a = u2, b = v, c2, o1 = u_next, o2 = v_next
+//   HEAT_ADJ: a = g_next, o1 = g_u2
+//   WAVE_ADJ: a = g_un, b = g_vn, c2, u2, o1 = g_u2, o2 = g_v, gc2 (+=)
+enum { STEP2D_HEAT = 0, STEP2D_WAVE = 1, STEP2D_WAVE_ADJ = 2, STEP2D_HEAT_ADJ = 3 };
+constexpr int STEP2D_NA = -100;
+int step2d_launch(int op, const Geom &G, int ncomp, const float *a, const float *b,
+                  const float *c2, const float *u2, float *o1, float *o2, float *gc2, float wm,
+                  float wc, float wp, float alpha, float dt, int first, const int64_t *step,
+                  int ghost, hipStream_t s);
+
 }  // namespace pde

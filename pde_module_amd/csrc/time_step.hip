@@ -127,10 +127,11 @@ __global__ __launch_bounds__(256) void wave_step_kernel(const float *__restrict_
                                                         float *__restrict__ un, float *vn,
                                                         StepGeom g, float wm, float wc,
                                                         float wp, float alpha, float dt,
-                                                        int first)
+                                                        int first, const int64_t *step)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= g.R) return;
+    if (step) first = (*step == 0);
     const bool in2 = r >= g.lo2 && r < g.hi2;
     for (int64_t i0 = blockIdx.z; i0 < g.P0; i0 += gridDim.z)
         for (int64_t i1 = blockIdx.y; i1 < g.P1; i1 += gridDim.y) {
@@ -193,8 +194,11 @@ extern "C" int pde_heat_step_f32(const float *u, float *un, int64_t nx, int64_t 
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
-    StepGeom g = step_geom(G, ncomp);
     hipStream_t s = as_stream(stream);
+    st = step2d_launch(STEP2D_HEAT, G, ncomp, u, nullptr, nullptr, nullptr, un, nullptr, nullptr,
+                       wm, wc, wp, alpha, dt, 0, nullptr, ghost, s);
+    if (st != STEP2D_NA) return st;
+    StepGeom g = step_geom(G, ncomp);
     const dim3 grid = step_grid(g), block(256);
     switch (G.ndim) {
     case 1: hipLaunchKernelGGL(heat_step_kernel<1>, grid, block, 0, s, u, un, g, wm, wc, wp, alpha, dt); break;
@@ -204,26 +208,49 @@ extern "C" int pde_heat_step_f32(const float *u, float *un, int64_t nx, int64_t 
     return launch_status();
 }
 
-extern "C" int pde_wave_step_f32(const float *u, const float *v, const float *c2, float *un,
-                                 float *vn, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
-                                 float wm, float wc, float wp, float alpha, float dt,
-                                 int32_t first_step, void *stream)
+static int wave_step_impl(const float *u, const float *v, const float *c2, float *un, float *vn,
+                          int64_t nx, int64_t ny, int64_t nz, int32_t ghost, float wm, float wc,
+                          float wp, float alpha, float dt, int32_t first_step,
+                          const int64_t *step, void *stream)
 {
     if (!u || !v || !un || !vn || u == un || un == vn || un == v || vn == u) return PDE_EINVAL;
     Geom G;
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
-    StepGeom g = step_geom(G, 1);
     hipStream_t s = as_stream(stream);
-    const dim3 grid = step_grid(g), block(256);
     const int f = first_step ? 1 : 0;
+    const int64_t *ctr = step;
+    st = step2d_launch(STEP2D_WAVE, G, 1, u, v, c2, nullptr, un, vn, nullptr, wm, wc, wp, alpha,
+                       dt, f, ctr, ghost, s);
+    if (st != STEP2D_NA) return st;
+    StepGeom g = step_geom(G, 1);
+    const dim3 grid = step_grid(g), block(256);
     switch (G.ndim) {
-    case 1: hipLaunchKernelGGL(wave_step_kernel<1>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f); break;
-    case 2: hipLaunchKernelGGL(wave_step_kernel<2>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f); break;
-    default: hipLaunchKernelGGL(wave_step_kernel<3>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f); break;
+    case 1: hipLaunchKernelGGL(wave_step_kernel<1>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f, ctr); break;
+    case 2: hipLaunchKernelGGL(wave_step_kernel<2>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f, ctr); break;
+    default: hipLaunchKernelGGL(wave_step_kernel<3>, grid, block, 0, s, u, v, c2, un, vn, g, wm, wc, wp, alpha, dt, f, ctr); break;
     }
     return launch_status();
+}
+
+extern "C" int pde_wave_step_f32(const float *u, const float *v, const float *c2, float *un,
+                                 float *vn, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                 float wm, float wc, float wp, float alpha, float dt,
+                                 int32_t first_step, void *stream)
+{
+    return wave_step_impl(u, v, c2, un, vn, nx, ny, nz, ghost, wm, wc, wp, alpha, dt, first_step,
+                          nullptr, stream);
+}
+
+extern "C" int pde_wave_step_ctr_f32(const float *u, const float *v, const float *c2, float *un,
+                                     float *vn, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                     float wm, float wc, float wp, float alpha, float dt,
+                                     const int64_t *step, void *stream)
+{
+    if (!step) return PDE_EINVAL;
+    return wave_step_impl(u, v, c2, un, vn, nx, ny, nz, ghost, wm, wc, wp, alpha, dt, 0, step,
+                          stream);
 }
 
 extern "C" int pde_version(void) { return 100; }
@@ -278,5 +305,64 @@ extern "C" int pde_scatter_add_f32(float *field, const int64_t *idx, const float
     if (n == 0) return PDE_OK;
     hipLaunchKernelGGL(scatter_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        as_stream(stream), field, idx, vals, n, scale);
+    return launch_status();
+}
+
+// Step-counter forms: the row of `vals` / `out` is chosen by a device-resident
+// step counter, so one captured graph of a loop segment can be replayed for
+// every segment.
+namespace pde {
+
+__global__ __launch_bounds__(256) void gather_step_kernel(const float *__restrict__ f,
+                                                          const int64_t *__restrict__ idx,
+                                                          float *__restrict__ out, int64_t n,
+                                                          int64_t stride,
+                                                          const int64_t *__restrict__ step)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[*step * stride + i] = f[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void scatter_add_step_kernel(
+    float *__restrict__ f, const int64_t *__restrict__ idx, const float *__restrict__ vals,
+    int64_t n, int64_t stride, float scale, const int64_t *__restrict__ step)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) f[idx[i]] = f[idx[i]] + scale * vals[*step * stride + i];
+}
+
+__global__ void counter_add_kernel(int64_t *c, int64_t delta)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + delta;
+}
+
+}  // namespace pde
+
+extern "C" int pde_gather_step_f32(const float *field, const int64_t *idx, float *out, int64_t n,
+                                   int64_t stride, const int64_t *step, void *stream)
+{
+    if (!field || !out || !step || n < 0 || stride < 0 || (n > 0 && !idx)) return PDE_EINVAL;
+    if (n == 0) return PDE_OK;
+    hipLaunchKernelGGL(gather_step_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), field, idx, out, n, stride, step);
+    return launch_status();
+}
+
+extern "C" int pde_scatter_add_step_f32(float *field, const int64_t *idx, const float *vals,
+                                        int64_t n, int64_t stride, float scale,
+                                        const int64_t *step, void *stream)
+{
+    if (!field || !step || n < 0 || stride < 0 || (n > 0 && (!idx || !vals))) return PDE_EINVAL;
+    if (n == 0) return PDE_OK;
+    hipLaunchKernelGGL(scatter_add_step_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), field, idx, vals, n, stride, scale, step);
+    return launch_status();
+}
+
+extern "C" int pde_counter_add(int64_t *counter, int64_t delta, void *stream)
+{
+    if (!counter) return PDE_EINVAL;
+    hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(64), 0, as_stream(stream), counter,
+                       delta);
     return launch_status();
 }

@@ -32,6 +32,14 @@ def _grid(x):
     return tuple(int(s) for s in x.shape[:3])
 
 
+GRAPH_CHUNK = 50   # steps per captured graph when every state is kept
+
+
+def _copy(src, dst):
+    """Device copy through the library's kernel (safe inside graph capture)."""
+    N.call("pde_copy_f32", src.data_ptr(), dst.data_ptr(), src.numel(), N.stream_of(src))
+
+
 class HeatRollout:
     """``HeatRollout(boundary, dx, ghost_cells, alpha, dt, steps, checkpoint_every)``;
     ``rollout(u0)`` returns u after ``steps`` steps, differentiable in u0."""
@@ -109,91 +117,207 @@ class _HeatFn(torch.autograd.Function):
 
 class WaveRollout:
     """``WaveRollout(boundary, dx, ghost_cells, dt, steps, checkpoint_every,
-    alpha=1, src_idx=None, src_wavelet=None, rec_idx=None)``.
+    alpha=1, src_idx=None, src_wavelet=None, rec_idx=None, use_graph=False)``.
 
     ``rollout(c2, u0, v0)`` returns ``(u_N, v_N, records)`` with ``records`` of
     shape (steps, n_receivers) (u after each step at the receivers' flat
     indices).  Per step n: boundary in place on u; ``v[src] += dt *
     src_wavelet[n]``; ``v_next = v + dt*(c2*(alpha*lap(u2)))``; ``u_next = u2 +
     dt*(v if n == 0 else v_next)``; sample u_next.  Differentiable in c2, u0
-    and v0."""
+    and v0.
+
+    Every per-step input that changes with n (the source row, the receiver
+    row, the first-step flag) is read through a device-resident step counter,
+    so with ``use_graph=True`` one HIP graph per loop segment (forward,
+    recompute, adjoint) is captured once and replayed for every segment; the
+    eager path runs the same kernel sequence."""
 
     def __init__(self, boundary, dx, ghost_cells, dt, steps, checkpoint_every=50, alpha=1.0,
-                 src_idx=None, src_wavelet=None, rec_idx=None):
+                 src_idx=None, src_wavelet=None, rec_idx=None, use_graph=False):
         self.bc = boundary
         self.w = laplacian_weights(dx)
         self.g = int(ghost_cells)
         self.alpha = np.float32(alpha)
         self.dt = np.float32(dt)
         self.steps = int(steps)
-        self.K = max(1, int(checkpoint_every))
+        # checkpoint_every=None keeps every step's state (no recompute in the
+        # backward; steps + 1 fields of memory)
+        self.store_all = checkpoint_every is None
+        self.K = GRAPH_CHUNK if self.store_all else max(1, min(int(checkpoint_every), self.steps))
+        self.K = min(self.K, self.steps)
         self.src_idx, self.src_wavelet, self.rec_idx = src_idx, src_wavelet, rec_idx
-        if src_idx is not None and (src_wavelet is None or src_wavelet.shape[0] < self.steps):
-            raise ValueError("src_wavelet needs one row per step")
+        if src_idx is not None:
+            if src_wavelet is None or src_wavelet.dim() != 2 or src_wavelet.shape[0] < self.steps \
+                    or src_wavelet.shape[1] != src_idx.numel():
+                raise ValueError("src_wavelet must be (steps, n_sources)")
+            self.src_wavelet = src_wavelet.contiguous()
+        self.use_graph = bool(use_graph)
+        self._shape = None
+        self._c2 = None
+        self._graphs = {}
 
-    def _step(self, u, v, c2, out, n, rec):
+    # ---------------------------------------------------------------- buffers
+    def _prepare(self, like, c2):
+        # device tables must exist before any graph capture (no uploads inside)
+        self.bc._ensure_table(like)
+        self.bc.table.prepare_adjoint()
+        shape = tuple(like.shape)
+        if self._shape == shape and self._dev == like.device and \
+                (self._c2 is None) == (c2 is None):
+            return
+        dev = like.device
+        self._shape, self._dev = shape, dev
+        E = lambda *s: torch.empty(s + shape, dtype=like.dtype, device=dev)
+        self._a, self._b, self._v = E(), E(), E()
+        self._hist = E((self.steps if self.store_all else self.K) + 1)
+        self._vt = E()
+        self._lu, self._lv, self._gu2, self._gvn = E(), E(), E(), E()
+        self._gc2 = E()
+        self._c2 = E() if c2 is not None else None
+        nseg = (self.steps + self.K - 1) // self.K
+        if self.store_all:
+            self._cu = self._cv = None
+        else:
+            self._cu, self._cv = E(nseg), E(nseg)
+        nrec = self.rec_idx.numel() if self.rec_idx is not None else 0
+        self._nrec = nrec
+        self._rec = torch.empty((self.steps, max(nrec, 1)), dtype=like.dtype, device=dev)
+        self._grec = torch.zeros_like(self._rec)
+        self._ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._graphs = {}
+
+    def _set_counter(self, n):
+        self._ctr.fill_(int(n))
+
+    # ---------------------------------------------------------- step pieces
+    def _step(self, u, v, c2, out, record):
         st = N.stream_of(u)
         self.bc.apply_inplace(u)
         if self.src_idx is not None:
-            N.call("pde_scatter_add_f32", v.data_ptr(), self.src_idx.data_ptr(),
-                   self.src_wavelet[n].data_ptr(), self.src_idx.numel(), self.dt, st)
-        N.call("pde_wave_step_f32", u.data_ptr(), v.data_ptr(),
+            N.call("pde_scatter_add_step_f32", v.data_ptr(), self.src_idx.data_ptr(),
+                   self.src_wavelet.data_ptr(), self.src_idx.numel(), self.src_idx.numel(),
+                   self.dt, self._ctr.data_ptr(), st)
+        N.call("pde_wave_step_ctr_f32", u.data_ptr(), v.data_ptr(),
                c2.data_ptr() if c2 is not None else None, out.data_ptr(), v.data_ptr(),
-               *_grid(u), self.g, *self.w, self.alpha, self.dt, 1 if n == 0 else 0, st)
-        if rec is not None:
-            N.call("pde_gather_f32", out.data_ptr(), self.rec_idx.data_ptr(), rec[n].data_ptr(),
-                   self.rec_idx.numel(), st)
+               *_grid(u), self.g, *self.w, self.alpha, self.dt, self._ctr.data_ptr(), st)
+        if record and self._nrec:
+            N.call("pde_gather_step_f32", out.data_ptr(), self.rec_idx.data_ptr(),
+                   self._rec.data_ptr(), self._nrec, self._rec.shape[1], self._ctr.data_ptr(), st)
+        N.call("pde_counter_add", self._ctr.data_ptr(), 1, st)
 
-    def forward_states(self, c2, u0, v0):
-        a, b, v = u0.clone(), torch.empty_like(u0), v0.clone()
-        nrec = self.rec_idx.numel() if self.rec_idx is not None else 0
-        rec = torch.empty((self.steps, max(nrec, 1)), dtype=u0.dtype, device=u0.device) if nrec else None
-        ckpt = {0: (u0.clone(), v0.clone())}
-        for n in range(self.steps):
-            self._step(a, v, c2, b, n, rec)
+    def _fwd_segment(self, L, c2):
+        """L steps from self._a / self._v; the state ends in self._a."""
+        a, b = self._a, self._b
+        for _ in range(L):
+            self._step(a, self._v, c2, b, True)
             a, b = b, a
-            if (n + 1) % self.K == 0 and n + 1 < self.steps:
-                ckpt[n + 1] = (a.clone(), v.clone())
-        if rec is None:
-            rec = torch.zeros((self.steps, 0), dtype=u0.dtype, device=u0.device)
-        return a, v, rec, ckpt
+        if a is not self._a:
+            _copy(a, self._a)
 
-    def backward_states(self, c2, ckpt, gu, gv, grec, need_c2):
-        st = N.stream_of(gu)
-        shape = gu.shape
-        lu, lv = gu.clone(), gv.clone()
-        gu2, gvn = torch.empty_like(lu), torch.empty_like(lu)
-        gc2 = torch.zeros_like(lu) if need_c2 else None
-        starts = sorted(ckpt)
-        hist = None
-        vtmp = torch.empty_like(lu)
-        utmp = torch.empty_like(lu)
-        for s_i in reversed(range(len(starts))):
-            n0 = starts[s_i]
-            n1 = starts[s_i + 1] if s_i + 1 < len(starts) else self.steps
-            L = n1 - n0
-            if hist is None or hist.shape[0] < L:
-                hist = torch.empty((L,) + tuple(shape), dtype=lu.dtype, device=lu.device)
-            u_c, v_c = ckpt[n0]
-            hist[0].copy_(u_c)
-            vtmp.copy_(v_c)
-            for i in range(L):
-                n = n0 + i
-                nxt = hist[i + 1] if i + 1 < L else utmp
-                self._step(hist[i], vtmp, c2, nxt, n, None)
-            # after the recompute hist[i] holds u2 of step n0 + i
-            for i in reversed(range(L)):
-                n = n0 + i
-                if grec is not None and self.rec_idx is not None:
-                    N.call("pde_scatter_add_f32", lu.data_ptr(), self.rec_idx.data_ptr(),
-                           grec[n].data_ptr(), self.rec_idx.numel(), np.float32(1.0), st)
-                N.call("pde_wave_step_adjoint_f32", lu.data_ptr(), lv.data_ptr(),
-                       c2.data_ptr() if c2 is not None else None, hist[i].data_ptr(),
-                       gu2.data_ptr(), gvn.data_ptr(), gc2.data_ptr() if gc2 is not None else None,
-                       *_grid(lu), self.g, *self.w, self.alpha, self.dt, 1 if n == 0 else 0, st)
-                self.bc.table.adjoint(gu2, lu)
-                lv, gvn = gvn, lv
-        return gc2, lu, lv
+    def _hist_segment(self, i0, L, c2, record):
+        """L steps from hist[i0] / vt writing hist[i0 + i + 1]; afterwards
+        hist[i0 + i] holds u2 of that step."""
+        for i in range(i0, i0 + L):
+            self._step(self._hist[i], self._vt, c2, self._hist[i + 1], record)
+
+    def _adj_segment(self, i0, L, c2, need_c2):
+        """L transposed steps on hist[i0 + L - 1] .. hist[i0], last to first.
+        The gradients live in lu / lv on entry and on exit."""
+        st = N.stream_of(self._lu)
+        lu, lv, gu2, gvn = self._lu, self._lv, self._gu2, self._gvn
+        N.call("pde_counter_add", self._ctr.data_ptr(), -1, st)
+        for i in reversed(range(i0, i0 + L)):
+            if self._nrec:
+                N.call("pde_scatter_add_step_f32", lu.data_ptr(), self.rec_idx.data_ptr(),
+                       self._grec.data_ptr(), self._nrec, self._grec.shape[1], np.float32(1.0),
+                       self._ctr.data_ptr(), st)
+            N.call("pde_wave_step_adjoint_ctr_f32", lu.data_ptr(), lv.data_ptr(),
+                   c2.data_ptr() if c2 is not None else None, self._hist[i].data_ptr(),
+                   gu2.data_ptr(), gvn.data_ptr(),
+                   self._gc2.data_ptr() if need_c2 else None, *_grid(lu), self.g, *self.w,
+                   self.alpha, self.dt, self._ctr.data_ptr(), st)
+            self.bc.table.adjoint_inplace(gu2)
+            lu, gu2 = gu2, lu
+            lv, gvn = gvn, lv
+            N.call("pde_counter_add", self._ctr.data_ptr(), -1, st)
+        if lu is not self._lu:
+            _copy(lu, self._lu)
+            _copy(lv, self._lv)
+
+    def _run(self, key, fn, *args):
+        """Run fn eagerly, or through a captured graph replayed per call."""
+        if not self.use_graph:
+            fn(*args)
+            return
+        gr = self._graphs.get(key)
+        if gr is None:
+            torch.cuda.synchronize(self._dev)
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr):
+                fn(*args)
+            self._graphs[key] = gr
+        gr.replay()
+
+    # ---------------------------------------------------------------- passes
+    def forward_states(self, c2, u0, v0):
+        self._prepare(u0, c2)
+        if c2 is not None:
+            self._c2.copy_(c2)
+            c2 = self._c2
+        self._set_counter(0)
+        nseg = (self.steps + self.K - 1) // self.K
+        if self.store_all:
+            self._hist[0].copy_(u0)
+            self._vt.copy_(v0)
+            for s in range(nseg):
+                n0 = s * self.K
+                L = min(self.K, self.steps - n0)
+                self._run(("fwd_all", s, c2 is None), self._hist_segment, n0, L, c2, True)
+            uN, vN = self._hist[self.steps], self._vt
+        else:
+            self._a.copy_(u0)
+            self._v.copy_(v0)
+            for s in range(nseg):
+                L = min(self.K, self.steps - s * self.K)
+                self._cu[s].copy_(self._a)
+                self._cv[s].copy_(self._v)
+                self._run(("fwd", L, c2 is None), self._fwd_segment, L, c2)
+            uN, vN = self._a, self._v
+        rec = self._rec[:, :self._nrec] if self._nrec else self._rec[:, :0]
+        return uN.clone(), vN.clone(), rec.clone()
+
+    def backward_states(self, c2, gu, gv, grec, need_c2):
+        self.bc.table.prepare_adjoint()
+        if c2 is not None:
+            self._c2.copy_(c2)
+            c2 = self._c2
+        self._lu.copy_(gu)
+        self._lv.copy_(gv)
+        if self._nrec:
+            if grec is None:
+                self._grec.zero_()
+            else:
+                self._grec[:, :self._nrec].copy_(grec)
+        if need_c2:
+            self._gc2.zero_()
+        nseg = (self.steps + self.K - 1) // self.K
+        if self.store_all:
+            for s in reversed(range(nseg)):
+                n0 = s * self.K
+                L = min(self.K, self.steps - n0)
+                self._set_counter(n0 + L)
+                self._run(("adj_all", s, c2 is None, need_c2), self._adj_segment, n0, L, c2,
+                          need_c2)
+        else:
+            for s in reversed(range(nseg)):
+                n0 = s * self.K
+                L = min(self.K, self.steps - n0)
+                self._hist[0].copy_(self._cu[s])
+                self._vt.copy_(self._cv[s])
+                self._set_counter(n0)
+                self._run(("rec", L, c2 is None), self._hist_segment, 0, L, c2, False)
+                self._run(("adj", L, c2 is None, need_c2), self._adj_segment, 0, L, c2, need_c2)
+        return (self._gc2.clone() if need_c2 else None), self._lu.clone(), self._lv.clone()
 
     def __call__(self, c2, u0, v0):
         self.bc._ensure_table(u0)
@@ -207,18 +331,18 @@ class _WaveFn(torch.autograd.Function):
         N.check_field(v0, "v0")
         if c2 is not None:
             N.check_field(c2, "c2")
-        uN, vN, rec, ckpt = ro.forward_states(c2.detach() if c2 is not None else None,
-                                              u0.detach(), v0.detach())
-        ctx.ro, ctx.ckpt = ro, ckpt
-        ctx.c2 = c2.detach() if c2 is not None else None
+        c2d = c2.detach() if c2 is not None else None
+        uN, vN, rec = ro.forward_states(c2d, u0.detach(), v0.detach())
+        ctx.ro, ctx.c2 = ro, c2d
         ctx.need_c2 = c2 is not None and ctx.needs_input_grad[0]
+        ctx.like = u0
         return uN, vN, rec
 
     @staticmethod
     def backward(ctx, guN, gvN, grec):
-        ref = ctx.ckpt[0][0]
+        ref = ctx.like
         guN = torch.zeros_like(ref) if guN is None else guN.contiguous()
         gvN = torch.zeros_like(ref) if gvN is None else gvN.contiguous()
         grec = grec.contiguous() if grec is not None and grec.numel() else None
-        gc2, gu0, gv0 = ctx.ro.backward_states(ctx.c2, ctx.ckpt, guN, gvN, grec, ctx.need_c2)
+        gc2, gu0, gv0 = ctx.ro.backward_states(ctx.c2, guN, gvN, grec, ctx.need_c2)
         return gc2, gu0, gv0, None

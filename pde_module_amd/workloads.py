@@ -57,3 +57,62 @@ def operator_3d_fields(seed=0, n=512, ghost_cells=1):
     s[g:P - g, g:P - g, g:P - g, 0] = rng.uniform(-1, 1, size=(n, n, n))
     v[g:P - g, g:P - g, g:P - g, :] = rng.uniform(-1, 1, size=(n, n, n, 3))
     return s, v
+
+
+def smoothed_field(rng, n, corr):
+    """Gaussian random field on an n x n periodic grid: white noise filtered by a
+    Gaussian of standard deviation ``corr`` cells (in Fourier space),
+    normalised to max |g| = 1."""
+    noise = rng.standard_normal((n, n))
+    k = np.fft.fftfreq(n)
+    kx, ky = np.meshgrid(k, np.fft.rfftfreq(n), indexing="ij")
+    filt = np.exp(-2.0 * (np.pi * corr) ** 2 * (kx ** 2 + ky ** 2))
+    g = np.fft.irfft2(np.fft.rfft2(noise) * filt, s=(n, n))
+    return g / np.max(np.abs(g))
+
+
+def ricker(t, f0, t0):
+    a = (np.pi * f0 * (t - t0)) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+@dataclass
+class Wave2D:
+    n: int
+    g: int
+    dx: float
+    dt: float
+    steps: int
+    c2: np.ndarray          # (n+2g, n+2g, 1, 1) float32, squared wave speed per node
+    src_idx: np.ndarray     # (1,) flat node index (centre)
+    wavelet: np.ndarray     # (steps, 1) float32 source samples
+    rec_idx: np.ndarray     # (nrec,) flat node indices, row next to the +Y wall
+    f0: float
+
+
+def wave_2d_config(seed=0, n=2048, steps=2000, nrec=64, corr=32.0, ghost_cells=1,
+                   perturbation=0.3):
+    """2D acoustic workload: c = 1 + perturbation * g (g a smoothed Gaussian
+    random field, correlation length ``corr`` cells), dt = 0.5 dx / c_max (CFL
+    0.5), Ricker source at the centre with f0 = c_min / (20 dx) (20 cells per
+    wavelength) and t0 = 1.5 / f0, ``nrec`` receivers evenly spaced on the row
+    next to the +Y wall, Dirichlet-0 walls.  ``perturbation=0`` gives the
+    unperturbed model (c = 1)."""
+    rng = np.random.default_rng(seed)
+    g = ghost_cells
+    P = n + 2 * g
+    dx = 1.0 / (n - 1)
+    field = smoothed_field(rng, n, corr) if perturbation else np.zeros((n, n))
+    c = 1.0 + perturbation * field
+    cpad = np.pad(c, g, mode="edge")
+    c2 = (cpad ** 2).astype(np.float32).reshape(P, P, 1, 1)
+    cmax, cmin = float(c.max()), float(c.min())
+    dt = 0.5 * dx / cmax
+    f0 = cmin / (20.0 * dx)
+    t = np.arange(steps) * dt
+    wav = ricker(t, f0, 1.5 / f0).astype(np.float32)[:, None]
+    src = np.array([(g + n // 2) * P + (g + n // 2)], dtype=np.int64)
+    cols = np.linspace(g + 1, g + n - 2, nrec).round().astype(np.int64)
+    row = P - 2 - g                                # next to the +Y wall
+    rec = cols * P + row
+    return Wave2D(n, g, dx, dt, steps, c2, src, wav, rec, f0)
