@@ -283,6 +283,51 @@ int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int64_t nzero,
 /* out[i] = scale * in[i] (the dt * g factor of ForwardEuler's transpose). */
 int pde_scale_f32(const float *in, float *out, int64_t n, float scale, void *stream);
 
+/* ------------------------------------------------------------ node-wise glue */
+/* The element-wise operations between stencil outputs.  `nodes` counts padded
+ * nodes (ghosts included: the reference's element-wise launches cover the
+ * whole array); per-node values are contiguous (AoS). */
+
+/* out[n][i][j] = a[n][i] * b[n][j]; a is (nodes, la), b (nodes, lb), out
+ * (nodes, la, lb).  Replaces OuterProduct (pm/FDM/module/elementWiseOps.py:10-49,
+ * element op wp.outer at :18) launched through elementWiseOp
+ * (pm/FDM/functional/_elementWiseOps.py:5-31). */
+int pde_outer_f32(const float *a, const float *b, float *out, int64_t nodes, int32_t la,
+                  int32_t lb, void *stream);
+
+/* out[n][k] = s[n] * b[n][k]; s is a (nodes, 1) field.  Replaces
+ * scalarVectorMult (pm/FDM/module/elementWiseOps.py:52-89; element op
+ * pm/FDM/kernel/elementWiseOps.py:6-23).  out may alias b. */
+int pde_scalar_mul_f32(const float *s, const float *b, float *out, int64_t nodes, int32_t lb,
+                       void *stream);
+
+/* out[n][k] = b[n][k] / s[n] (IEEE division).  Replaces the MapWise map
+ * `lambda m, rho: m / rho[0]` (examples/LDC/LDC_RK2.py:117,
+ * pm/stencil/mapWise.py:8-44).  out may alias b. */
+int pde_div_scalar_f32(const float *b, const float *s, float *out, int64_t nodes, int32_t lb,
+                       void *stream);
+
+/* out[n][i] = sum_j m[n][i][j] * v[n][j], j ascending; m is (nodes, rows,
+ * cols) row-major.  Replaces the Warp array product `du * u` of a mat(N, D)
+ * field and a vec(D) field (examples/LDC/LDC_No_Graph.py:87). */
+int pde_matvec_f32(const float *m, const float *v, float *out, int64_t nodes, int32_t rows,
+                   int32_t cols, void *stream);
+
+/* out[i] = c0*x0[i] + c1*x1[i] + ... over k (1..4) terms, left to right, for
+ * i < n floats.  With coefficients +-1 this is exactly the chain of Warp
+ * array operators `a - b + c + d` (examples/LDC/LDC_No_Graph.py:90,
+ * examples/LDC/LDC_RK2.py:133) in one pass.  out may alias any x. */
+int pde_lincomb_f32(const float *x0, const float *x1, const float *x2, const float *x3, float c0,
+                    float c1, float c2, float c3, int32_t k, float *out, int64_t n,
+                    void *stream);
+
+/* out[n][c] = scale * (x[n][c] - shift) on component comp (every component
+ * when comp < 0), other components copied.  Replaces the MapWise map
+ * get_p_op, rho -> c2 * (rho - density) (examples/LDC/LDC_RK2.py:111-116).
+ * out may alias x. */
+int pde_affine_f32(const float *x, float *out, int64_t nodes, int32_t ncomp, int32_t comp,
+                   float scale, float shift, void *stream);
+
 /* ------------------------------------------------------------------- library */
 
 /* Library version (major*10000 + minor*100 + patch). */

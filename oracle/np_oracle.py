@@ -287,6 +287,60 @@ def wave_step(u, v, w, c2, g, dt_step, bc, first):
     return u_next, v_next
 
 
+# --------------------------------------------------------------------------- node-wise glue
+# Element-wise operations over every padded node (ghosts included), as the
+# reference's element-wise launches cover the whole array.
+def outer(a, b):
+    """wp.outer per node: (..., la), (..., lb) -> (..., la, lb), M[i][j] = a[i]*b[j]
+    (reference ``pm/FDM/module/elementWiseOps.py:18``)."""
+    return a[..., :, None] * b[..., None, :]
+
+
+def scalar_vector_multiply(s, b):
+    """s[0] * b per node, s a vec(1) field (reference
+    ``pm/FDM/kernel/elementWiseOps.py:17-21``)."""
+    sc = s.reshape(s.shape[:3] + (1,) * (b.ndim - 3))
+    return sc * b
+
+
+def divide_by_scalar(m, rho):
+    """m / rho[0] per node (reference ``examples/LDC/LDC_RK2.py:117``)."""
+    return m / rho.reshape(rho.shape[:3] + (1,) * (m.ndim - 3))
+
+
+def matvec(m, v):
+    """Warp mat * vec per node: out[i] = m[i][0]*v[0] + m[i][1]*v[1] + ...,
+    columns ascending (the array product ``du * u``,
+    ``examples/LDC/LDC_No_Graph.py:87``)."""
+    acc = m[..., :, 0] * v[..., None, 0]
+    for j in range(1, m.shape[-1]):
+        acc = acc + m[..., :, j] * v[..., None, j]
+    return acc
+
+
+def linear_combination(*terms):
+    """c0*x0 + c1*x1 + ... left to right (the array-operator chain
+    ``dp - u_conv + u_diff``, ``examples/LDC/LDC_No_Graph.py:90``)."""
+    (c0, x0), rest = terms[0], terms[1:]
+    dt = x0.dtype.type
+    acc = dt(c0) * x0
+    for c, x in rest:
+        acc = acc + dt(c) * x
+    return acc
+
+
+def shifted_scale(x, scale, shift, comp=-1):
+    """scale * (x - shift) on component comp (all when comp < 0)
+    (``get_p_op``, ``examples/LDC/LDC_RK2.py:111-116``)."""
+    dt = x.dtype.type
+    y = dt(scale) * (x - dt(shift))
+    if comp < 0:
+        return y
+    out = x.copy()
+    out[..., comp] = y[..., comp]
+    return out
+
+
 # --------------------------------------------------------------------------- adjoints
 # Transposes of the operators above, seen as linear maps from the whole padded
 # field to its interior nodes (output ghosts are not part of the range).  Each

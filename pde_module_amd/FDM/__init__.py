@@ -1,7 +1,8 @@
 """Uniform-grid finite differences (reference ``pm/FDM/__init__.py:1-13``)."""
-from .module import Divergence, ExplicitUniformGridStencil, Grad, Laplacian
+from .module import (Divergence, ExplicitUniformGridStencil, Grad, Laplacian, OuterProduct,
+                     scalarVectorMult)
 from .boundary import GridBoundary, PeriodicBoundary
 from . import functional
 
 __all__ = ["Laplacian", "Grad", "Divergence", "GridBoundary", "PeriodicBoundary",
-           "ExplicitUniformGridStencil", "functional"]
+           "ExplicitUniformGridStencil", "OuterProduct", "scalarVectorMult", "functional"]

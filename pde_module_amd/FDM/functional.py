@@ -85,3 +85,26 @@ def fused_lap_grad_div(s, v, lap_out, grad_out, div_out, dx, ghost_cells,
            np.float32(alpha_lap), np.float32(alpha_grad), np.float32(alpha_div),
            N.stream_of(s))
     return lap_out, grad_out, div_out
+
+
+def elementWiseOp(kernel, array_A, array_B, output_array, device=None):
+    """Apply the element op ``kernel`` (an ``element_ops`` op or its name) to A
+    and B into ``output_array`` over every padded node (reference
+    ``pm/FDM/functional/_elementWiseOps.py:5-31``)."""
+    from ..stencil.elementWise import resolve_element_op
+    op = resolve_element_op(kernel)
+    N.check_field(array_A, "array_A")
+    N.check_field(array_B, "array_B")
+    N.check_field(output_array, "output_array")
+    want = op.out_shape(tuple(array_A.shape[3:]), tuple(array_B.shape[3:]))
+    if tuple(output_array.shape) != tuple(array_A.shape[:3]) + tuple(want) or \
+            tuple(array_B.shape[:3]) != tuple(array_A.shape[:3]):
+        raise AssertionError("element-wise shapes do not match")
+    op.launch(array_A, array_B, output_array)
+    return output_array
+
+
+def linear_combination(output_array, *terms):
+    """``output = c0*x0 + c1*x1 + ...`` (up to four ``(c, x)`` terms) in one pass."""
+    from ..stencil.element_ops import linear_combination as _lc
+    return _lc(output_array, *terms)

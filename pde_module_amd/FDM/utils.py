@@ -19,4 +19,8 @@ def get_ghost_shape(grid_shape, ghost_cells):
 
 
 def dimension_of(grid_shape):
-    return sum(1 for s in grid_shape if s > 1)
+    """Number of grid axes with more than one point.  Only the first three
+    entries are grid axes: a torch field's ``shape`` carries its element dims
+    after them (a Warp array's shape stops at the grid), so ``Grad(2, u.shape,
+    ...)`` means the same here as in the reference."""
+    return sum(1 for s in tuple(grid_shape)[:3] if s > 1)

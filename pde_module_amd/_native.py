@@ -40,6 +40,12 @@ SIGNATURES = {
                                   _F, _F, _F, _I32, _P],
     "pde_bc_adjoint_f32": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
+    "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
+    "pde_scalar_mul_f32": [_P, _P, _P, _I64, _I32, _P],
+    "pde_div_scalar_f32": [_P, _P, _P, _I64, _I32, _P],
+    "pde_matvec_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
+    "pde_lincomb_f32": [_P, _P, _P, _P, _F, _F, _F, _F, _I32, _P, _I64, _P],
+    "pde_affine_f32": [_P, _P, _I64, _I32, _I32, _F, _F, _P],
     "pde_bc_adjoint_inplace_f32": [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P],
     "pde_gather_f32": [_P, _P, _P, _I64, _P],
     "pde_wave_step_ctr_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,

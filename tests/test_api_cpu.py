@@ -82,3 +82,24 @@ def test_mesh_layout_contract():
     assert tuple(f.shape) == (7, 5, 1, 2)
     f = create_structured_field(m, "node", 1, func=lambda x, y, z: x + 10 * y, device="cpu")
     assert float(f[2, 3, 0, 0]) == pytest.approx(0.25 + 10 * 0.5)
+
+
+def test_grid_shape_may_carry_element_dims():
+    """Grad/Divergence take the field's shape; a torch shape ends with the
+    element dims, which are not grid axes."""
+    Grad(2, (23, 23, 1, 2), 0.05, 1)
+    Divergence(2, (23, 23, 1, 2), 0.05, 1)
+    with pytest.raises(AssertionError):
+        Divergence(3, (23, 23, 1, 3), 0.05, 1)
+
+
+def test_element_ops_reject_arbitrary_functions():
+    from pde_module_amd.stencil import ElementWise, MapWise, element_ops
+    with pytest.raises(TypeError):
+        ElementWise(lambda a, b: a + b)
+    with pytest.raises(TypeError):
+        MapWise("not_a_map")
+    assert ElementWise("outer").element_op is element_ops.outer
+    x = torch.zeros(4, 4, 1, 2)
+    with pytest.raises(RuntimeError):
+        ElementWise("add")(x, x)
