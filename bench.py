@@ -135,6 +135,74 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
             "note": "graph-replayed 50-step segments; recompute from checkpoints in the backward"}
 
 
+def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
+    """Batched heat workload: 512 independent 2D grids, sides log-uniform in
+    [64, 1024] (seed 3: 89.3 M interior points), diffusivity U[0.005, 0.05],
+    dt = 0.2 dx^2 / alpha, Dirichlet 0 walls (the legacy GridBoundary
+    default), 8 Gaussian bumps each.  The ensemble (autotuned tiles,
+    graph-replayed 50-step chunks) runs all 500 steps; the per-instance
+    baseline (one HeatStep per grid, launched grid by grid) is timed over
+    ``baseline_steps`` steps of every grid."""
+    import torch
+    from pde_module_amd.batch import HeatEnsemble, PerInstanceHeat
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    from pde_module_amd import _native as NV
+    from pde_module_amd.workloads import heat_ensemble_config
+    cfg = heat_ensemble_config(seed=seed)
+    count = len(cfg.n)
+    fields = [torch.from_numpy(cfg.u0(i, seed)).to(dev) for i in range(count)]
+    mk = lambda: [LegacyGridBoundary(f, float(cfg.dx[i]), cfg.g) for i, f in enumerate(fields)]
+    dxs, alphas, dts = (list(map(float, a)) for a in (cfg.dx, cfg.alpha, cfg.dt))
+    ens = HeatEnsemble(mk(), dxs, cfg.g, alphas, dts, device=dev)
+    ens.load(fields)
+    best = ens.autotune(steps=6, reps=2)
+    ens.load(fields)
+    ens.run(2 * ens.graph_chunk)               # capture + warm up
+    torch.cuda.synchronize()
+    ens.load(fields)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    # kernel-only time of the batched step (HIP events on the launch stream)
+    tiles, nt = ens._tiles(best)
+    a, b = ens.bufs
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        NV.call("pde_heat_batch_step_f32", a.data_ptr(), b.data_ptr(), ens.grid_dev.data_ptr(),
+                ens.n, tiles.data_ptr(), nt, best.rows_per_lane, best.waves, best.blocks,
+                NV.stream_of(a))
+    e1.record(stream)
+    e1.synchronize()
+    t_kernel = e0.elapsed_time(e1) / reps * 1e-3
+    padded = int((ens.p1 * ens.p2).sum())
+    ref = PerInstanceHeat(mk(), dxs, cfg.g, alphas, dts)
+    ref.load(fields)
+    ref.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref.run(baseline_steps)
+    torch.cuda.synchronize()
+    t_ref = (time.perf_counter() - t0) / baseline_steps
+    return {"instances": count, "interior_points": ens.points, "steps": steps,
+            "batched_ms_per_step": round(t_batch * 1e3, 4),
+            "per_instance_ms_per_step": round(t_ref * 1e3, 4),
+            "speedup": round(t_ref / t_batch, 2),
+            "batched_glups": round(ens.points / t_batch / 1e9, 1),
+            "tile_config": {"rows_per_lane": best.rows_per_lane, "waves": best.waves,
+                            "blocks": best.blocks, "xcd_slots": best.xcd_slots},
+            "autotune_ms_per_step": {f"{k[0]}x{k[1]}/{k[2]}/{k[3]}": round(v, 4)
+                                     for k, v in sorted(ens.timings.items())},
+            "step_kernel_us": round(t_kernel * 1e6, 1),
+            "step_kernel_GBs": round(8 * padded / t_kernel / 1e9, 1),
+            "note": "batched: boundary + step kernels for all grids per step, 50-step graphs; "
+                    "per-instance: HeatStep per grid (boundary + fused step launches), eager"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:

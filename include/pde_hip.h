@@ -283,6 +283,45 @@ int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int64_t nzero,
 /* out[i] = scale * in[i] (the dt * g factor of ForwardEuler's transpose). */
 int pde_scale_f32(const float *in, float *out, int64_t n, float scale, void *stream);
 
+/* ----------------------------------------------------------- batched grids */
+/* Many independent 2D scalar heat grids of uneven sizes, packed one after the
+ * other in one float buffer, stepped together.  The reference runs one grid
+ * per module call (examples/2D_Heat_Equation.py:58-62: boundary, Laplacian,
+ * ForwardEuler); an ensemble there is a Python loop over instances.  These
+ * entry points have no reference counterpart: they are the batched form of
+ * pde_bc_apply_inplace_f32 + pde_heat_step_f32, giving every grid the same
+ * bits as stepping it alone. */
+typedef struct pde_batch_grid {
+    int64_t offset;    /* first float of the grid's padded field in the buffer */
+    int64_t op_begin;  /* its boundary ops: [op_begin, op_end) of the packed  */
+    int64_t op_end;    /* table (dst/src are indices into the whole buffer)   */
+    int32_t p1, p2;    /* padded rows, padded columns (column index fastest)  */
+    int32_t ghost;     /* ghost width g (interior = [g, p - g) on both axes)  */
+    int32_t reserved0;
+    float wm, wc, wp;  /* Laplacian weights 1/dx^2, -2/dx^2, 1/dx^2           */
+    float alpha, dt;
+    float reserved1;
+} pde_batch_grid; /* 64 bytes */
+
+/* Boundary of every grid, in place: workgroup i applies grid i's ops
+ * (at most pde_batch_bc_capacity() of them) with the two-phase semantics of
+ * pde_bc_apply_inplace_f32.  t: device time scalar for ramp values, or NULL. */
+int pde_bc_apply_batch_inplace_f32(float *u, const pde_batch_grid *grids, int64_t ngrids,
+                                   const pde_bc_op *ops, const float *t, void *stream);
+
+/* Largest number of boundary ops of one grid in pde_bc_apply_batch_inplace_f32. */
+int pde_batch_bc_capacity(void);
+
+/* Fused heat step of every grid (un = u2 + dt * alpha * lap(u2) on interior
+ * nodes, u2 elsewhere) over a tile list: tiles[4*i .. 4*i+3] = (grid, first
+ * row, first 4-column chunk, segment width cw in {64, 32, 16}); a tile covers
+ * waves * (64 / cw) * rows_per_lane rows and 4 * cw columns.  rows_per_lane
+ * in {1, 2, 4, 8}, waves in {1, 2, 4}; blocks = 0 launches one block per
+ * tile, else `blocks` blocks loop over the tiles. */
+int pde_heat_batch_step_f32(const float *u, float *un, const pde_batch_grid *grids,
+                            int64_t ngrids, const int32_t *tiles, int64_t ntiles,
+                            int32_t rows_per_lane, int32_t waves, int64_t blocks, void *stream);
+
 /* ------------------------------------------------------------ node-wise glue */
 /* The element-wise operations between stencil outputs.  `nodes` counts padded
  * nodes (ghosts included: the reference's element-wise launches cover the

@@ -40,6 +40,9 @@ SIGNATURES = {
                                   _F, _F, _F, _I32, _P],
     "pde_bc_adjoint_f32": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
+    "pde_bc_apply_batch_inplace_f32": [_P, _P, _I64, _P, _P, _P],
+    "pde_batch_bc_capacity": [],
+    "pde_heat_batch_step_f32": [_P, _P, _P, _I64, _P, _I64, _I32, _I32, _I64, _P],
     "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
     "pde_scalar_mul_f32": [_P, _P, _P, _I64, _I32, _P],
     "pde_div_scalar_f32": [_P, _P, _P, _I64, _I32, _P],
@@ -65,6 +68,13 @@ BC_OP_DTYPE = np.dtype([("dst", "<i8"), ("src", "<i8"), ("kind", "<i4"),
                         ("value", "<f4"), ("coef", "<f4"), ("tmax", "<f4")])
 BC_SET, BC_COPY, BC_GHOST_D, BC_GHOST_N = 0, 1, 2, 3
 assert BC_OP_DTYPE.itemsize == 32
+
+# struct pde_batch_grid
+BATCH_GRID_DTYPE = np.dtype([("offset", "<i8"), ("op_begin", "<i8"), ("op_end", "<i8"),
+                             ("p1", "<i4"), ("p2", "<i4"), ("ghost", "<i4"),
+                             ("reserved0", "<i4"), ("wm", "<f4"), ("wc", "<f4"), ("wp", "<f4"),
+                             ("alpha", "<f4"), ("dt", "<f4"), ("reserved1", "<f4")])
+assert BATCH_GRID_DTYPE.itemsize == 64
 
 
 class PDEError(RuntimeError):

@@ -73,6 +73,34 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblocks)
     return (b & 7u) * per + (b >> 3);
 }
 
+// Value of one resolved boundary op (PDE_BC_* kinds) read from `in`; t is the
+// device time scalar of ramp values (null = 0).
+__device__ __forceinline__ float op_value(const pde_bc_op &op, const float *t)
+{
+    if (op.tmax > 0.0f) {
+        // wp.min(t / T_max, 1) * const  (boundary_functions.py:85-89).  A null
+        // t is the reference's default time array, zeros(1)
+        // (functional_boundary.py:237).
+        const float tv = t ? t[0] : 0.0f;
+        float ramp = tv / op.tmax;
+        ramp = ramp < 1.0f ? ramp : 1.0f;
+        return ramp * op.value;
+    }
+    return op.value;
+}
+
+__device__ __forceinline__ float op_eval(const pde_bc_op &op, const float *__restrict__ in,
+                                         const float *t)
+{
+    const float v = op_value(op, t);
+    switch (op.kind) {
+    case PDE_BC_SET: return v;
+    case PDE_BC_COPY: return in[op.src];
+    case PDE_BC_GHOST_D: return 2.0f * v - in[op.src];
+    default: return op.coef * v + in[op.src];
+    }
+}
+
 // 2D scalar row-tile kernels of the fused steps and their transposes
 // (steps2d.hip).  a/b/c2/u2 are inputs, o1/o2/gc2 outputs:
 //   HEAT:     a = u2, o1 = u_next

@@ -11,124 +11,12 @@
 //
 // Arithmetic is the same, operation for operation, as the generic kernels in
 // time_step.hip and adjoint.hip; results are identical bit for bit.
-#include "pde_common.hpp"
+#include "tile2d.hpp"
 
 namespace pde {
 
-typedef float v4 __attribute__((ext_vector_type(4)));
-typedef float v4u __attribute__((ext_vector_type(4), aligned(4)));
-
 constexpr int RX = 4;     // rows per lane
 constexpr int BY2 = 4;    // waves per block (stacked along rows)
-
-struct Tile2 {
-    int64_t P1, P2;       // padded rows, columns
-    int64_t g;            // ghost width
-    int64_t nq;           // 4-column chunks per row (ceil(P2 / 4))
-    uint32_t tq, tr;      // tiles along columns (64 chunks each) and rows (BY2 * RX each)
-};
-
-__device__ __forceinline__ bool inside(int64_t v, int64_t g, int64_t P) { return v >= g && v < P - g; }
-
-// Loads the 4 columns k0 .. k0+3 of row x (clamped to the array: columns past
-// P2 read as 0; rows outside [0, P1) read as 0).
-__device__ __forceinline__ v4 ld_row4(const float *__restrict__ f, const Tile2 &t, int64_t x,
-                                      int64_t k0)
-{
-    v4 r = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (x < 0 || x >= t.P1) return r;
-    const float *p = f + x * t.P2 + k0;
-    if (k0 + 4 <= t.P2) {
-        const v4u q = *reinterpret_cast<const v4u *>(p);
-        return v4{q.x, q.y, q.z, q.w};
-    }
-    const int64_t n = t.P2 - k0;
-    r.x = p[0];
-    if (n > 1) r.y = p[1];
-    if (n > 2) r.z = p[2];
-    return r;
-}
-
-__device__ __forceinline__ void st_row4(float *__restrict__ f, const Tile2 &t, int64_t x,
-                                        int64_t k0, v4 v)
-{
-    float *p = f + x * t.P2 + k0;
-    if (k0 + 4 <= t.P2) {
-        *reinterpret_cast<v4u *>(p) = v4u{v.x, v.y, v.z, v.w};
-        return;
-    }
-    const int64_t n = t.P2 - k0;
-    p[0] = v.x;
-    if (n > 1) p[1] = v.y;
-    if (n > 2) p[2] = v.z;
-}
-
-__device__ __forceinline__ float el(const v4 &v, int i)
-{
-    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
-}
-
-__device__ __forceinline__ void set_el(v4 &v, int i, float s)
-{
-    if (i == 0) v.x = s;
-    else if (i == 1) v.y = s;
-    else if (i == 2) v.z = s;
-    else v.w = s;
-}
-
-// Column neighbours of the 4 values of a lane: left of element 0 and right of
-// element 3, from the neighbouring lanes (or memory at the wave's edges).
-__device__ __forceinline__ void col_edges(const float *__restrict__ f, const Tile2 &t, int64_t x,
-                                          int64_t k0, const v4 &c, int lane, float &left,
-                                          float &right)
-{
-    left = __shfl_up(c.w, 1, 64);
-    right = __shfl_down(c.x, 1, 64);
-    if (lane == 0) left = (k0 > 0 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 - 1] : 0.0f;
-    if (lane == 63) right = (k0 + 4 < t.P2 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 + 4] : 0.0f;
-}
-
-// Laplacian sum at element i of a lane: rows up/mid/down, column edges.
-__device__ __forceinline__ float acc5(const v4 &up, const v4 &mid, const v4 &dn, float left,
-                                      float right, int i, float wm, float wc, float wp)
-{
-    const float c = el(mid, i);
-    const float zl = i == 0 ? left : el(mid, i - 1);
-    const float zr = i == 3 ? right : el(mid, i + 1);
-    float acc = 0.0f;
-    acc += wm * el(up, i);
-    acc += wc * c;
-    acc += wp * el(dn, i);
-    acc += wm * zl;
-    acc += wc * c;
-    acc += wp * zr;
-    return acc;
-}
-
-// Transposed sum at element i: the same neighbours in mirrored order
-// (s = -1 reads +1, s = +1 reads -1), each counted only when the neighbour
-// node is interior; h is 0 outside the interior, so that is implied except
-// for the centre term, which needs node (x, k) itself interior.
-__device__ __forceinline__ float acct5(const v4 &up, const v4 &mid, const v4 &dn, float left,
-                                       float right, int i, bool x_in, bool up_in, bool dn_in,
-                                       bool k_in, bool left_in, bool right_in, float wm,
-                                       float wc, float wp)
-{
-    float acc = 0.0f;
-    if (k_in) {
-        if (dn_in) acc += wm * el(dn, i);
-        if (x_in) acc += wc * el(mid, i);
-        if (up_in) acc += wp * el(up, i);
-    }
-    if (x_in) {
-        const float zr = i == 3 ? right : el(mid, i + 1);
-        const float zl = i == 0 ? left : el(mid, i - 1);
-        if (right_in) acc += wm * zr;
-        if (k_in) acc += wc * el(mid, i);
-        if (left_in) acc += wp * zl;
-    }
-    return acc;
-}
 
 enum Op2 { OP_HEAT = STEP2D_HEAT, OP_WAVE = STEP2D_WAVE, OP_WAVE_ADJ = STEP2D_WAVE_ADJ,
            OP_HEAT_ADJ = STEP2D_HEAT_ADJ };
@@ -171,7 +59,12 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         rin[i] = inside(k + 1, t.g, t.P2);
     }
 
-    if (OP == OP_HEAT || OP == OP_WAVE) {
+    if (OP == OP_HEAT) {
+        heat_rows<RX>(A.a, A.o1, t, x0, k0, lane_ok, lane, 64, A.wm, A.wc, A.wp, A.alpha, dt);
+        return;
+    }
+
+    if (OP == OP_WAVE) {
         v4 rows[RX + 2];
 #pragma unroll
         for (int r = 0; r < RX + 2; ++r) rows[r] = ld_row4(A.a, t, x0 - 1 + r, k0);
@@ -179,20 +72,11 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         for (int r = 0; r < RX; ++r) {
             const int64_t x = x0 + r;
             float left, right;
-            col_edges(A.a, t, x, k0, rows[r + 1], lane, left, right);
+            col_edges(A.a, t, x, k0, rows[r + 1], lane, 64, left, right);
             if (!lane_ok || x >= t.P1) continue;
             const bool xin = inside(x, t.g, t.P1);
             v4 un = rows[r + 1];
-            if (OP == OP_HEAT) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (xin && kin[i]) {
-                        const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i,
-                                               A.wm, A.wc, A.wp);
-                        set_el(un, i, el(rows[r + 1], i) + dt * (A.alpha * acc));
-                    }
-                st_row4(A.o1, t, x, k0, un);
-            } else {
+            {
                 const v4 vo = ld_row4(A.b, t, x, k0);
                 v4 cc = {1.0f, 1.0f, 1.0f, 1.0f};
                 if (A.c2) cc = ld_row4(A.c2, t, x, k0);
@@ -224,7 +108,7 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         for (int r = 0; r < RX; ++r) {
             const int64_t x = x0 + r;
             float left, right;
-            col_edges(A.a, t, x, k0, rows[r + 1], lane, left, right);
+            col_edges(A.a, t, x, k0, rows[r + 1], lane, 64, left, right);
             if (!lane_ok || x >= t.P1) continue;
             const bool xin = inside(x, t.g, t.P1);
             const bool upin = inside(x - 1, t.g, t.P1), dnin = inside(x + 1, t.g, t.P1);
@@ -281,8 +165,8 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         const int64_t x = x0 + r;
         const RowIn dn = load_row(x + 1);
         float hl, hrt, ul = 0.0f, urt = 0.0f;
-        col_edges(nullptr, t, -1, k0, mid.h, lane, hl, hrt);
-        if (A.gc2) col_edges(A.u2, t, x, k0, mid.u, lane, ul, urt);
+        col_edges(nullptr, t, -1, k0, mid.h, lane, 64, hl, hrt);
+        if (A.gc2) col_edges(A.u2, t, x, k0, mid.u, lane, 64, ul, urt);
         if (lane == 0) hl = h_at(x, k0 - 1);
         if (lane == 63) hrt = h_at(x, k0 + 4);
         if (lane_ok && x < t.P1) {

@@ -1,0 +1,160 @@
+// Row-tile helpers for 2D scalar fields (padded extents P1 rows x P2 columns,
+// column index fastest).  A lane owns 4 consecutive columns of a row, loaded
+// as one 16-byte access (unaligned dwordx4 is legal on gfx950); lanes are
+// grouped in row segments of `cw` lanes (64, 32 or 16) that cover 4 * cw
+// consecutive columns, and column neighbours cross lanes by shuffles within a
+// segment (the segment's edge lanes load their outer neighbour).
+#pragma once
+#include "pde_common.hpp"
+
+namespace pde {
+
+typedef float v4 __attribute__((ext_vector_type(4)));
+typedef float v4u __attribute__((ext_vector_type(4), aligned(4)));
+
+struct Tile2 {
+    int64_t P1, P2;       // padded rows, columns
+    int64_t g;            // ghost width
+    int64_t nq;           // 4-column chunks per row (ceil(P2 / 4))
+    uint32_t tq, tr;      // tiles along columns and rows (single-grid launches)
+};
+
+__device__ __forceinline__ bool inside(int64_t v, int64_t g, int64_t P) { return v >= g && v < P - g; }
+
+// Loads the 4 columns k0 .. k0+3 of row x (clamped to the array: columns past
+// P2 read as 0; rows outside [0, P1) read as 0).
+__device__ __forceinline__ v4 ld_row4(const float *__restrict__ f, const Tile2 &t, int64_t x,
+                                      int64_t k0)
+{
+    v4 r = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (x < 0 || x >= t.P1) return r;
+    const float *p = f + x * t.P2 + k0;
+    if (k0 + 4 <= t.P2) {
+        const v4u q = *reinterpret_cast<const v4u *>(p);
+        return v4{q.x, q.y, q.z, q.w};
+    }
+    const int64_t n = t.P2 - k0;
+    r.x = p[0];
+    if (n > 1) r.y = p[1];
+    if (n > 2) r.z = p[2];
+    return r;
+}
+
+__device__ __forceinline__ void st_row4(float *__restrict__ f, const Tile2 &t, int64_t x,
+                                        int64_t k0, v4 v)
+{
+    float *p = f + x * t.P2 + k0;
+    if (k0 + 4 <= t.P2) {
+        *reinterpret_cast<v4u *>(p) = v4u{v.x, v.y, v.z, v.w};
+        return;
+    }
+    const int64_t n = t.P2 - k0;
+    p[0] = v.x;
+    if (n > 1) p[1] = v.y;
+    if (n > 2) p[2] = v.z;
+}
+
+__device__ __forceinline__ float el(const v4 &v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ void set_el(v4 &v, int i, float s)
+{
+    if (i == 0) v.x = s;
+    else if (i == 1) v.y = s;
+    else if (i == 2) v.z = s;
+    else v.w = s;
+}
+
+// Column neighbours of the 4 values of a lane: left of element 0 and right of
+// element 3, from the neighbouring lanes of its row segment (sl = lane index
+// in the segment of cw lanes), or memory at the segment's edges.
+__device__ __forceinline__ void col_edges(const float *__restrict__ f, const Tile2 &t, int64_t x,
+                                          int64_t k0, const v4 &c, int sl, int cw, float &left,
+                                          float &right)
+{
+    left = __shfl_up(c.w, 1, cw);
+    right = __shfl_down(c.x, 1, cw);
+    if (sl == 0) left = (k0 > 0 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 - 1] : 0.0f;
+    if (sl == cw - 1)
+        right = (k0 + 4 < t.P2 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 + 4] : 0.0f;
+}
+
+// Laplacian sum at element i of a lane: rows up/mid/down, column edges.
+__device__ __forceinline__ float acc5(const v4 &up, const v4 &mid, const v4 &dn, float left,
+                                      float right, int i, float wm, float wc, float wp)
+{
+    const float c = el(mid, i);
+    const float zl = i == 0 ? left : el(mid, i - 1);
+    const float zr = i == 3 ? right : el(mid, i + 1);
+    float acc = 0.0f;
+    acc += wm * el(up, i);
+    acc += wc * c;
+    acc += wp * el(dn, i);
+    acc += wm * zl;
+    acc += wc * c;
+    acc += wp * zr;
+    return acc;
+}
+
+// Transposed sum at element i: the same neighbours in mirrored order
+// (s = -1 reads +1, s = +1 reads -1), each counted only when the neighbour
+// node is interior; h is 0 outside the interior, so that is implied except
+// for the centre term, which needs node (x, k) itself interior.
+__device__ __forceinline__ float acct5(const v4 &up, const v4 &mid, const v4 &dn, float left,
+                                       float right, int i, bool x_in, bool up_in, bool dn_in,
+                                       bool k_in, bool left_in, bool right_in, float wm,
+                                       float wc, float wp)
+{
+    float acc = 0.0f;
+    if (k_in) {
+        if (dn_in) acc += wm * el(dn, i);
+        if (x_in) acc += wc * el(mid, i);
+        if (up_in) acc += wp * el(up, i);
+    }
+    if (x_in) {
+        const float zr = i == 3 ? right : el(mid, i + 1);
+        const float zl = i == 0 ? left : el(mid, i - 1);
+        if (right_in) acc += wm * zr;
+        if (k_in) acc += wc * el(mid, i);
+        if (left_in) acc += wp * zl;
+    }
+    return acc;
+}
+
+// One lane's share of the fused heat step u_next = u2 + dt * (alpha *
+// acc(u2)) on interior nodes, u_next = u2 elsewhere: rows x0 .. x0+RXT-1 of
+// its 4 columns k0 .. k0+3.  Every lane of the wave calls it (shuffles).
+template <int RXT>
+__device__ __forceinline__ void heat_rows(const float *__restrict__ u, float *__restrict__ un,
+                                          const Tile2 &t, int64_t x0, int64_t k0, bool lane_ok,
+                                          int sl, int cw, float wm, float wc, float wp,
+                                          float alpha, float dt)
+{
+    bool kin[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) kin[i] = inside(k0 + i, t.g, t.P2);
+    v4 rows[RXT + 2];
+#pragma unroll
+    for (int r = 0; r < RXT + 2; ++r) rows[r] = ld_row4(u, t, x0 - 1 + r, k0);
+#pragma unroll
+    for (int r = 0; r < RXT; ++r) {
+        const int64_t x = x0 + r;
+        float left, right;
+        col_edges(u, t, x, k0, rows[r + 1], sl, cw, left, right);
+        if (!lane_ok || x >= t.P1) continue;
+        const bool xin = inside(x, t.g, t.P1);
+        v4 o = rows[r + 1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (xin && kin[i]) {
+                const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i, wm, wc,
+                                       wp);
+                set_el(o, i, el(rows[r + 1], i) + dt * (alpha * acc));
+            }
+        st_row4(un, t, x, k0, o);
+    }
+}
+
+}  // namespace pde

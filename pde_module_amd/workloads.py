@@ -116,3 +116,47 @@ def wave_2d_config(seed=0, n=2048, steps=2000, nrec=64, corr=32.0, ghost_cells=1
     row = P - 2 - g                                # next to the +Y wall
     rec = cols * P + row
     return Wave2D(n, g, dx, dt, steps, c2, src, wav, rec, f0)
+
+
+def gaussian_bumps_separable(rng, coords_x, coords_y, n_bumps=8):
+    """The same bump field as ``gaussian_bumps`` (same draws, same order),
+    each bump evaluated as the outer product exp(-dx^2/2w^2) exp(-dy^2/2w^2):
+    O(n) exponentials per bump instead of O(n^2)."""
+    centres = rng.uniform(0.2, 0.8, size=(n_bumps, 2))
+    widths = rng.uniform(0.02, 0.08, size=n_bumps)
+    amps = rng.uniform(0.5, 1.0, size=n_bumps)
+    u = np.zeros((len(coords_x), len(coords_y)))
+    for (cx, cy), w, a in zip(centres, widths, amps):
+        ex = np.exp(-((coords_x - cx) ** 2) / (2.0 * w * w))
+        ey = np.exp(-((coords_y - cy) ** 2) / (2.0 * w * w))
+        u += a * np.outer(ex, ey)
+    return u
+
+
+@dataclass
+class HeatEnsembleConfig:
+    g: int
+    n: np.ndarray        # interior nodes per side, per instance
+    dx: np.ndarray
+    alpha: np.ndarray
+    dt: np.ndarray
+    steps: int
+
+    def u0(self, i, seed):
+        """Initial field of instance i: 8 Gaussian bumps from default_rng([seed, i])."""
+        n, g = int(self.n[i]), self.g
+        coords = np.arange(-g, n + g, dtype=np.float64) * self.dx[i]
+        u = gaussian_bumps_separable(np.random.default_rng([seed, i]), coords, coords)
+        return u.astype(np.float32).reshape(n + 2 * g, n + 2 * g, 1, 1)
+
+
+def heat_ensemble_config(seed=3, count=512, n_min=64, n_max=1024, steps=500, ghost_cells=1):
+    """Batched heat workload: side lengths rint(exp(U[ln n_min, ln n_max])),
+    diffusivities U[0.005, 0.05] (both from default_rng(seed), in that order),
+    dx = 1/(n-1), dt = 0.2 dx^2 / alpha, Dirichlet 0 walls."""
+    rng = np.random.default_rng(seed)
+    n = np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), count))).astype(np.int64)
+    alpha = rng.uniform(0.005, 0.05, count)
+    dx = 1.0 / (n - 1)
+    dt = 0.2 * dx * dx / alpha
+    return HeatEnsembleConfig(ghost_cells, n, dx, alpha, dt, steps)
