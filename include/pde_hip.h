@@ -262,8 +262,77 @@ int pde_wave_step_adjoint_ctr_f32(const float *gun, const float *gvn, const floa
                                   float w_center, float w_plus, float alpha, float dt,
                                   const int64_t *step, void *stream);
 
+/* ------------------------------------------- fused wave loop steps (2D grids) */
+/* Point sources and receivers of a 2D scalar field, listed by padded row:
+ * the points of row x are [row_start[x], row_start[x+1]) (row_start has
+ * p1 + 1 entries, p1 = padded rows); col[p] is the point's padded column and
+ * slot[p] its column in `data`, whose row n belongs to step n (row stride
+ * `stride` floats).  Sources read data (v[point] += scale * data[n][slot]
+ * before the step, points in list order); receivers write it
+ * (data[n][slot] = u_next[point]). */
+typedef struct pde_point_set {
+    const int32_t *row_start;
+    const int32_t *col;
+    const int32_t *slot;
+    float *data;
+    int64_t stride;
+    float scale;
+    int32_t reserved;
+} pde_point_set;
+
+/* One step of the wave loop on a 2D scalar field in one launch: the source
+ * injection, the fused u/v step of pde_wave_step_ctr_f32 and the receiver
+ * samples, with n = *step and *step_next = n + 1 written by the kernel
+ * (step and step_next are two device counters used alternately, so no
+ * kernel reads a counter another one of the same step writes).  sources and
+ * receivers are host pointers to point sets (NULL: none).  Same values as
+ * pde_scatter_add_step_f32 + pde_wave_step_ctr_f32 + pde_gather_step_f32 +
+ * pde_counter_add.  PDE_ESHAPE unless the field is 2D.  No reference
+ * counterpart (the reference loop is examples/wave2d/standingWave.py:70-76,
+ * without sources or receivers). */
+int pde_wave_step_fused_f32(const float *u, const float *v, const float *c2, float *u_next,
+                            float *v_next, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                            float w_minus, float w_center, float w_plus, float alpha, float dt,
+                            const int64_t *step, int64_t *step_next,
+                            const pde_point_set *sources, const pde_point_set *receivers,
+                            void *stream);
+
+/* The transpose of one such step in one launch: the receiver gradients
+ * (receivers->data rows, scale 1) added to g_un at the receivers, then
+ * pde_wave_step_adjoint_ctr_f32, with n = *step and *step_next = n - 1. */
+int pde_wave_step_adjoint_fused_f32(const float *gun, const float *gvn, const float *c2,
+                                    const float *u2, float *gu2, float *gv, float *gc2,
+                                    int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                    float w_minus, float w_center, float w_plus, float alpha,
+                                    float dt, const int64_t *step, int64_t *step_next,
+                                    const pde_point_set *receivers, void *stream);
+
+/* *dst = *src + delta on the device (moves a step counter between the two
+ * alternating counters of the fused steps).  No reference counterpart. */
+int pde_counter_copy(const int64_t *src, int64_t *dst, int64_t delta, void *stream);
+
+/* pde_bc_apply_inplace_f32 in one full-chip launch.  The host orders the ops
+ * in conflict-closed chunks: chunk c is ops [chunk_start[c], chunk_start[c+1])
+ * (at most 256), and no op reads a scalar that an op of another chunk writes;
+ * each workgroup then applies its chunk with the two-phase semantics on its
+ * own.  No reference counterpart (the in-place form of the boundary apply). */
+int pde_bc_apply_chunked_inplace_f32(float *u, const pde_bc_op *ops, const int64_t *chunk_start,
+                                     int64_t nchunks, const float *t, void *stream);
+
+/* pde_bc_adjoint_inplace_f32 in one full-chip launch over conflict-closed
+ * chunks: chunk c stages sources [item_start[c], item_start[c+1]) (at most
+ * 256; src_idx/src_flags/off in chunk order, off indexing cdst/csign) and
+ * clears zero_idx[zero_start[c] .. zero_start[c+1]); every written scalar
+ * that a chunk's sources read belongs to the same chunk. */
+int pde_bc_adjoint_chunked_inplace_f32(float *g, const int64_t *src_idx, const int32_t *src_flags,
+                                       const int64_t *off, const int64_t *cdst,
+                                       const float *csign, const int64_t *item_start,
+                                       const int64_t *zero_idx, const int64_t *zero_start,
+                                       int64_t nchunks, void *stream);
+
 /* Transpose of the linear part of pde_bc_apply_f32: gin = gout, then
- * gin[zero_idx[i]] = 0 for every written scalar, then for each distinct
+ * gin[zero_idx[i]] = 0 (the written scalars; the host passes those that are
+ * not sources, the others are overwritten next), then for each distinct
  * source src_idx[i] (flag bit 0 set when it is itself written):
  * gin[src] = (flag ? 0 : gout[src]) + sum over q in [off[i], off[i+1]) of
  * csign[q] * gout[cdst[q]].  The host builds these arrays from the op table. */
@@ -273,8 +342,11 @@ int pde_bc_adjoint_f32(const float *gout, float *gin, int64_t n, const int64_t *
                        int64_t nsrc, void *stream);
 
 /* The same transpose applied to g in place: the new source values are first
- * gathered into staging[nsrc] from the untouched g, then every written scalar
- * is zeroed, then the staged values are stored.  No full-field copy. */
+ * gathered into staging[nsrc] from the untouched g, then, in one launch, the
+ * staged values are stored and the zero_idx scalars zeroed.  Here zero_idx
+ * must not contain any src_idx entry (the written scalars that are not
+ * sources; a source that is also written gets its staged value, whose flag
+ * already drops its own term).  No full-field copy. */
 int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int64_t nzero,
                                const int64_t *src_idx, const int32_t *src_flags,
                                const int64_t *off, const int64_t *cdst, const float *csign,

@@ -1,5 +1,10 @@
 """Device side of a resolved boundary: the op table uploaded once at setup,
-the 1-element time array and the staging buffer for in-place application."""
+the 1-element time array and the staging buffer for in-place application.
+
+The table is stored in conflict-closed chunk order (``tables.chunk_ops``), so
+in-place application and its transpose are one full-chip launch each; when a
+table cannot be chunked (a conflict component larger than a workgroup) the
+two-launch gather/scatter forms are used."""
 from __future__ import annotations
 
 import numpy as np
@@ -10,7 +15,14 @@ from ... import _native as N
 
 class DeviceBoundaryTable:
     def __init__(self, ops: np.ndarray, device: torch.device):
+        from .tables import chunk_ops
         self.nops = int(len(ops))
+        ch = chunk_ops(ops) if self.nops else None
+        self.chunk_start = None
+        if ch is not None:
+            ops = ch[0]
+            self.chunk_start = torch.from_numpy(ch[1]).to(device)
+            self.nchunks = len(ch[1]) - 1
         self.needs_t = bool(self.nops and np.any(ops["tmax"] > 0))
         raw = np.ascontiguousarray(ops).view(np.uint8) if self.nops else np.zeros(32, np.uint8)
         self.ops = torch.from_numpy(raw.copy()).to(device)
@@ -18,6 +30,7 @@ class DeviceBoundaryTable:
         self.host_ops = ops
         self.device = device
         self._adj = None
+        self._adj_chunks = None
         # Largest scalar index any op touches; checked against the field size at
         # every call so a bad table raises on the host instead of faulting.
         if self.nops:
@@ -49,6 +62,11 @@ class DeviceBoundaryTable:
         unmodified u first)."""
         N.check_field(u, "field")
         self._check_bounds(u.numel())
+        if self.chunk_start is not None:
+            N.call("pde_bc_apply_chunked_inplace_f32", N.dev_ptr(u), self.ops.data_ptr(),
+                   self.chunk_start.data_ptr(), self.nchunks,
+                   t.data_ptr() if t is not None else None, N.stream_of(u))
+            return u
         N.call("pde_bc_apply_inplace_f32", N.dev_ptr(u), self.ops.data_ptr(), self.nops,
                t.data_ptr() if t is not None else None, self.staging.data_ptr(),
                N.stream_of(u))
@@ -62,9 +80,15 @@ class DeviceBoundaryTable:
         if not self.nops:
             self._adj = (None, 0, None, None, None, None, None, 0)
             return
-        from .tables import transpose_table
+        from .tables import chunk_transpose, transpose_table
         T = transpose_table(self.host_ops)
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        C = chunk_transpose(T)
+        self._adj_chunks = None
+        if C is not None:
+            self._adj_chunks = tuple(dev(C[k]) for k in ("src_idx", "flags", "off", "cdst",
+                                                         "csign", "item_start", "zero_idx",
+                                                         "zero_start")) + (len(C["item_start"]) - 1,)
         self._adj = (dev(T["zero_idx"]), len(T["zero_idx"]), dev(T["src_idx"]), dev(T["flags"]),
                      dev(T["off"]), dev(T["cdst"]), dev(T["csign"]), len(T["src_idx"]))
         self._adj_staging = torch.empty(max(len(T["src_idx"]), 1), dtype=torch.float32,
@@ -91,6 +115,11 @@ class DeviceBoundaryTable:
         if nz == 0 and ns == 0:
             return g
         p = lambda t: t.data_ptr() if t is not None else None
+        if self._adj_chunks is not None:
+            *arrs, nch = self._adj_chunks
+            N.call("pde_bc_adjoint_chunked_inplace_f32", N.dev_ptr(g), *(a.data_ptr() for a in arrs),
+                   nch, N.stream_of(g))
+            return g
         N.call("pde_bc_adjoint_inplace_f32", N.dev_ptr(g), p(z), nz, p(s), p(f), p(o), p(cd),
                p(cs), ns, self._adj_staging.data_ptr(), N.stream_of(g))
         return g

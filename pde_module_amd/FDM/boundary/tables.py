@@ -143,11 +143,13 @@ def legacy_ops(grid_shape, ncomp, ijk, interior, btype, bvalue, dx32):
 
 
 def transpose_table(ops):
-    """Arrays of pde_bc_adjoint_f32 for a resolved op table: every written
-    scalar is zeroed, then each distinct source gathers sign * g[dst] over the
-    ops that read it (sorted by dst), plus g[src] itself unless it is written.
+    """Arrays of pde_bc_adjoint_f32 for a resolved op table: each distinct
+    source gathers sign * g[dst] over the ops that read it (sorted by dst),
+    plus g[src] itself unless it is written (flag 1); every written scalar
+    that is not a source is zeroed (zero_idx: disjoint from src_idx, so the
+    in-place form can zero and store in one launch).
     Signs: COPY +1, Neumann ghost +1, Dirichlet ghost -1; SET reads nothing."""
-    zero_idx = np.ascontiguousarray(ops["dst"], dtype=np.int64)
+    dst = np.ascontiguousarray(ops["dst"], dtype=np.int64)
     reads = ops[ops["kind"] != BC_SET]
     sign = np.where(reads["kind"] == BC_GHOST_D, np.float32(-1.0), np.float32(1.0)).astype(np.float32)
     order = np.lexsort((reads["dst"], reads["src"]))
@@ -156,6 +158,140 @@ def transpose_table(ops):
     csign = np.ascontiguousarray(sign[order])
     src_idx, start = np.unique(src_sorted, return_index=True)
     off = np.append(start, len(src_sorted)).astype(np.int64)
-    flags = np.isin(src_idx, zero_idx).astype(np.int32)
+    flags = np.isin(src_idx, dst).astype(np.int32)
+    zero_idx = np.ascontiguousarray(dst[~np.isin(dst, src_idx)], dtype=np.int64)
     return dict(zero_idx=zero_idx, src_idx=np.ascontiguousarray(src_idx, dtype=np.int64),
                 flags=flags, off=off, cdst=cdst, csign=csign)
+
+
+# ----------------------------------------------------------------- chunking
+# One full-chip launch can apply a table in place if it is split into
+# conflict-closed chunks: no entry reads a scalar that an entry of another
+# chunk writes.  Each workgroup then does its chunk's reads, a barrier and its
+# writes.  Conflicts are rare (ops near corners whose source is itself a
+# boundary node), so chunks are just runs of at most CHUNK entries cut at
+# component boundaries.
+CHUNK = 256
+
+
+def _components(n, a, b):
+    """Connected-component label per unit 0..n-1 for edges (a[i], b[i])."""
+    lab = np.arange(n, dtype=np.int64)
+    if len(a) == 0:
+        return lab
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    while True:
+        la, lb = lab[a], lab[b]
+        m = np.minimum(la, lb)
+        before = lab.copy()
+        np.minimum.at(lab, la, m)
+        np.minimum.at(lab, lb, m)
+        while True:                      # pointer jumping to the roots
+            nl = lab[lab]
+            if np.array_equal(nl, lab):
+                break
+            lab = nl
+        if np.array_equal(lab, before):
+            return lab
+
+
+def _pack(labels, weight, cap):
+    """Units grouped by component (stable), cut into runs whose total weight is
+    at most cap without splitting a component.  Returns (order, run bounds in
+    units) or None if a component alone exceeds cap."""
+    order = np.argsort(labels, kind="stable")
+    ls = labels[order]
+    n = len(order)
+    if n == 0:
+        return order, np.zeros(1, np.int64)
+    ends = np.append(np.flatnonzero(np.diff(ls)) + 1, n)          # component ends
+    cw = np.concatenate([[0], np.cumsum(weight[order])])           # weight prefix
+    comp_w = cw[ends] - cw[np.concatenate([[0], ends[:-1]])]
+    if comp_w.max() > cap:
+        return None
+    bounds = [0]
+    cur = 0
+    wend = cw[ends]
+    while cur < n:
+        k = np.searchsorted(wend, cw[cur] + cap, side="right") - 1
+        nxt = int(ends[k])
+        if nxt <= cur:                   # zero-weight tail etc.
+            nxt = int(ends[np.searchsorted(ends, cur, side="right")])
+        bounds.append(nxt)
+        cur = nxt
+    return order, np.asarray(bounds, np.int64)
+
+
+def chunk_ops(ops, cap=CHUNK):
+    """(ops in chunk order, chunk_start) for pde_bc_apply_chunked_inplace_f32,
+    or None when a conflict component is larger than a chunk."""
+    n = len(ops)
+    if n == 0:
+        return ops, np.zeros(1, np.int64)
+    dst = np.asarray(ops["dst"], np.int64)
+    by_dst = np.argsort(dst, kind="stable")
+    sd = dst[by_dst]
+    reads = np.flatnonzero(ops["kind"] != BC_SET)
+    src = np.asarray(ops["src"], np.int64)[reads]
+    pos = np.minimum(np.searchsorted(sd, src), n - 1)
+    hit = sd[pos] == src
+    labels = _components(n, reads[hit], by_dst[pos[hit]])
+    packed = _pack(labels, np.ones(n, np.int64), cap)
+    if packed is None:
+        return None
+    order, bounds = packed
+    return np.ascontiguousarray(ops[order]), bounds
+
+
+def chunk_transpose(T, cap=CHUNK):
+    """Chunk-ordered arrays of pde_bc_adjoint_chunked_inplace_f32 from a
+    transpose_table dict (at most cap sources plus zero entries per chunk), or
+    None when a conflict component is larger than that."""
+    src, off = T["src_idx"], T["off"]
+    zero = T["zero_idx"]
+    ni, nz = len(src), len(zero)
+    U = ni + nz
+    if U == 0:
+        return None
+    loc = np.concatenate([src, zero]).astype(np.int64)           # scalar each unit writes
+    by_loc = np.argsort(loc, kind="stable")
+    sl = loc[by_loc]
+    counts = np.diff(off)
+    item_of_q = np.repeat(np.arange(ni, dtype=np.int64), counts)
+    cd = np.asarray(T["cdst"], np.int64)
+    pos = np.minimum(np.searchsorted(sl, cd), U - 1)
+    hit = sl[pos] == cd
+    labels = _components(U, item_of_q[hit], by_loc[pos[hit]])
+    # zero entries count too, so no chunk collects a long tail of them
+    weight = np.ones(U, np.int64)
+    packed = _pack(labels, weight, cap)
+    if packed is None:
+        return None
+    order, bounds = packed
+    is_item = order < ni
+    item_start, zero_start = [0], [0]
+    items, zeros = [], []
+    for c in range(len(bounds) - 1):
+        seg = order[bounds[c]:bounds[c + 1]]
+        m = seg < ni
+        items.append(seg[m])
+        zeros.append(seg[~m] - ni)
+        item_start.append(item_start[-1] + int(m.sum()))
+        zero_start.append(zero_start[-1] + int((~m).sum()))
+    perm = np.concatenate(items) if items else np.zeros(0, np.int64)
+    zperm = np.concatenate(zeros) if zeros else np.zeros(0, np.int64)
+    assert is_item.sum() == len(perm) == ni
+    cnt = counts[perm]
+    new_off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    # gather each source's (cdst, csign) segment in the new order
+    seg_start = np.repeat(off[:-1][perm] - new_off[:-1], cnt)
+    qidx = np.arange(int(new_off[-1]), dtype=np.int64) + seg_start
+    return dict(src_idx=np.ascontiguousarray(src[perm], np.int64),
+                flags=np.ascontiguousarray(T["flags"][perm], np.int32),
+                off=new_off,
+                cdst=np.ascontiguousarray(cd[qidx], np.int64),
+                csign=np.ascontiguousarray(np.asarray(T["csign"])[qidx], np.float32),
+                item_start=np.asarray(item_start, np.int64),
+                zero_idx=np.ascontiguousarray(zero[zperm], np.int64),
+                zero_start=np.asarray(zero_start, np.int64))

@@ -50,6 +50,8 @@ SIGNATURES = {
     "pde_lincomb_f32": [_P, _P, _P, _P, _F, _F, _F, _F, _I32, _P, _I64, _P],
     "pde_affine_f32": [_P, _P, _I64, _I32, _I32, _F, _F, _P],
     "pde_bc_adjoint_inplace_f32": [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P],
+    "pde_bc_apply_chunked_inplace_f32": [_P, _P, _P, _I64, _P, _P],
+    "pde_bc_adjoint_chunked_inplace_f32": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "pde_gather_f32": [_P, _P, _P, _I64, _P],
     "pde_wave_step_ctr_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
                               _P, _P],
@@ -58,6 +60,11 @@ SIGNATURES = {
     "pde_gather_step_f32": [_P, _P, _P, _I64, _I64, _P, _P],
     "pde_scatter_add_step_f32": [_P, _P, _P, _I64, _I64, _F, _P, _P],
     "pde_counter_add": [_P, _I64, _P],
+    "pde_counter_copy": [_P, _P, _I64, _P],
+    "pde_wave_step_fused_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
+                                _P, _P, _P, _P, _P],
+    "pde_wave_step_adjoint_fused_f32": [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F,
+                                        _F, _F, _F, _F, _P, _P, _P, _P],
     "pde_scatter_add_f32": [_P, _P, _P, _I64, _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],
@@ -75,6 +82,12 @@ BATCH_GRID_DTYPE = np.dtype([("offset", "<i8"), ("op_begin", "<i8"), ("op_end", 
                              ("reserved0", "<i4"), ("wm", "<f4"), ("wc", "<f4"), ("wp", "<f4"),
                              ("alpha", "<f4"), ("dt", "<f4"), ("reserved1", "<f4")])
 assert BATCH_GRID_DTYPE.itemsize == 64
+
+# struct pde_point_set (pointers as integers)
+POINT_SET_DTYPE = np.dtype([("row_start", "<u8"), ("col", "<u8"), ("slot", "<u8"),
+                            ("data", "<u8"), ("stride", "<i8"), ("scale", "<f4"),
+                            ("reserved", "<i4")])
+assert POINT_SET_DTYPE.itemsize == 48
 
 
 class PDEError(RuntimeError):

@@ -6,8 +6,10 @@ ForwardEuler: ``examples/2D_Heat_Equation.py:58-62``).  ``HeatEnsemble`` packs
 every instance's padded field into one buffer and steps all of them with two
 launches per step:
 
-1. ``pde_bc_apply_batch_inplace_f32`` — workgroup i applies instance i's
-   boundary ops in place (its slice of one packed op table);
+1. ``pde_bc_apply_chunked_inplace_f32`` over one packed op table: every
+   instance's ops in conflict-closed chunks of at most 256, one workgroup per
+   chunk (``pde_bc_apply_batch_inplace_f32``, one workgroup per instance, when
+   some table cannot be chunked);
 2. ``pde_heat_batch_step_f32`` — one tile list over all instances; narrow
    instances use row segments of 32 or 16 lanes so most lanes stay busy.
 
@@ -151,13 +153,22 @@ class HeatEnsemble:
         self.points = int(sum((p1 - 2 * self.g) * (p2 - 2 * self.g)
                               for p1, p2 in zip(self.p1, self.p2)))
 
-        # packed boundary table, every index shifted to the packed buffer
+        # packed boundary table, every index shifted to the packed buffer, each
+        # instance's ops in conflict-closed chunk order
+        from .FDM.boundary.tables import chunk_ops
         cap = N.load().pde_batch_bc_capacity()
         tables, begin = [], 0
+        chunk_starts, chunked = [np.zeros(1, np.int64)], True
         grid = np.zeros(self.n, N.BATCH_GRID_DTYPE)
         for i, bc in enumerate(boundaries):
             ops = np.array(bc.resolved_ops(), dtype=N.BC_OP_DTYPE, copy=True)
-            if len(ops) > cap:
+            ch = chunk_ops(ops) if len(ops) else (ops, np.zeros(1, np.int64))
+            if ch is None:
+                chunked = False
+            else:
+                ops = np.array(ch[0], copy=True)
+                chunk_starts.append(ch[1][1:] + begin)
+            if not chunked and len(ops) > cap:
                 raise ValueError(f"instance {i} has {len(ops)} boundary ops; at most {cap}")
             rd = ops["src"][ops["kind"] != N.BC_SET]
             idx = np.concatenate([ops["dst"], rd]) if len(ops) else np.zeros(0, np.int64)
@@ -176,6 +187,11 @@ class HeatEnsemble:
         dev = self.device
         self.grid_dev = torch.from_numpy(grid.view(np.uint8).copy()).to(dev)
         self.ops_dev = torch.from_numpy(allops.view(np.uint8).copy()).to(dev)
+        self.chunk_start = None
+        if chunked:
+            cs = np.concatenate(chunk_starts).astype(np.int64)
+            self.chunk_start = torch.from_numpy(cs).to(dev)
+            self.nchunks = len(cs) - 1
         self.t = torch.zeros(1, dtype=torch.float32, device=dev)
         self.bufs = [torch.zeros(self.total, dtype=torch.float32, device=dev) for _ in range(2)]
         self.cur = 0
@@ -219,8 +235,13 @@ class HeatEnsemble:
 
     def _step(self, src: torch.Tensor, dst: torch.Tensor, cfg: TileConfig) -> None:
         st = N.stream_of(src)
-        N.call("pde_bc_apply_batch_inplace_f32", src.data_ptr(), self.grid_dev.data_ptr(),
-               self.n, self.ops_dev.data_ptr(), self.t.data_ptr() if self.needs_t else None, st)
+        tp = self.t.data_ptr() if self.needs_t else None
+        if self.chunk_start is not None:
+            N.call("pde_bc_apply_chunked_inplace_f32", src.data_ptr(), self.ops_dev.data_ptr(),
+                   self.chunk_start.data_ptr(), self.nchunks, tp, st)
+        else:
+            N.call("pde_bc_apply_batch_inplace_f32", src.data_ptr(), self.grid_dev.data_ptr(),
+                   self.n, self.ops_dev.data_ptr(), tp, st)
         tiles, nt = self._tiles(cfg)
         N.call("pde_heat_batch_step_f32", src.data_ptr(), dst.data_ptr(),
                self.grid_dev.data_ptr(), self.n, tiles.data_ptr(), nt, cfg.rows_per_lane,

@@ -282,13 +282,114 @@ __global__ __launch_bounds__(256) void bc_adj_gather_stage_kernel(
     staging[i] = acc;
 }
 
+// Second phase of the in-place transpose: threads [0, nsrc) store the staged
+// source values, threads [nsrc, nsrc + nzero) zero the written scalars that
+// are not sources (the two index sets are disjoint).
 __global__ __launch_bounds__(256) void bc_adj_write_kernel(float *__restrict__ g,
                                                            const int64_t *__restrict__ src,
                                                            const float *__restrict__ staging,
-                                                           int64_t nsrc)
+                                                           int64_t nsrc,
+                                                           const int64_t *__restrict__ zero_idx,
+                                                           int64_t nzero)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < nsrc) g[src[i]] = staging[i];
+    else if (i < nsrc + nzero) g[zero_idx[i - nsrc]] = 0.0f;
+}
+
+// The in-place transpose by one workgroup of 1024 threads for small tables
+// (latency-bound; one CU cannot stream a large one): staged values (same accumulation order as the stage kernel: own
+// term, then csign[q] * g[cdst[q]] for q ascending) go to LDS; after the
+// barrier the sources are stored and the zero_idx scalars cleared.  Each
+// thread's sources are processed BC_UNROLL at a time, stage by stage, so the
+// dependent loads of different sources overlap.
+__global__ __launch_bounds__(1024) void bc_adj_inplace_block_kernel(
+    float *__restrict__ g, const int64_t *__restrict__ zero_idx, int64_t nzero,
+    const int64_t *__restrict__ src, const int32_t *__restrict__ flags,
+    const int64_t *__restrict__ off, const int64_t *__restrict__ cdst,
+    const float *__restrict__ csign, int64_t nsrc)
+{
+    __shared__ float vals[BC_BLOCK_CAP];
+    const int n = (int)nsrc;
+    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
+        int64_t sj[BC_UNROLL], o0[BC_UNROLL], o1[BC_UNROLL], d0[BC_UNROLL];
+        int32_t fj[BC_UNROLL];
+        float acc[BC_UNROLL], c0[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) {
+                sj[j] = src[m];
+                fj[j] = flags[m];
+                o0[j] = off[m];
+                o1[j] = off[m + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) {
+                acc[j] = (fj[j] & 1) ? 0.0f : g[sj[j]];
+                if (o1[j] > o0[j]) {
+                    d0[j] = cdst[o0[j]];
+                    c0[j] = csign[o0[j]];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) {
+                float a = acc[j];
+                if (o1[j] > o0[j]) a += c0[j] * g[d0[j]];
+                for (int64_t q = o0[j] + 1; q < o1[j]; ++q) a += csign[q] * g[cdst[q]];
+                vals[m] = a;
+            }
+        }
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
+        int64_t sj[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) sj[j] = src[m];
+        }
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) g[sj[j]] = vals[m];
+        }
+    }
+    for (int64_t i = threadIdx.x; i < nzero; i += 1024) g[zero_idx[i]] = 0.0f;
+}
+
+// The in-place transpose over conflict-closed chunks (host-built): chunk c
+// holds sources [item_start[c], item_start[c+1]) (at most BC_CHUNK) and zero
+// entries [zero_start[c], zero_start[c+1]); every scalar a chunk's sources
+// read that the transpose writes belongs to the same chunk.  One workgroup
+// per chunk: stage, barrier, store and zero.
+__global__ __launch_bounds__(BC_CHUNK) void bc_adj_chunk_inplace_kernel(
+    float *__restrict__ g, const int64_t *__restrict__ src, const int32_t *__restrict__ flags,
+    const int64_t *__restrict__ off, const int64_t *__restrict__ cdst,
+    const float *__restrict__ csign, const int64_t *__restrict__ item_start,
+    const int64_t *__restrict__ zero_idx, const int64_t *__restrict__ zero_start)
+{
+    const int64_t ib = item_start[blockIdx.x], ie = item_start[blockIdx.x + 1];
+    const int64_t i = ib + threadIdx.x;
+    const bool on = i < ie && ie - ib <= BC_CHUNK;
+    float acc = 0.0f;
+    int64_t s = 0;
+    if (on) {
+        s = src[i];
+        acc = (flags[i] & 1) ? 0.0f : g[s];
+        for (int64_t q = off[i]; q < off[i + 1]; ++q) acc += csign[q] * g[cdst[q]];
+    }
+    __syncthreads();
+    if (on) g[s] = acc;
+    for (int64_t z = zero_start[blockIdx.x] + threadIdx.x; z < zero_start[blockIdx.x + 1];
+         z += BC_CHUNK)
+        g[zero_idx[z]] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(const float *in, float *out, int64_t n, float s)
@@ -466,21 +567,21 @@ extern "C" int pde_bc_adjoint_inplace_f32(float *g, const int64_t *zero_idx, int
         return PDE_EINVAL;
     hipStream_t s = as_stream(stream);
     int st = PDE_OK;
+    if (nsrc <= 2048 && nzero <= 4096) {
+        if (nsrc + nzero == 0) return PDE_OK;
+        hipLaunchKernelGGL(bc_adj_inplace_block_kernel, dim3(1), dim3(1024), 0, s, g, zero_idx,
+                           nzero, src_idx, src_flags, off, cdst, csign, nsrc);
+        return launch_status();
+    }
     if (nsrc > 0) {
         hipLaunchKernelGGL(bc_adj_gather_stage_kernel, dim3((unsigned)((nsrc + 255) / 256)),
                            dim3(256), 0, s, g, src_idx, src_flags, off, cdst, csign, nsrc, staging);
         st = launch_status();
         if (st) return st;
     }
-    if (nzero > 0) {
-        hipLaunchKernelGGL(bc_adj_zero_kernel, dim3((unsigned)((nzero + 255) / 256)), dim3(256), 0,
-                           s, g, zero_idx, nzero);
-        st = launch_status();
-        if (st) return st;
-    }
-    if (nsrc > 0) {
-        hipLaunchKernelGGL(bc_adj_write_kernel, dim3((unsigned)((nsrc + 255) / 256)), dim3(256),
-                           0, s, g, src_idx, staging, nsrc);
+    if (nsrc + nzero > 0) {
+        hipLaunchKernelGGL(bc_adj_write_kernel, dim3((unsigned)((nsrc + nzero + 255) / 256)),
+                           dim3(256), 0, s, g, src_idx, staging, nsrc, zero_idx, nzero);
         st = launch_status();
     }
     return st;
@@ -492,5 +593,24 @@ extern "C" int pde_scale_f32(const float *in, float *out, int64_t n, float scale
     if (n == 0) return PDE_OK;
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        as_stream(stream), in, out, n, scale);
+    return launch_status();
+}
+
+extern "C" int pde_bc_adjoint_chunked_inplace_f32(float *g, const int64_t *src_idx,
+                                                  const int32_t *src_flags, const int64_t *off,
+                                                  const int64_t *cdst, const float *csign,
+                                                  const int64_t *item_start,
+                                                  const int64_t *zero_idx,
+                                                  const int64_t *zero_start, int64_t nchunks,
+                                                  void *stream)
+{
+    if (!g || nchunks < 0) return PDE_EINVAL;
+    if (nchunks == 0) return PDE_OK;
+    if (!src_idx || !src_flags || !off || !cdst || !csign || !item_start || !zero_idx ||
+        !zero_start || nchunks > 0x7fffffff)
+        return PDE_EINVAL;
+    hipLaunchKernelGGL(bc_adj_chunk_inplace_kernel, dim3((unsigned)nchunks), dim3(BC_CHUNK), 0,
+                       as_stream(stream), g, src_idx, src_flags, off, cdst, csign, item_start,
+                       zero_idx, zero_start);
     return launch_status();
 }

@@ -16,14 +16,8 @@
 
 namespace pde {
 
-constexpr int BATCH_BC_CAP = 16384;   // ops per grid staged in LDS (64 KiB)
+constexpr int BATCH_BC_CAP = BC_BLOCK_CAP;   // ops per grid staged in LDS
 static_assert(sizeof(pde_batch_grid) == 64, "pde_batch_grid layout");
-
-// Each thread owns ops tid, tid + 1024, ... (BC_UNROLL per pass): all their
-// op records are loaded before any field value, so a pass costs two memory
-// round trips instead of two per op; destinations stay in registers for the
-// scatter.
-constexpr int BC_UNROLL = 8;
 
 __global__ __launch_bounds__(1024) void bc_batch_inplace_kernel(float *__restrict__ u,
                                                                 const pde_batch_grid *grids,
@@ -32,36 +26,9 @@ __global__ __launch_bounds__(1024) void bc_batch_inplace_kernel(float *__restric
 {
     __shared__ float vals[BATCH_BC_CAP];
     const pde_batch_grid G = grids[blockIdx.x];
-    const pde_bc_op *gops = ops + G.op_begin;
-    const int n = (int)(G.op_end - G.op_begin);
-    if (G.op_end - G.op_begin <= 0 || G.op_end - G.op_begin > BATCH_BC_CAP) return;
-    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
-        pde_bc_op op[BC_UNROLL];
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) {
-            const int m = base + j * 1024 + threadIdx.x;
-            if (m < n) op[j] = gops[m];
-        }
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) {
-            const int m = base + j * 1024 + threadIdx.x;
-            if (m < n) vals[m] = op_eval(op[j], u, t);
-        }
-    }
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
-        int64_t dst[BC_UNROLL];
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) {
-            const int m = base + j * 1024 + threadIdx.x;
-            if (m < n) dst[j] = gops[m].dst;
-        }
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) {
-            const int m = base + j * 1024 + threadIdx.x;
-            if (m < n) u[dst[j]] = vals[m];
-        }
-    }
+    const int64_t n = G.op_end - G.op_begin;
+    if (n <= 0 || n > BATCH_BC_CAP) return;   // rejected on the host
+    bc_inplace_block(u, ops + G.op_begin, (int)n, t, vals);
 }
 
 template <int RXT, int BYT>

@@ -42,16 +42,36 @@ __global__ __launch_bounds__(256) void bc_scatter_kernel(float *__restrict__ u,
     u[ops[m].dst] = staging[m];
 }
 
-// Small tables (one workgroup's worth) gather and scatter in one launch with a
-// workgroup barrier between the phases.
+// Small tables (up to 4096 ops: latency-bound) gather and scatter in one
+// launch of one workgroup, with a workgroup barrier between the phases.
 __global__ __launch_bounds__(1024) void bc_inplace_small_kernel(float *__restrict__ u,
                                                                 const pde_bc_op *__restrict__ ops,
                                                                 int64_t nops, const float *t)
 {
-    __shared__ float vals[4096];
-    for (int64_t m = threadIdx.x; m < nops; m += 1024) vals[m] = op_eval(ops[m], u, t);
+    __shared__ float vals[BC_BLOCK_CAP];
+    bc_inplace_block(u, ops, (int)nops, t, vals);
+}
+
+// Conflict-closed chunks (host-built): no op reads a scalar that an op of
+// another chunk writes, so each workgroup applies its chunk in place on its
+// own (all reads, workgroup barrier, all writes) and one full-chip launch
+// applies the whole table.
+__global__ __launch_bounds__(BC_CHUNK) void bc_chunk_inplace_kernel(
+    float *__restrict__ u, const pde_bc_op *__restrict__ ops,
+    const int64_t *__restrict__ chunk_start, const float *t)
+{
+    const int64_t b = chunk_start[blockIdx.x], e = chunk_start[blockIdx.x + 1];
+    const int64_t m = b + threadIdx.x;
+    const bool on = m < e && e - b <= BC_CHUNK;
+    float v = 0.0f;
+    int64_t dst = 0;
+    if (on) {
+        const pde_bc_op op = ops[m];
+        v = op_eval(op, u, t);
+        dst = op.dst;
+    }
     __syncthreads();
-    for (int64_t m = threadIdx.x; m < nops; m += 1024) u[ops[m].dst] = vals[m];
+    if (on) u[dst] = v;
 }
 
 }  // namespace pde
@@ -88,5 +108,17 @@ extern "C" int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t 
     int st = launch_status();
     if (st) return st;
     hipLaunchKernelGGL(bc_scatter_kernel, grid, block, 0, s, u, ops, nops, staging);
+    return launch_status();
+}
+
+extern "C" int pde_bc_apply_chunked_inplace_f32(float *u, const pde_bc_op *ops,
+                                                const int64_t *chunk_start, int64_t nchunks,
+                                                const float *t, void *stream)
+{
+    if (!u || nchunks < 0) return PDE_EINVAL;
+    if (nchunks == 0) return PDE_OK;
+    if (!ops || !chunk_start || nchunks > 0x7fffffff) return PDE_EINVAL;
+    hipLaunchKernelGGL(bc_chunk_inplace_kernel, dim3((unsigned)nchunks), dim3(BC_CHUNK), 0,
+                       as_stream(stream), u, ops, chunk_start, t);
     return launch_status();
 }

@@ -101,6 +101,48 @@ __device__ __forceinline__ float op_eval(const pde_bc_op &op, const float *__res
     }
 }
 
+// In-place boundary application by one workgroup of 1024 threads: every op's
+// value is computed from the untouched field into LDS (vals, n <= its
+// capacity), then the workgroup barrier, then the stores.  Each thread's ops
+// (tid, tid + 1024, ...) are loaded BC_UNROLL at a time before any field value,
+// so a pass costs two memory round trips, not two per op.
+constexpr int BC_BLOCK_CAP = 16384;   // ops (64 KiB of LDS)
+constexpr int BC_CHUNK = 256;         // ops per conflict-closed chunk (one workgroup)
+constexpr int BC_UNROLL = 8;
+
+__device__ __forceinline__ void bc_inplace_block(float *__restrict__ u,
+                                                 const pde_bc_op *__restrict__ ops, int n,
+                                                 const float *t, float *vals)
+{
+    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
+        pde_bc_op op[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) op[j] = ops[m];
+        }
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) vals[m] = op_eval(op[j], u, t);
+        }
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024 * BC_UNROLL) {
+        int64_t dst[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) dst[j] = ops[m].dst;
+        }
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) {
+            const int m = base + j * 1024 + (int)threadIdx.x;
+            if (m < n) u[dst[j]] = vals[m];
+        }
+    }
+}
+
 // 2D scalar row-tile kernels of the fused steps and their transposes
 // (steps2d.hip).  a/b/c2/u2 are inputs, o1/o2/gc2 outputs:
 //   HEAT:     a = u2, o1 = u_next

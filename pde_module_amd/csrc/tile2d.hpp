@@ -76,9 +76,14 @@ __device__ __forceinline__ void col_edges(const float *__restrict__ f, const Til
 {
     left = __shfl_up(c.w, 1, cw);
     right = __shfl_down(c.x, 1, cw);
-    if (sl == 0) left = (k0 > 0 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 - 1] : 0.0f;
-    if (sl == cw - 1)
-        right = (k0 + 4 < t.P2 && x >= 0 && x < t.P1) ? f[x * t.P2 + k0 + 4] : 0.0f;
+    // both segment edges load in one instruction (the other lanes idle)
+    const bool lo = sl == 0, hi = sl == cw - 1;
+    if (lo || hi) {
+        const int64_t k = lo ? k0 - 1 : k0 + 4;
+        const float v = (k >= 0 && k < t.P2 && x >= 0 && x < t.P1) ? f[x * t.P2 + k] : 0.0f;
+        if (lo) left = v;
+        if (hi) right = v;
+    }
 }
 
 // Laplacian sum at element i of a lane: rows up/mid/down, column edges.

@@ -336,6 +336,11 @@ __global__ void counter_add_kernel(int64_t *c, int64_t delta)
     if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + delta;
 }
 
+__global__ void counter_copy_kernel(const int64_t *src, int64_t *dst, int64_t delta)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0] + delta;
+}
+
 }  // namespace pde
 
 extern "C" int pde_gather_step_f32(const float *field, const int64_t *idx, float *out, int64_t n,
@@ -363,6 +368,14 @@ extern "C" int pde_counter_add(int64_t *counter, int64_t delta, void *stream)
 {
     if (!counter) return PDE_EINVAL;
     hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(64), 0, as_stream(stream), counter,
+                       delta);
+    return launch_status();
+}
+
+extern "C" int pde_counter_copy(const int64_t *src, int64_t *dst, int64_t delta, void *stream)
+{
+    if (!src || !dst) return PDE_EINVAL;
+    hipLaunchKernelGGL(counter_copy_kernel, dim3(1), dim3(64), 0, as_stream(stream), src, dst,
                        delta);
     return launch_status();
 }

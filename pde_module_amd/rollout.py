@@ -183,11 +183,55 @@ class WaveRollout:
         self._nrec = nrec
         self._rec = torch.empty((self.steps, max(nrec, 1)), dtype=like.dtype, device=dev)
         self._grec = torch.zeros_like(self._rec)
-        self._ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+        # two step counters, used alternately by the fused 2D steps (slot 0
+        # only by the generic path)
+        self._ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+        self._fused = sum(1 for s_ in shape[:3] if s_ > 1) == 2 and shape[3] == 1
+        if self._fused:
+            self._build_point_sets(shape)
         self._graphs = {}
 
+    def _build_point_sets(self, shape):
+        """Row-indexed source / receiver lists of the fused 2D steps (struct
+        pde_point_set): points sorted by padded row, list order kept within a
+        row."""
+        p1, p2 = [int(s_) for s_ in shape[:3] if s_ > 1]
+        dev = self._dev
+        self._pts_keep = []
+
+        def build(idx, data, stride, scale):
+            e = idx.detach().cpu().numpy().astype(np.int64)
+            if e.size and (e.min() < 0 or e.max() >= p1 * p2):
+                raise IndexError("point index outside the field")
+            rows, cols = e // p2, e % p2
+            order = np.argsort(rows, kind="stable")
+            row_start = np.searchsorted(rows[order], np.arange(p1 + 1)).astype(np.int32)
+            arrs = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
+                    (row_start, cols[order].astype(np.int32), order.astype(np.int32))]
+            self._pts_keep += arrs + [data]
+            rec = np.zeros(1, N.POINT_SET_DTYPE)
+            rec[0] = (arrs[0].data_ptr(), arrs[1].data_ptr(), arrs[2].data_ptr(),
+                      data.data_ptr(), stride, np.float32(scale), 0)
+            self._pts_keep.append(rec)
+            return rec
+
+        self._ps_src = (build(self.src_idx, self.src_wavelet, self.src_wavelet.shape[1], self.dt)
+                        if self.src_idx is not None else None)
+        if self._nrec:
+            self._ps_rec = build(self.rec_idx, self._rec, self._rec.shape[1], 1.0)
+            self._ps_grec = build(self.rec_idx, self._grec, self._grec.shape[1], 1.0)
+        else:
+            self._ps_rec = self._ps_grec = None
+
+    @staticmethod
+    def _ps(rec):
+        return rec.ctypes.data if rec is not None else None
+
     def _set_counter(self, n):
-        self._ctr.fill_(int(n))
+        self._ctr[0].fill_(int(n))
+
+    def _c(self, slot):
+        return self._ctr.data_ptr() + 8 * slot
 
     # ---------------------------------------------------------- step pieces
     def _step(self, u, v, c2, out, record):
@@ -205,26 +249,72 @@ class WaveRollout:
                    self._rec.data_ptr(), self._nrec, self._rec.shape[1], self._ctr.data_ptr(), st)
         N.call("pde_counter_add", self._ctr.data_ptr(), 1, st)
 
+    def _step_fused(self, u, v, c2, out, record, slot):
+        """One launch per step after the boundary (2D scalar fields): source,
+        u/v update, receivers and the counter move slot -> 1 - slot."""
+        self.bc.apply_inplace(u)
+        N.call("pde_wave_step_fused_f32", u.data_ptr(), v.data_ptr(),
+               c2.data_ptr() if c2 is not None else None, out.data_ptr(), v.data_ptr(),
+               *_grid(u), self.g, *self.w, self.alpha, self.dt, self._c(slot),
+               self._c(1 - slot), self._ps(self._ps_src),
+               self._ps(self._ps_rec) if record else None, N.stream_of(u))
+
+    def _run_steps(self, pairs, c2, record):
+        """Steps over (u, out) pairs with v = the segment's velocity buffer;
+        the counter starts and ends in slot 0."""
+        if not self._fused:
+            for u, v, out in pairs:
+                self._step(u, v, c2, out, record)
+            return
+        slot = 0
+        for u, v, out in pairs:
+            self._step_fused(u, v, c2, out, record, slot)
+            slot = 1 - slot
+        if slot:
+            N.call("pde_counter_copy", self._c(1), self._c(0), 0, N.stream_of(self._ctr))
+
     def _fwd_segment(self, L, c2):
         """L steps from self._a / self._v; the state ends in self._a."""
         a, b = self._a, self._b
+        pairs = []
         for _ in range(L):
-            self._step(a, self._v, c2, b, True)
+            pairs.append((a, self._v, b))
             a, b = b, a
+        self._run_steps(pairs, c2, True)
         if a is not self._a:
             _copy(a, self._a)
 
     def _hist_segment(self, i0, L, c2, record):
         """L steps from hist[i0] / vt writing hist[i0 + i + 1]; afterwards
         hist[i0 + i] holds u2 of that step."""
-        for i in range(i0, i0 + L):
-            self._step(self._hist[i], self._vt, c2, self._hist[i + 1], record)
+        self._run_steps([(self._hist[i], self._vt, self._hist[i + 1])
+                         for i in range(i0, i0 + L)], c2, record)
 
     def _adj_segment(self, i0, L, c2, need_c2):
         """L transposed steps on hist[i0 + L - 1] .. hist[i0], last to first.
         The gradients live in lu / lv on entry and on exit."""
         st = N.stream_of(self._lu)
         lu, lv, gu2, gvn = self._lu, self._lv, self._gu2, self._gvn
+        if self._fused:
+            # counter n0 + L in slot 0 -> n0 + L - 1 in slot 1, then down by one
+            # per step, alternating slots
+            N.call("pde_counter_copy", self._c(0), self._c(1), -1, st)
+            slot = 1
+            for i in reversed(range(i0, i0 + L)):
+                N.call("pde_wave_step_adjoint_fused_f32", lu.data_ptr(), lv.data_ptr(),
+                       c2.data_ptr() if c2 is not None else None, self._hist[i].data_ptr(),
+                       gu2.data_ptr(), gvn.data_ptr(),
+                       self._gc2.data_ptr() if need_c2 else None, *_grid(lu), self.g, *self.w,
+                       self.alpha, self.dt, self._c(slot), self._c(1 - slot),
+                       self._ps(self._ps_grec), st)
+                slot = 1 - slot
+                self.bc.table.adjoint_inplace(gu2)
+                lu, gu2 = gu2, lu
+                lv, gvn = gvn, lv
+            if lu is not self._lu:
+                _copy(lu, self._lu)
+                _copy(lv, self._lv)
+            return
         N.call("pde_counter_add", self._ctr.data_ptr(), -1, st)
         for i in reversed(range(i0, i0 + L)):
             if self._nrec:

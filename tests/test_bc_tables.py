@@ -134,3 +134,76 @@ def test_out_of_grid_tables_raise_on_host():
     tab.min_index = int(min(ops["dst"].min(), src.min()))
     with pytest.raises(IndexError):
         tab._check_bounds(64)
+
+
+def _chunk_cases():
+    cases = []
+    u = field((9, 8, 1), 1)
+    b = LegacyGridBoundary(u, 0.1, 1)
+    cases.append(("legacy2d", u, b))
+    u = field((7, 8, 9), 3)
+    b = LegacyGridBoundary(u, 0.1, 1)
+    b.impermeable("+Z")
+    b.vonNeumann_BC("-X", 0.5)
+    cases.append(("legacy3d", u, b))
+    u = field((30, 28, 1), 2)
+    b = GridBoundary(u, 0.1, 1)
+    b.no_slip("ALL")
+    b.dirichlet_BC("+Y", 1.0, 0)
+    cases.append(("lid", u, b))
+    u = field((300, 200, 1), 1)
+    b = LegacyGridBoundary(u, 0.1, 1)
+    cases.append(("large", u, b))
+    return cases
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_chunked_tables_are_conflict_closed_and_exact(case):
+    """Chunked in-place application (each chunk reads, then writes, chunks one
+    after the other) equals the two-phase apply; the chunked transpose applied
+    the same way equals the transpose oracle."""
+    from pde_module_amd.FDM.boundary.tables import CHUNK, chunk_ops, chunk_transpose, transpose_table
+    name, u, bc = _chunk_cases()[case]
+    ops = bc.resolved_ops()
+    ch = chunk_ops(ops)
+    assert ch is not None
+    cops, starts = ch
+    assert sorted(map(tuple, cops.tolist())) == sorted(map(tuple, ops.tolist()))
+    assert np.all(np.diff(starts) <= CHUNK) and starts[-1] == len(ops)
+    x = np.random.default_rng(1).uniform(-1, 1, u.shape).astype(np.float32)
+    flat = x.ravel().copy()
+    for c in range(len(starts) - 1):
+        part = cops[starts[c]:starts[c + 1]]
+        vals = apply_ops_host(flat, part).ravel()[part["dst"]]   # reads of the current field
+        flat[part["dst"]] = vals
+    assert same_bits(flat.reshape(x.shape), apply_ops_host(x, ops))
+
+    T = transpose_table(ops)
+    C = chunk_transpose(T)
+    assert C is not None and np.all(np.diff(C["item_start"]) <= CHUNK)
+    gflat = np.random.default_rng(2).uniform(-1, 1, x.size).astype(np.float32)
+    want = O.boundary_linear_adjoint(gflat.copy(), [(int(o["dst"]), int(o["src"]),
+                                      {0: 0, 1: 1, 2: -1, 3: 1}[int(o["kind"])]) for o in ops])
+    g = gflat.copy()
+    for c in range(len(C["item_start"]) - 1):
+        i0, i1 = C["item_start"][c], C["item_start"][c + 1]
+        staged = []
+        for i in range(i0, i1):
+            acc = np.float32(0.0) if C["flags"][i] & 1 else g[C["src_idx"][i]]
+            for q in range(C["off"][i], C["off"][i + 1]):
+                acc = np.float32(acc + C["csign"][q] * g[C["cdst"][q]])
+            staged.append(acc)
+        g[C["src_idx"][i0:i1]] = staged
+        g[C["zero_idx"][C["zero_start"][c]:C["zero_start"][c + 1]]] = 0.0
+    assert same_bits(g, want.ravel())
+
+
+def test_chunking_refuses_oversized_components():
+    from pde_module_amd._native import BC_COPY, BC_OP_DTYPE
+    from pde_module_amd.FDM.boundary.tables import chunk_ops
+    ops = np.zeros(300, BC_OP_DTYPE)
+    ops["dst"] = np.arange(300)
+    ops["src"] = np.arange(1, 301)          # a chain: each op reads the next one's target
+    ops["kind"] = BC_COPY
+    assert chunk_ops(ops) is None
+    assert chunk_ops(ops[:200]) is not None

@@ -131,3 +131,23 @@ def test_boundary_adjoint_matches_oracle_and_dot(dev, legacy):
     lhs = float(np.sum(out.cpu().numpy().astype(np.float64) * y))
     rhs = float(np.sum(x.astype(np.float64) * gin.cpu().numpy()))
     assert abs(lhs - rhs) <= 1e-5 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("shape,C", [((14, 13, 1), 2), ((70, 70, 70), 1)])
+def test_boundary_adjoint_inplace_equals_out_of_place(dev, shape, C):
+    """In-place transpose (one workgroup below 16384 sources, stage + store
+    kernels above: the 70^3 case) equals the out-of-place one bit for bit."""
+    print(f"bc adjoint in place {shape} C={C}", flush=True)
+    rng = np.random.default_rng(8)
+    x = T(rng.uniform(-1, 1, shape + (C,)).astype(np.float32), dev)
+    bc = LegacyGridBoundary(x, 0.1, 1)
+    bc.vonNeumann_BC("-X", 0.5)
+    bc._ensure_table(x)
+    y = T(rng.uniform(-1, 1, shape + (C,)).astype(np.float32), dev)
+    want = torch.empty_like(y)
+    bc.table.adjoint(y, want)
+    got = y.clone()
+    bc.table.adjoint_inplace(got)
+    assert same_bits(got.cpu().numpy(), want.cpu().numpy())
+    assert same_bits(want.cpu().numpy(), O.boundary_linear_adjoint(y.cpu().numpy(),
+                                                                  _writes(bc.table.host_ops)))

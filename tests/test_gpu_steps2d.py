@@ -102,3 +102,98 @@ def test_wave_adjoint_tile(dev, shape, g, first, with_c2, with_gc2):
     assert same_bits(ogv.cpu().numpy(), gv)
     want_gc2 = gc2_0 + dc2 if with_gc2 else gc2_0
     assert same_bits(d["gc2"].cpu().numpy(), want_gc2)
+
+
+def _point_set(idx, p2, data, scale):
+    e = np.asarray(idx, np.int64)
+    rows, cols = e // p2, e % p2
+    order = np.argsort(rows, kind="stable")
+    return rows, cols, order
+
+
+@pytest.mark.parametrize("shape", [(40, 37, 1), (1, 70, 300)])
+def test_fused_wave_steps_match_separate_kernels(dev, shape):
+    """pde_wave_step_fused_f32 / pde_wave_step_adjoint_fused_f32 against the
+    separate source, step, receiver and counter kernels, over 5 steps and back."""
+    print(f"fused wave steps {shape}", flush=True)
+    p1, p2 = [s for s in shape if s > 1]
+    rng = np.random.default_rng(9)
+    u, v, c2 = (T(x, dev) for x in _fields(shape, 3, 11))
+    steps, g = 5, 1
+    src = np.array([5 * p2 + 7, 2 * p2 + 3, 5 * p2 + 7, (p1 - 3) * p2 + p2 - 4])
+    rec = np.array([1 * p2 + 1, 6 * p2 + 9, 3 * p2 + 4, (p1 - 2) * p2 + 2])
+    wav = T(rng.uniform(-1, 1, (steps, len(src))).astype(np.float32), dev)
+    w = O.laplacian_weights(0.1)
+    alpha, dt = np.float32(0.7), np.float32(0.05)
+    keep = []
+
+    def ps(idx, data, stride, scale):
+        rows, cols, order = _point_set(idx, p2, data, scale)
+        rs = np.searchsorted(rows[order], np.arange(p1 + 1)).astype(np.int32)
+        arrs = [T(rs, dev), T(cols[order].astype(np.int32), dev), T(order.astype(np.int32), dev)]
+        keep.extend(arrs)
+        r = np.zeros(1, N.POINT_SET_DTYPE)
+        r[0] = (arrs[0].data_ptr(), arrs[1].data_ptr(), arrs[2].data_ptr(), data.data_ptr(),
+                stride, np.float32(scale), 0)
+        keep.append(r)
+        return r.ctypes.data
+
+    st = N.stream_of(u)
+    # fused path
+    fu, fv = u.clone(), v.clone()
+    fo = torch.empty_like(u)
+    frec = torch.zeros(steps, len(rec), device=dev)
+    ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+    ps_src, ps_rec = ps(src, wav, len(src), dt), ps(rec, frec, len(rec), 1.0)
+    slot = 0
+    for n in range(steps):
+        N.call("pde_wave_step_fused_f32", fu.data_ptr(), fv.data_ptr(), c2.data_ptr(),
+               fo.data_ptr(), fv.data_ptr(), *shape, g, *w, alpha, dt,
+               ctr.data_ptr() + 8 * slot, ctr.data_ptr() + 8 * (1 - slot), ps_src, ps_rec, st)
+        fu, fo = fo, fu
+        slot = 1 - slot
+    # separate kernels (distinct source indices there: the duplicate is summed
+    # by two single-source calls in list order)
+    su, sv = u.clone(), v.clone()
+    so = torch.empty_like(u)
+    srec = torch.zeros(steps, len(rec), device=dev)
+    c1 = torch.zeros(1, dtype=torch.int64, device=dev)
+    sidx = T(src, dev)
+    ridx = T(rec, dev)
+    for n in range(steps):
+        for j in range(len(src)):
+            N.call("pde_scatter_add_step_f32", sv.data_ptr(), sidx[j:].data_ptr(),
+                   wav[:, j:].data_ptr(), 1, len(src), dt, c1.data_ptr(), st)
+        N.call("pde_wave_step_ctr_f32", su.data_ptr(), sv.data_ptr(), c2.data_ptr(),
+               so.data_ptr(), sv.data_ptr(), *shape, g, *w, alpha, dt, c1.data_ptr(), st)
+        N.call("pde_gather_step_f32", so.data_ptr(), ridx.data_ptr(), srec.data_ptr(), len(rec),
+               len(rec), c1.data_ptr(), st)
+        N.call("pde_counter_add", c1.data_ptr(), 1, st)
+        su, so = so, su
+    assert same_bits(fu.cpu().numpy(), su.cpu().numpy())
+    assert same_bits(fv.cpu().numpy(), sv.cpu().numpy())
+    assert same_bits(frec.cpu().numpy(), srec.cpu().numpy())
+    assert int(ctr[slot]) == steps == int(c1[0])
+
+    # transposes, one step from n = steps - 1
+    gun, gvn, u2 = (T(x, dev) for x in _fields(shape, 3, 12))
+    grec = T(rng.uniform(-1, 1, (steps, len(rec))).astype(np.float32), dev)
+    outs = [torch.empty_like(u) for _ in range(4)]
+    gc_f, gc_s = torch.zeros_like(u), torch.zeros_like(u)
+    ctr.fill_(0)
+    ctr[1] = steps - 1
+    ps_grec = ps(rec, grec, len(rec), 1.0)
+    N.call("pde_wave_step_adjoint_fused_f32", gun.data_ptr(), gvn.data_ptr(), c2.data_ptr(),
+           u2.data_ptr(), outs[0].data_ptr(), outs[1].data_ptr(), gc_f.data_ptr(), *shape, g, *w,
+           alpha, dt, ctr.data_ptr() + 8, ctr.data_ptr(), ps_grec, st)
+    c1.fill_(steps - 1)
+    gun2 = gun.clone()
+    N.call("pde_scatter_add_step_f32", gun2.data_ptr(), ridx.data_ptr(), grec.data_ptr(),
+           len(rec), len(rec), np.float32(1.0), c1.data_ptr(), st)
+    N.call("pde_wave_step_adjoint_ctr_f32", gun2.data_ptr(), gvn.data_ptr(), c2.data_ptr(),
+           u2.data_ptr(), outs[2].data_ptr(), outs[3].data_ptr(), gc_s.data_ptr(), *shape, g, *w,
+           alpha, dt, c1.data_ptr(), st)
+    assert same_bits(outs[0].cpu().numpy(), outs[2].cpu().numpy())
+    assert same_bits(outs[1].cpu().numpy(), outs[3].cpu().numpy())
+    assert same_bits(gc_f.cpu().numpy(), gc_s.cpu().numpy())
+    assert int(ctr[0]) == steps - 2
