@@ -3,14 +3,18 @@ one ghost layer), one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-A step is one periodic Laplacian application over the whole grid: the periodic
-ghost fill (a surface-sized launch) followed by the Laplacian kernel.  The
-inputs (s ~ U[-1, 1], seeded) are resident in HBM before timing starts.
-Printed: one JSON line with GLUPS (interior lattice updates per second / 1e9),
-the Laplacian kernel's achieved bandwidth against the 8 TB/s HBM peak (HIP
-events on the launch stream, algorithmic 8 bytes per update), the Grad, Div and
-fused lap+grad+div passes on the same grid, and the CPU restatement timed on a
-bounded slab of the same workload on this host.
+A step is one periodic Laplacian application over the whole grid: one launch
+of the wrapped-index Laplacian (PeriodicLaplacian), which gives bit for bit the
+values of a periodic ghost fill followed by the drop-in Laplacian module.  The
+drop-in path itself (PeriodicBoundary + Laplacian, two modules, four launches)
+is timed beside it.  The inputs (s ~ U[-1, 1], seeded) are resident in HBM
+before timing starts.  Printed: one JSON line with GLUPS (interior lattice
+updates per second / 1e9), the step kernel's achieved bandwidth against the
+8 TB/s HBM peak (HIP events on the launch stream, algorithmic 8 bytes per
+update), the drop-in Laplacian kernel's, the Grad, Div and fused lap+grad+div
+passes on the same grid, the 2048^2 wave workload's forward+adjoint vs forward
+time, and the CPU restatement timed on a bounded slab of the same workload on
+this host.
 """
 from __future__ import annotations
 
@@ -40,20 +44,28 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--quick", action="store_true",
+                   help="operators only: skip the wave and ensemble workloads (profiling runs)")
     return p.parse_args()
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def pmc_traffic():
-    """HBM bytes per Laplacian launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when no summary is present."""
+    """HBM bytes per launch of the 3D Laplacian kernels from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration): {kernel: bytes}, or None when no summary is
+    present."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "lap3d_512_pmc.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return float(d["lap3d_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
+        return {k: float(v) for k, v in d["bytes_per_launch"].items()}
+    except (OSError, KeyError, ValueError, AttributeError):
         return None
 
 
@@ -223,10 +235,14 @@ def main():
     periodic(s)
     periodic(v)
     lap = pm.Laplacian(1, 1.0 / N, G)
+    plap = pm.PeriodicLaplacian(1, 1.0 / N, G)
     grad = pm.Grad(1, (P, P, P), 1.0 / N, G)
     div = pm.Divergence(3, (P, P, P), 1.0 / N, G)
 
     def step():
+        return plap(s, 1.0)
+
+    def dropin_step():
         periodic(s)
         return lap(s, 1.0)
 
@@ -255,8 +271,11 @@ def main():
         return a.elapsed_time(b) / reps * 1e-3
 
     reps = max(10, args.steps)
+    t_step_kernel = event_time(step, reps)
+    step_gbs = LAP_BYTES * LUPS / t_step_kernel / 1e9
     t_lap = event_time(lambda: lap(s, 1.0), reps)
     lap_gbs = LAP_BYTES * LUPS / t_lap / 1e9
+    t_dropin = event_time(dropin_step, reps)
     out_l = torch.empty_like(s)
     out_g = torch.empty_like(v)
     out_d = torch.empty_like(s)
@@ -268,7 +287,14 @@ def main():
                                         s.numel(), NV.stream_of(s)), reps // 2)
     copy_gbs = 2 * 4 * s.numel() / t_copy / 1e9
 
-    wave = wave_adjoint_timing(dev)
+    progress(f"operators timed: step {t_step_kernel * 1e6:.1f} us, drop-in laplacian "
+             f"{t_lap * 1e6:.1f} us")
+    wave = ensemble = None
+    if not args.quick:
+        wave = wave_adjoint_timing(dev)
+        progress("wave forward / forward+adjoint timed")
+        ensemble = heat_ensemble_timing(dev)
+        progress("heat ensemble timed")
     traffic = pmc_traffic()
     rec = {
         "metric": "GLUPS & % HBM peak (3D 7-pt FP32); fwd+adjoint vs fwd step time",
@@ -284,16 +310,24 @@ def main():
         "dtype": "f32",
         "data": "synthetic: s ~ U[-1,1] (torch.Generator seed 0), periodic ghosts",
         "config": {"workload": "3D 7-point Laplacian, 512^3 periodic grid (514^3 padded, g=1), "
-                               "step = periodic ghost fill + Laplacian",
+                               "step = one periodic Laplacian launch (wrapped neighbour "
+                               "indices; equal bit for bit to periodic ghost fill + Laplacian)",
                    "grid_padded": [P, P, P], "interior_points": LUPS,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(lap_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": "lap3d_tile", "avg_launch_us": round(t_lap * 1e6, 2),
+        "roofline": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("lap3d_band<true>") if traffic else None,
+                     "kernel": "lap3d_band<true> (periodic)",
+                     "avg_launch_us": round(t_step_kernel * 1e6, 2),
                      "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
         "extra": {
-            "lap_glups_kernel_only": round(LUPS / t_lap / 1e9, 1),
+            "dropin_laplacian_kernel": {
+                "kernel": "lap3d_band<false>", "us": round(t_lap * 1e6, 2),
+                "GBs": round(lap_gbs, 1), "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
+                "glups": round(LUPS / t_lap / 1e9, 1),
+                "traffic": traffic.get("lap3d_band<false>") if traffic else None},
+            "dropin_step_fill_plus_laplacian": {
+                "us": round(t_dropin * 1e6, 2), "glups": round(LUPS / t_dropin / 1e9, 1)},
             "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1)},
             "div": {"us": round(t_div * 1e6, 1), "GBs": round(DIV_BYTES * LUPS / t_div / 1e9, 1)},
             "fused_lap_grad_div": {"us": round(t_fused * 1e6, 1),
@@ -301,8 +335,9 @@ def main():
                                    "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},
             "copy_GBs": round(copy_gbs, 1),
             "wave2048_fwd_vs_fwd_adjoint": wave,
+            "heat_ensemble_512": ensemble,
         },
-        "fwd_adjoint_vs_fwd": wave["ratio"],
+        "fwd_adjoint_vs_fwd": wave["ratio"] if wave else None,
     }
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)

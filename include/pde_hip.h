@@ -47,12 +47,22 @@ int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_t ny, int64
                       float w_plus, float alpha, void *stream);
 
 /* The same Laplacian through the straightforward one-thread-per-element
- * kernel only (pde_laplacian_f32 switches 3D scalar fields to an x-marching
- * kernel).  Same results bit for bit; kept so the two can be compared and
+ * kernel only (pde_laplacian_f32 switches 3D scalar fields with at least 4
+ * interior rows to an x-marching tile kernel).  Same results bit for bit; kept so the two can be compared and
  * timed side by side.  Replaces the same reference call as pde_laplacian_f32. */
 int pde_laplacian_generic_f32(const float *in, float *out, int64_t nx, int64_t ny,
                               int64_t nz, int32_t ncomp, int32_t ghost, float w_minus,
                               float w_center, float w_plus, float alpha, void *stream);
+
+/* Periodic Laplacian: the values pde_laplacian_f32 gives after
+ * pde_periodic_fill_f32, computed from interior values only (a neighbour index
+ * that leaves the interior wraps to the opposite interior index); no ghost is
+ * read or written.  No reference counterpart (the reference has no periodic
+ * boundary); it serves periodic grids such as the 3D operator benchmark,
+ * whose per-step ghost fill it makes unnecessary. */
+int pde_laplacian_periodic_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                               int64_t nz, int32_t ncomp, int32_t ghost, float w_minus,
+                               float w_center, float w_plus, float alpha, void *stream);
 
 /* Central gradient of a vec(ncomp) field into an ncomp x D matrix per node
  * (D = number of eligible axes; for ncomp == 1 this is also the vec(D)

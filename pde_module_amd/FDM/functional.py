@@ -45,6 +45,21 @@ def laplacian(kernel, input_array, alpha, output_array, ghost_cells, device=None
     return output_array
 
 
+def laplacian_periodic(kernel, input_array, alpha, output_array, ghost_cells, device=None):
+    """``laplacian`` on a periodic grid: every neighbour index that leaves the
+    interior wraps to the opposite interior index, so the input's ghost layers
+    are neither read nor written (same values as a periodic ghost fill followed
+    by ``laplacian``).  No reference counterpart (the reference has no periodic
+    boundary)."""
+    _check(kernel, input_array, output_array, "laplacian")
+    X, Y, Z = _grid(input_array)
+    wm, wc, wp = kernel.weights
+    N.call("pde_laplacian_periodic_f32", N.dev_ptr(input_array), N.dev_ptr(output_array),
+           X, Y, Z, int(kernel.in_shape[0]), int(ghost_cells), wm, wc, wp, np.float32(alpha),
+           N.stream_of(input_array))
+    return output_array
+
+
 def grad(kernel, input_array, alpha, output_array, ghost_cells, device=None):
     _check(kernel, input_array, output_array, "grad")
     X, Y, Z = _grid(input_array)

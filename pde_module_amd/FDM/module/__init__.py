@@ -1,8 +1,8 @@
 from .explicitUniformGridStencil import ExplicitUniformGridStencil
-from .laplacian import Laplacian
+from .laplacian import Laplacian, PeriodicLaplacian
 from .grad import Grad
 from .divergence import Divergence
 from .elementWiseOps import OuterProduct, scalarVectorMult
 
-__all__ = ["ExplicitUniformGridStencil", "Laplacian", "Grad", "Divergence", "OuterProduct",
+__all__ = ["ExplicitUniformGridStencil", "Laplacian", "PeriodicLaplacian", "Grad", "Divergence", "OuterProduct",
            "scalarVectorMult"]

@@ -4,7 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 from ...stencil import setup
-from ..functional import laplacian
+from ..functional import laplacian, laplacian_periodic
 from ..kernel import create_Laplacian_kernel
 from .explicitUniformGridStencil import ExplicitUniformGridStencil
 
@@ -42,3 +42,15 @@ class Laplacian(ExplicitUniformGridStencil):
     def forward(self, input_array, alpha: float = 1.0, *args, **kwargs):
         return laplacian(self.kernel, input_array, alpha, self.output_array, self.ghost_cells,
                          device=self.device)
+
+
+class PeriodicLaplacian(Laplacian):
+    """``Laplacian`` on a periodic grid (no reference counterpart: the reference
+    has no periodic boundary).  ``PeriodicLaplacian(inputs, dx, ghost_cells)``;
+    ``lap(u, alpha)`` gives the values ``Laplacian`` gives after
+    ``PeriodicBoundary(ghost_cells)(u)``, reading interior values only: u's
+    ghost layers are neither read nor written."""
+
+    def forward(self, input_array, alpha: float = 1.0, *args, **kwargs):
+        return laplacian_periodic(self.kernel, input_array, alpha, self.output_array,
+                                  self.ghost_cells, device=self.device)

@@ -7,10 +7,11 @@ Fields are FP32 torch tensors on the GPU of padded shape (X, Y, Z) + element
 shape (C,) or (N, D).
 """
 from . import _native
-from .FDM import Divergence, Grad, GridBoundary, Laplacian, PeriodicBoundary
+from .FDM import (Divergence, Grad, GridBoundary, Laplacian, PeriodicBoundary,
+                  PeriodicLaplacian)
 from .time_step import ForwardEuler, HeatStep, WaveStep
 from .mesh import UniformGridMesh, create_structured_field
 
-__all__ = ["Laplacian", "Grad", "Divergence", "GridBoundary", "PeriodicBoundary",
+__all__ = ["Laplacian", "PeriodicLaplacian", "Grad", "Divergence", "GridBoundary", "PeriodicBoundary",
            "ForwardEuler", "HeatStep", "WaveStep", "UniformGridMesh",
            "create_structured_field"]

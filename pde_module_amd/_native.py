@@ -20,6 +20,7 @@ _P, _I64, _I32, _F = C.c_void_p, C.c_int64, C.c_int32, C.c_float
 SIGNATURES = {
     "pde_laplacian_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_laplacian_generic_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_laplacian_periodic_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_grad_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_divergence_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_fused_lap_grad_div_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32,

@@ -67,30 +67,45 @@ __global__ __launch_bounds__(256) void lap_generic(const float *__restrict__ in,
         }
 }
 
-// 3D scalar Laplacian, x-marching row tiles.
+// 3D scalar Laplacian: x-marching row tiles in an XCD band order.
 //
-// A block owns TB rows (y) x 512 points (z) and walks NXC planes along the
-// slowest axis x, two planes per step.  Each lane holds 4 consecutive z
+// A block owns LB_TB rows (y) x 512 points (z) of LB_NXC consecutive planes
+// (x) and walks them one plane per step.  Each lane holds 4 consecutive z
 // points, so every global access is a 16-byte load or store (unaligned by the
-// ghost offset, which the hardware serves); planes x-1 .. x+2 stay in
-// registers, so each plane is read from memory once; the y neighbours of a row
-// come from the rows of the same block through LDS (only the tile's top and
-// bottom halo rows are loaded), and the z neighbours from the adjacent lanes by
-// shuffles (lane 0 and lane 63 of a wave load their outer one).  Tiles are
-// numbered z-fastest, then y, then x.
+// ghost offset, which the hardware serves).  Planes x-1, x, x+1 ride in
+// registers; plane x+2, the next plane's tile-edge halo rows and its row-end
+// columns are loaded before the step's barrier, so their latency overlaps the
+// current plane.  y neighbours come through LDS from the rows of the block (the
+// tile's top and bottom rows load their outer halo row), z neighbours by
+// shuffles (a wave's end lanes load their outer column).  Outputs are written
+// with non-temporal stores: nothing in the launch reads them back.
+//
+// Block order: hardware block b runs on XCD b % 8 (blocks are dealt
+// round-robin over the 8 XCDs, each with its own L2).  Within every x-chunk
+// the tiles are split into 8 contiguous bands, one per XCD, so the halo rows
+// that vertically adjacent tiles share are read through one L2, while all XCDs
+// sweep the same x-chunks at the same time.  Short x-chunks (4 planes) keep
+// many independent blocks per CU to cover the per-plane barrier.  A padding
+// slot of the last band exits at once.
+//
+// PER (periodic) replaces every neighbour index that leaves the interior by
+// the opposite interior index instead of the ghost, which gives the values of
+// a periodic ghost fill followed by the plain kernel without touching ghosts.
 //
 // Lanes whose 4 points run past the interior use scalar loads and stores;
 // lanes and rows entirely outside it still take part in the barriers.
-struct Tile3 {
+struct Band3 {
     int64_t SX, SY;        // plane and row pitch (floats)
     int64_t E0, E1, E2;    // interior extents
     int64_t g;
-    uint32_t tz, ty;       // tile counts along z and y
+    uint32_t ty;           // tiles along y
+    uint32_t T;            // tiles per x-chunk (z tiles x y tiles)
+    uint32_t per;          // tiles per XCD band (ceil(T / 8))
 };
 
-constexpr int LAP_TB = 8;      // rows per block
-constexpr int LAP_NXC = 64;    // planes per block
-constexpr int LAP_TZ = 512;    // z points per block (128 lanes x 4)
+constexpr int LB_TB = 4;       // rows per block
+constexpr int LB_NXC = 4;      // planes per block
+constexpr int LB_TZ = 512;     // z points per block (128 lanes x 4)
 
 typedef float pde_f4 __attribute__((ext_vector_type(4)));
 typedef float pde_f4u __attribute__((ext_vector_type(4), aligned(4)));
@@ -104,6 +119,11 @@ __device__ __forceinline__ pde_f4 ld4(const float *p)
 __device__ __forceinline__ void st4(float *p, pde_f4 v)
 {
     *reinterpret_cast<pde_f4u *>(p) = pde_f4u{v.x, v.y, v.z, v.w};
+}
+
+__device__ __forceinline__ void st4_nt(float *p, pde_f4 v)
+{
+    __builtin_nontemporal_store(pde_f4u{v.x, v.y, v.z, v.w}, reinterpret_cast<pde_f4u *>(p));
 }
 
 // Loads the n (1..3) valid points of a partial lane; the rest read as 0.
@@ -138,117 +158,161 @@ __device__ __forceinline__ float lap7(float xm, float xc, float xp, float ym, fl
     return alpha * acc;
 }
 
-// One output plane of a lane: centre cc with x neighbours cm / cp, y
-// neighbours lo / hi, z neighbours from the shuffles (zl, zr).
-__device__ __forceinline__ void lap_lane_plane(float *__restrict__ out, const float *__restrict__ in,
-                                               int64_t ec, int mode, int np, int64_t nk, int lane,
-                                               pde_f4 cm, pde_f4 cc, pde_f4 cp, pde_f4 lo,
-                                               pde_f4 hi, float zl, float zr, float wm, float wc,
-                                               float wp, float alpha)
+// Interior index of the neighbour i (-1 .. E) of an interior index along an
+// axis of interior extent E: the ghost index itself, or the wrapped one.
+template <bool PER>
+__device__ __forceinline__ int64_t nb_index(int64_t i, int64_t E)
 {
-    if (lane == 0) zl = in[ec - 1];
-    if (mode == 2) {
-        if (lane == 63 || nk == 4) zr = in[ec + 4];
-        pde_f4 o;
-        o.x = lap7(cm.x, cc.x, cp.x, lo.x, hi.x, zl, cc.y, wm, wc, wp, alpha);
-        o.y = lap7(cm.y, cc.y, cp.y, lo.y, hi.y, cc.x, cc.z, wm, wc, wp, alpha);
-        o.z = lap7(cm.z, cc.z, cp.z, lo.z, hi.z, cc.y, cc.w, wm, wc, wp, alpha);
-        o.w = lap7(cm.w, cc.w, cp.w, lo.w, hi.w, cc.z, zr, wm, wc, wp, alpha);
-        st4(out + ec, o);
-        return;
-    }
-    // partial lane: points 0 .. np-1; the right neighbour of the last one is
-    // the ghost column after the interior
-    const float c[4] = {cc.x, cc.y, cc.z, cc.w};
-    const float a[4] = {cm.x, cm.y, cm.z, cm.w};
-    const float p[4] = {cp.x, cp.y, cp.z, cp.w};
-    const float l[4] = {lo.x, lo.y, lo.z, lo.w};
-    const float h[4] = {hi.x, hi.y, hi.z, hi.w};
-    const float zlast = in[ec + np];
-    for (int t = 0; t < np; ++t) {
-        const float zm = t == 0 ? zl : c[t > 0 ? t - 1 : 0];
-        const float zp = t == np - 1 ? zlast : c[t + 1 < 4 ? t + 1 : 3];
-        out[ec + t] = lap7(a[t], c[t], p[t], l[t], h[t], zm, zp, wm, wc, wp, alpha);
-    }
+    if (!PER) return i;
+    return i < 0 ? i + E : (i >= E ? i - E : i);
 }
 
-__global__ __launch_bounds__(128 * LAP_TB) void lap3d_tile(const float *__restrict__ in,
-                                                          float *__restrict__ out, Tile3 m,
+template <bool PER>
+__global__ __launch_bounds__(128 * LB_TB) void lap3d_band(const float *__restrict__ in,
+                                                          float *__restrict__ out, Band3 m,
                                                           float wm, float wc, float wp,
                                                           float alpha)
 {
-    __shared__ pde_f4 rows[2][2][LAP_TB][128];
-    uint32_t b = blockIdx.x;
-    const uint32_t zt = b % m.tz;
-    b /= m.tz;
-    const uint32_t yt = b % m.ty;
-    const uint32_t xt = b / m.ty;
+    __shared__ pde_f4 rows[2][LB_TB][128];
+    const uint32_t b = blockIdx.x, slot = b >> 3;
+    const uint32_t xt = slot / m.per;
+    const uint32_t lt = (b & 7u) * m.per + slot % m.per;
+    if (lt >= m.T) return;                                // whole block: no barrier skipped
+    const uint32_t yt = lt % m.ty, zt = lt / m.ty;
     const int q = threadIdx.x;          // lane within the 128-lane row group
     const int w = threadIdx.y;          // row within the tile
     const int lane = q & 63;
-    const int64_t k0 = (int64_t)zt * LAP_TZ + 4 * q;        // first interior z of the lane
+    const int64_t k0 = (int64_t)zt * LB_TZ + 4 * q;         // first interior z of the lane
     const int64_t nk = m.E2 - k0;                            // valid points from k0
     const int mode = nk >= 4 ? 2 : (nk > 0 ? 1 : 0);
     const int np = (int)(nk < 4 ? (nk > 0 ? nk : 0) : 4);
-    const int64_t j = (int64_t)yt * LAP_TB + w;
-    int64_t rows_here = m.E1 - (int64_t)yt * LAP_TB;
-    if (rows_here > LAP_TB) rows_here = LAP_TB;
-    const bool act = w < rows_here && mode != 0;
-    const bool top = w == 0, bot = w == rows_here - 1;
-    const int64_t hoff = top ? -m.SY : m.SY;                 // halo row of an edge row
-    const int64_t x0 = (int64_t)xt * LAP_NXC;
-    int64_t nx = m.E0 - x0;
-    if (nx > LAP_NXC) nx = LAP_NXC;
-    int64_t e = (x0 + m.g) * m.SX + (j + m.g) * m.SY + (k0 + m.g);
+    // the last tile along y is shifted up to end at the last row, so every
+    // tile has LB_TB rows (E1 >= LB_TB); rows it shares with the tile before
+    // are written twice with the same values
+    int64_t j0 = (int64_t)yt * LB_TB;
+    if (j0 > m.E1 - LB_TB) j0 = m.E1 - LB_TB;
+    const int64_t j = j0 + w;
+    const bool act = mode != 0;
+    const bool top = w == 0, bot = w == LB_TB - 1;
+    // z neighbours from memory: lane 0 its left one, a wave's last lane or the
+    // lane holding the row's last point its right one.  One of them is
+    // prefetched with the planes; the right one of a lane 0 that also holds
+    // the row's last point (a z tile of at most 4 points) is loaded when used.
+    const bool zl_ld = act && lane == 0;
+    const bool ze_ld = zl_ld || (act && (lane == 63 || nk <= 4));
+    const int64_t g = m.g;
+    const int64_t x0 = (int64_t)xt * LB_NXC;
+    const int nx = (int)(m.E0 - x0 < LB_NXC ? m.E0 - x0 : LB_NXC);
+    // float offsets within a plane: the lane's centre, its tile-edge halo row,
+    // its memory-read z neighbour
+    const int64_t rc = (j + g) * m.SY + (k0 + g);
+    const int64_t rh = (nb_index<PER>(top ? j - 1 : j + 1, m.E1) + g) * m.SY + (k0 + g);
+    const int64_t rz = (j + g) * m.SY + nb_index<PER>(zl_ld ? k0 - 1 : k0 + np, m.E2) + g;
+    auto plane = [&](int64_t x) -> int64_t { return (nb_index<PER>(x, m.E0) + g) * m.SX; };
     const pde_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    pde_f4 xm = zero, xc = zero;
+    pde_f4 xm = zero, xc = zero, xp = zero, h = zero;
+    float ze = 0.0f;
     if (act) {
-        xm = ld4_mode(in + e - m.SX, mode, np);
-        xc = ld4_mode(in + e, mode, np);
+        const int64_t p0 = plane(x0);
+        xm = ld4_mode(in + plane(x0 - 1) + rc, mode, np);
+        xc = ld4_mode(in + p0 + rc, mode, np);
+        xp = ld4_mode(in + plane(x0 + 1) + rc, mode, np);
+        if (top || bot) h = ld4_mode(in + p0 + rh, mode, np);
+        if (ze_ld) ze = in[p0 + rz];
     }
     int buf = 0;
-    for (int64_t x = 0; x < nx; x += 2, e += 2 * m.SX) {
-        const bool two = x + 1 < nx;     // uniform over the block
-        pde_f4 p1 = zero, p2 = zero, h0 = zero, h1 = zero, h0b = zero, h1b = zero;
-        if (act) {
-            p1 = ld4_mode(in + e + m.SX, mode, np);
-            if (two) p2 = ld4_mode(in + e + 2 * m.SX, mode, np);
-            if (top || bot) {
-                h0 = ld4_mode(in + e + hoff, mode, np);
-                if (two) h1 = ld4_mode(in + e + m.SX + hoff, mode, np);
-            }
-            if (top && bot) {            // a one-row tile needs both halos
-                h0b = ld4_mode(in + e + m.SY, mode, np);
-                if (two) h1b = ld4_mode(in + e + m.SX + m.SY, mode, np);
-            }
+    for (int x = 0; x < nx; ++x) {
+        const int64_t xa = x0 + x;                  // interior plane of this step
+        pde_f4 xpp = zero, hn = zero;
+        float zen = 0.0f;
+        if (act && x + 1 < nx) {                     // x + 1 < nx is uniform over the block
+            const int64_t p1 = plane(xa + 1);
+            xpp = ld4_mode(in + plane(xa + 2) + rc, mode, np);
+            if (top || bot) hn = ld4_mode(in + p1 + rh, mode, np);
+            if (ze_ld) zen = in[p1 + rz];
         }
-        rows[buf][0][w][q] = xc;
-        rows[buf][1][w][q] = p1;
+        rows[buf][w][q] = xc;
         __syncthreads();
-        {
-            float zl = __shfl_up(xc.w, 1, 64);
-            float zr = __shfl_down(xc.x, 1, 64);
-            if (act) {
-                const pde_f4 lo = top ? h0 : rows[buf][0][w > 0 ? w - 1 : 0][q];
-                const pde_f4 hi = bot ? (top ? h0b : h0) : rows[buf][0][w + 1 < LAP_TB ? w + 1 : w][q];
-                lap_lane_plane(out, in, e, mode, np, nk, lane, xm, xc, p1, lo, hi, zl, zr, wm,
-                               wc, wp, alpha);
+        float l = __shfl_up(xc.w, 1, 64);
+        float r = __shfl_down(xc.x, 1, 64);
+        if (act) {
+            if (zl_ld) l = ze;
+            else if (ze_ld) r = ze;
+            if (zl_ld && nk <= 4)
+                r = in[(xa + g) * m.SX + (j + g) * m.SY + nb_index<PER>(k0 + np, m.E2) + g];
+            const pde_f4 lo = top ? h : rows[buf][w > 0 ? w - 1 : 0][q];
+            const pde_f4 hi = bot ? h : rows[buf][w + 1 < LB_TB ? w + 1 : w][q];
+            float *o = out + (xa + g) * m.SX + rc;
+            if (mode == 2) {
+                pde_f4 v;
+                v.x = lap7(xm.x, xc.x, xp.x, lo.x, hi.x, l, xc.y, wm, wc, wp, alpha);
+                v.y = lap7(xm.y, xc.y, xp.y, lo.y, hi.y, xc.x, xc.z, wm, wc, wp, alpha);
+                v.z = lap7(xm.z, xc.z, xp.z, lo.z, hi.z, xc.y, xc.w, wm, wc, wp, alpha);
+                v.w = lap7(xm.w, xc.w, xp.w, lo.w, hi.w, xc.z, r, wm, wc, wp, alpha);
+                st4_nt(o, v);
+            } else {
+                const float c[4] = {xc.x, xc.y, xc.z, xc.w};
+                const float a[4] = {xm.x, xm.y, xm.z, xm.w};
+                const float p[4] = {xp.x, xp.y, xp.z, xp.w};
+                const float lv[4] = {lo.x, lo.y, lo.z, lo.w};
+                const float hv[4] = {hi.x, hi.y, hi.z, hi.w};
+                for (int t = 0; t < np; ++t) {
+                    const float zm = t == 0 ? l : c[t > 0 ? t - 1 : 0];
+                    const float zp = t == np - 1 ? r : c[t + 1 < 4 ? t + 1 : 3];
+                    o[t] = lap7(a[t], c[t], p[t], lv[t], hv[t], zm, zp, wm, wc, wp, alpha);
+                }
             }
         }
-        if (two) {
-            float zl = __shfl_up(p1.w, 1, 64);
-            float zr = __shfl_down(p1.x, 1, 64);
-            if (act) {
-                const pde_f4 lo = top ? h1 : rows[buf][1][w > 0 ? w - 1 : 0][q];
-                const pde_f4 hi = bot ? (top ? h1b : h1) : rows[buf][1][w + 1 < LAP_TB ? w + 1 : w][q];
-                lap_lane_plane(out, in, e + m.SX, mode, np, nk, lane, xc, p1, p2, lo, hi, zl,
-                               zr, wm, wc, wp, alpha);
-            }
-        }
-        xm = p1;
-        xc = p2;
+        xm = xc;
+        xc = xp;
+        xp = xpp;
+        h = hn;
+        ze = zen;
         buf ^= 1;
     }
+}
+
+// Generic periodic Laplacian (any ncomp, 1..3 eligible axes): thread per
+// interior element; a neighbour index that leaves the interior wraps to the
+// opposite interior index.
+template <int NDIM>
+__global__ __launch_bounds__(256) void lap_generic_periodic(const float *__restrict__ in,
+                                                            float *__restrict__ out, RowGeom g,
+                                                            int64_t comps, float wm, float wc,
+                                                            float wp, float alpha)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= g.R) return;
+    const int64_t k = r / comps, E2 = g.R / comps;
+    const int64_t rm = (k == 0 ? r + g.R : r) - comps;             // wrapped z neighbours
+    const int64_t rp = (k == E2 - 1 ? r - g.R : r) + comps;
+    for (int64_t i0 = blockIdx.z; i0 < g.E0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < g.E1; i1 += gridDim.y) {
+            const int64_t row = g.base + i0 * g.S0 + i1 * g.S1;
+            const int64_t e = row + r;
+            const float c = in[e];
+            float acc = 0.0f;
+            if (NDIM >= 3) {
+                const int64_t a = (i0 == 0 ? g.E0 - 1 : i0 - 1) - i0;
+                const int64_t b = (i0 == g.E0 - 1 ? 0 : i0 + 1) - i0;
+                acc += wm * in[e + a * g.S0];
+                acc += wc * c;
+                acc += wp * in[e + b * g.S0];
+            }
+            if (NDIM >= 2) {
+                const int64_t a = (i1 == 0 ? g.E1 - 1 : i1 - 1) - i1;
+                const int64_t b = (i1 == g.E1 - 1 ? 0 : i1 + 1) - i1;
+                acc += wm * in[e + a * g.S1];
+                acc += wc * c;
+                acc += wp * in[e + b * g.S1];
+            }
+            if (NDIM >= 1) {
+                acc += wm * in[row + rm];
+                acc += wc * c;
+                acc += wp * in[row + rp];
+            }
+            out[e] = alpha * acc;
+        }
 }
 
 // ------------------------------------------------------------------ Grad
@@ -481,6 +545,40 @@ static dim3 row_grid(const RowGeom &rg)
 
 }  // namespace pde
 
+namespace pde {
+
+// The band kernel needs at least LB_TB interior rows (every tile has LB_TB
+// rows); smaller grids take the generic kernels.
+static bool lap3d_band_fits(const Geom &G) { return G.E[1] >= LB_TB; }
+
+template <bool PER>
+static int lap3d_band_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
+                             float wc, float wp, float alpha, hipStream_t s)
+{
+    Band3 m;
+    m.SX = G.S[0];
+    m.SY = G.S[1];
+    m.E0 = G.E[0];
+    m.E1 = G.E[1];
+    m.E2 = G.E[2];
+    m.g = ghost;
+    const int64_t tz = (G.E[2] + LB_TZ - 1) / LB_TZ;
+    const int64_t ty = (G.E[1] + LB_TB - 1) / LB_TB;
+    const int64_t T = tz * ty;
+    const int64_t per = (T + 7) / 8;
+    const int64_t nxt = (G.E[0] + LB_NXC - 1) / LB_NXC;
+    const int64_t nb = 8 * per * nxt;
+    if (ty > 0xffffffffLL || T > 0xffffffffLL || nb > 0x7fffffff) return PDE_ESHAPE;
+    m.ty = (uint32_t)ty;
+    m.T = (uint32_t)T;
+    m.per = (uint32_t)per;
+    hipLaunchKernelGGL(lap3d_band<PER>, dim3((unsigned)nb), dim3(128, LB_TB), 0, s, in, out, m,
+                       wm, wc, wp, alpha);
+    return launch_status();
+}
+
+}  // namespace pde
+
 using namespace pde;
 
 extern "C" int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_t ny,
@@ -492,23 +590,8 @@ extern "C" int pde_laplacian_f32(const float *in, float *out, int64_t nx, int64_
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     hipStream_t s = as_stream(stream);
-    if (G.ndim == 3 && ncomp == 1) {
-        Tile3 m;
-        m.SX = G.S[0];
-        m.SY = G.S[1];
-        m.E0 = G.E[0];
-        m.E1 = G.E[1];
-        m.E2 = G.E[2];
-        m.g = ghost;
-        m.tz = (uint32_t)((G.E[2] + LAP_TZ - 1) / LAP_TZ);
-        m.ty = (uint32_t)((G.E[1] + LAP_TB - 1) / LAP_TB);
-        const int64_t tx = (G.E[0] + LAP_NXC - 1) / LAP_NXC;
-        const int64_t nb = (int64_t)m.tz * m.ty * tx;
-        if (nb > 0x7fffffff) return PDE_ESHAPE;
-        hipLaunchKernelGGL(lap3d_tile, dim3((unsigned)nb), dim3(128, LAP_TB), 0, s, in, out, m,
-                           wm, wc, wp, alpha);
-        return launch_status();
-    }
+    if (G.ndim == 3 && ncomp == 1 && lap3d_band_fits(G))
+        return lap3d_band_launch<false>(in, out, G, ghost, wm, wc, wp, alpha, s);
     RowGeom rg = row_geom(G, ncomp);
     dim3 grid = row_grid(rg), block(256);
     switch (G.ndim) {
@@ -538,6 +621,32 @@ extern "C" int pde_laplacian_generic_f32(const float *in, float *out, int64_t nx
     case 1: hipLaunchKernelGGL(lap_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
     case 2: hipLaunchKernelGGL(lap_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
     default: hipLaunchKernelGGL(lap_generic<3>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha); break;
+    }
+    return launch_status();
+}
+
+// Periodic Laplacian: as pde_laplacian_f32 on a field whose ghost layers were
+// filled periodically (pde_periodic_fill_f32), without reading or writing any
+// ghost.  Every eligible interior extent must be >= 1.
+extern "C" int pde_laplacian_periodic_f32(const float *in, float *out, int64_t nx, int64_t ny,
+                                          int64_t nz, int32_t ncomp, int32_t ghost, float wm,
+                                          float wc, float wp, float alpha, void *stream)
+{
+    if (!in || !out || ncomp < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    hipStream_t s = as_stream(stream);
+    if (G.ndim == 3 && ncomp == 1 && lap3d_band_fits(G))
+        return lap3d_band_launch<true>(in, out, G, ghost, wm, wc, wp, alpha, s);
+    RowGeom rg = row_geom(G, ncomp);
+    dim3 grid = row_grid(rg), block(256);
+    const int64_t C = ncomp;
+    switch (G.ndim) {
+    case 1: hipLaunchKernelGGL(lap_generic_periodic<1>, grid, block, 0, s, in, out, rg, C, wm, wc, wp, alpha); break;
+    case 2: hipLaunchKernelGGL(lap_generic_periodic<2>, grid, block, 0, s, in, out, rg, C, wm, wc, wp, alpha); break;
+    default: hipLaunchKernelGGL(lap_generic_periodic<3>, grid, block, 0, s, in, out, rg, C, wm, wc, wp, alpha); break;
     }
     return launch_status();
 }

@@ -7,7 +7,7 @@ import torch
 from conftest import same_bits
 from helpers import ghost_free_mask
 from oracle import np_oracle as O
-from pde_module_amd import Divergence, Grad, Laplacian, PeriodicBoundary
+from pde_module_amd import Divergence, Grad, Laplacian, PeriodicBoundary, PeriodicLaplacian
 import pde_module_amd._native as N
 from pde_module_amd.FDM import functional as F
 
@@ -85,10 +85,12 @@ def test_output_ghosts_untouched(dev):
 
 
 @pytest.mark.parametrize("shape", [(131, 67, 200), (20, 19, 1030), (5, 3, 4), (34, 9, 517),
-                                   (70, 33, 515), (3, 3, 3), (9, 17, 1)])
+                                   (70, 33, 515), (3, 3, 3), (9, 17, 1), (10, 6, 4), (10, 6, 6),
+                                   (7, 13, 1030), (12, 6, 514), (9, 7, 7), (6, 11, 516)])
 def test_tiled_kernel_equals_generic_kernel(dev, shape):
-    """The 3D tiled Laplacian (partial lanes, partial row tiles, several z
-    tiles, short x chunks) against the one-thread-per-point kernel."""
+    """The 3D tiled Laplacian (partial lanes, a last row tile shifted onto the
+    one before, several z tiles, z tiles of 1..4 points, short x chunks)
+    against the one-thread-per-point kernel."""
     print(f"tiled vs generic {shape}", flush=True)
     u = torch.rand(shape + (1,), device=dev) * 2 - 1
     a = torch.zeros_like(u)
@@ -126,6 +128,33 @@ def test_periodic_fill_matches_oracle(dev, corc, shape, g):
     assert same_bits(got, want)
 
 
+PERIODIC_SHAPES = [((10, 11, 12), 1, 1), ((34, 9, 517), 1, 1), ((7, 13, 1030), 1, 1),
+                   ((12, 10, 9), 2, 1), ((10, 6, 4), 1, 1), ((5, 3, 4), 1, 1), ((10, 12, 1), 1, 1),
+                   ((9, 10, 1), 2, 3), ((1, 6, 5), 1, 2), ((8, 1, 1), 1, 1), ((11, 9, 8), 1, 3)]
+
+
+@pytest.mark.parametrize("shape,g,C", PERIODIC_SHAPES)
+def test_periodic_laplacian_matches_fill_then_laplacian(dev, corc, shape, g, C):
+    """PeriodicLaplacian == periodic ghost fill + Laplacian (the C oracle), bit
+    for bit on the interior; the input (ghosts included) and the output's
+    ghosts are left untouched."""
+    print(f"periodic laplacian shape={shape} g={g} C={C}", flush=True)
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-1, 1, shape + (C,)).astype(np.float32)
+    dx, alpha = 0.041, -0.7
+    lap = PeriodicLaplacian(C, dx, g)
+    ud = to_dev(u, dev)
+    lap.setup(ud)
+    lap.output_array.fill_(float("nan"))
+    got = lap(ud, alpha).cpu().numpy()
+    filled = corc.periodic_fill(u.copy(), g)
+    want = corc.laplacian(filled, lap.stencil, np.float32(alpha), g)
+    mask = ghost_free_mask(shape, g, (C,))
+    assert same_bits(got, want, mask)
+    assert np.all(np.isnan(got[~mask]))
+    assert same_bits(ud.cpu().numpy(), u)
+
+
 def test_full_size_periodic_laplacian_matches_oracle(dev, corc):
     """The 512^3 periodic benchmark grid (514^3 padded) against the OpenMP C
     restatement, bit for bit on all 134M interior points."""
@@ -141,3 +170,9 @@ def test_full_size_periodic_laplacian_matches_oracle(dev, corc):
     want = corc.laplacian(uh, lap.stencil, np.float32(1.0), g, nthreads=min(16, corc.max_threads()))
     inner = (slice(1, -1),) * 3
     assert np.array_equal(got[inner].view(np.uint32), want[inner].view(np.uint32))
+    # the wrapped-index kernel on the same grid, ghosts left stale
+    u[0].zero_()
+    u[:, -1].zero_()
+    u[:, :, 0].zero_()
+    per = PeriodicLaplacian(1, 1.0 / n, g)(u, 1.0).cpu().numpy()
+    assert np.array_equal(per[inner].view(np.uint32), want[inner].view(np.uint32))
