@@ -167,13 +167,99 @@ __device__ __forceinline__ int64_t nb_index(int64_t i, int64_t E)
     return i < 0 ? i + E : (i >= E ? i - E : i);
 }
 
-template <bool PER>
-__global__ __launch_bounds__(128 * LB_TB) void lap3d_band(const float *__restrict__ in,
-                                                          float *__restrict__ out, Band3 m,
-                                                          float wm, float wc, float wp,
-                                                          float alpha)
+// Weights of the band kernels: Laplacian (w, alpha) and central difference
+// (c, alpha of Grad and of Div).
+struct BandW {
+    float lm, lc, lp, la;
+    float cm, cc, cp, ga, da;
+};
+
+enum { BAND_LAP = 1, BAND_GRAD = 2, BAND_DIV = 4 };
+
+// Loads the 3 * n floats of n (1..3) vec3 nodes of a partial lane into the
+// three 16-byte words of a full lane's layout; the rest read as 0.
+__device__ __forceinline__ void ld12_part(const float *p, int n, pde_f4 &a, pde_f4 &b, pde_f4 &c)
 {
-    __shared__ pde_f4 rows[2][LB_TB][128];
+    float t[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) t[i] = i < 3 * n ? p[i] : 0.0f;
+    a = pde_f4{t[0], t[1], t[2], t[3]};
+    b = pde_f4{t[4], t[5], t[6], t[7]};
+    c = pde_f4{t[8], t[9], t[10], t[11]};
+}
+
+__device__ __forceinline__ void ld12_mode(const float *p, int mode, int np, pde_f4 &a, pde_f4 &b,
+                                          pde_f4 &c)
+{
+    if (mode == 2) {
+        a = ld4(p);
+        b = ld4(p + 4);
+        c = ld4(p + 8);
+    } else {
+        ld12_part(p, np, a, b, c);
+    }
+}
+
+// Components of the 4 vec3 nodes held in three 16-byte words (a, b, c).
+__device__ __forceinline__ pde_f4 comp_x(pde_f4 a, pde_f4 b, pde_f4 c) { return pde_f4{a.x, a.w, b.z, c.y}; }
+__device__ __forceinline__ pde_f4 comp_y(pde_f4 a, pde_f4 b, pde_f4 c) { return pde_f4{a.y, b.x, b.w, c.z}; }
+__device__ __forceinline__ pde_f4 comp_z(pde_f4 a, pde_f4 b, pde_f4 c) { return pde_f4{a.z, b.y, c.x, c.w}; }
+
+// The y components of the n valid nodes of a vec3 row segment (strided loads).
+__device__ __forceinline__ pde_f4 ld_comp_y(const float *p, int mode, int np)
+{
+    pde_f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int n = mode == 2 ? 4 : np;
+    v.x = p[1];
+    if (n > 1) v.y = p[4];
+    if (n > 2) v.z = p[7];
+    if (n > 3) v.w = p[10];
+    return v;
+}
+
+__device__ __forceinline__ float el4(const pde_f4 &v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// acc = ((0 + wm*m) + wc*c) + wp*p: one axis of a central stencil.
+__device__ __forceinline__ float axis3(float m, float c, float p, float wm, float wc, float wp)
+{
+    float acc = 0.0f;
+    acc += wm * m;
+    acc += wc * c;
+    acc += wp * p;
+    return acc;
+}
+
+// 3D band kernel family: OPS selects Laplacian and/or Grad of a scalar s and
+// Divergence of a vec3 v, all on one grid, in one pass (the fused operator
+// step).  The scalar rides as in the Laplacian above: planes x-1, x, x+1 and
+// the prefetched x+2 in registers, y neighbours through LDS, z neighbours by
+// shuffles.  For the vector, a lane holds its 4 nodes as three 16-byte words;
+// only the x components of plane x-1 are kept, the y components of the plane
+// go through LDS (the tile's edge rows load the halo row's y components), the
+// z components cross lanes by shuffles.  Grad writes each node's 3 slots, a
+// lane's 12 floats as three 16-byte stores.
+// The vector path's three planes of 12 floats make the Div and fused variants
+// register-heavy (84 and 137 VGPRs: 5 and 3 waves per SIMD); capping them at
+// 6 and 4 waves spills to scratch and measured slower (519 vs 454 us, 1136 vs
+// 957 us at 512^3), so no occupancy is forced.
+template <int OPS, bool PER>
+__global__ __launch_bounds__(128 * LB_TB) void ops3d_band(const float *__restrict__ s,
+                                                          const float *__restrict__ v,
+                                                          float *__restrict__ lap,
+                                                          float *__restrict__ grad,
+                                                          float *__restrict__ div, Band3 m,
+                                                          BandW W)
+{
+    constexpr bool SC = (OPS & (BAND_LAP | BAND_GRAD)) != 0;   // scalar path
+    constexpr bool VE = (OPS & BAND_DIV) != 0;                 // vector path
+    __shared__ pde_f4 rows[SC ? 2 : 1][LB_TB][128];
+    __shared__ pde_f4 rowsv[VE ? 2 : 1][LB_TB][128];
+    // Grad output staging: a wave's 64 x 12 floats, so that each of its three
+    // store instructions writes 1 KB contiguously
+    __shared__ pde_f4 gstage[(OPS & BAND_GRAD) ? 2 * LB_TB : 1][192];
     const uint32_t b = blockIdx.x, slot = b >> 3;
     const uint32_t xt = slot / m.per;
     const uint32_t lt = (b & 7u) * m.per + slot % m.per;
@@ -200,74 +286,187 @@ __global__ __launch_bounds__(128 * LB_TB) void lap3d_band(const float *__restric
     // the row's last point (a z tile of at most 4 points) is loaded when used.
     const bool zl_ld = act && lane == 0;
     const bool ze_ld = zl_ld || (act && (lane == 63 || nk <= 4));
+    // every lane of the wave holds 4 points (uniform over the wave)
+    const bool wave_full = __all(mode == 2);
     const int64_t g = m.g;
     const int64_t x0 = (int64_t)xt * LB_NXC;
     const int nx = (int)(m.E0 - x0 < LB_NXC ? m.E0 - x0 : LB_NXC);
-    // float offsets within a plane: the lane's centre, its tile-edge halo row,
+    // node offsets within a plane: the lane's centre, its tile-edge halo row,
     // its memory-read z neighbour
     const int64_t rc = (j + g) * m.SY + (k0 + g);
     const int64_t rh = (nb_index<PER>(top ? j - 1 : j + 1, m.E1) + g) * m.SY + (k0 + g);
     const int64_t rz = (j + g) * m.SY + nb_index<PER>(zl_ld ? k0 - 1 : k0 + np, m.E2) + g;
     auto plane = [&](int64_t x) -> int64_t { return (nb_index<PER>(x, m.E0) + g) * m.SX; };
     const pde_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    // scalar state
     pde_f4 xm = zero, xc = zero, xp = zero, h = zero;
     float ze = 0.0f;
+    // vector state: x components of plane x-1, planes x and x+1 as 3 words,
+    // the halo row's y components, the memory-read z neighbour's z component
+    pde_f4 vxm = zero, vca = zero, vcb = zero, vcc = zero, vpa = zero, vpb = zero, vpc = zero;
+    pde_f4 vh = zero;
+    float vze = 0.0f;
     if (act) {
-        const int64_t p0 = plane(x0);
-        xm = ld4_mode(in + plane(x0 - 1) + rc, mode, np);
-        xc = ld4_mode(in + p0 + rc, mode, np);
-        xp = ld4_mode(in + plane(x0 + 1) + rc, mode, np);
-        if (top || bot) h = ld4_mode(in + p0 + rh, mode, np);
-        if (ze_ld) ze = in[p0 + rz];
+        const int64_t p0 = plane(x0), pm1 = plane(x0 - 1), pp1 = plane(x0 + 1);
+        if (SC) {
+            xm = ld4_mode(s + pm1 + rc, mode, np);
+            xc = ld4_mode(s + p0 + rc, mode, np);
+            xp = ld4_mode(s + pp1 + rc, mode, np);
+            if (top || bot) h = ld4_mode(s + p0 + rh, mode, np);
+            if (ze_ld) ze = s[p0 + rz];
+        }
+        if (VE) {
+            pde_f4 a, bb, c;
+            ld12_mode(v + 3 * (pm1 + rc), mode, np, a, bb, c);
+            vxm = comp_x(a, bb, c);
+            ld12_mode(v + 3 * (p0 + rc), mode, np, vca, vcb, vcc);
+            ld12_mode(v + 3 * (pp1 + rc), mode, np, vpa, vpb, vpc);
+            if (top || bot) vh = ld_comp_y(v + 3 * (p0 + rh), mode, np);
+            if (ze_ld) vze = v[3 * (p0 + rz) + 2];
+        }
     }
     int buf = 0;
     for (int x = 0; x < nx; ++x) {
         const int64_t xa = x0 + x;                  // interior plane of this step
-        pde_f4 xpp = zero, hn = zero;
-        float zen = 0.0f;
+        pde_f4 xpp = zero, hn = zero, vqa = zero, vqb = zero, vqc = zero, vhn = zero;
+        float zen = 0.0f, vzen = 0.0f;
         if (act && x + 1 < nx) {                     // x + 1 < nx is uniform over the block
-            const int64_t p1 = plane(xa + 1);
-            xpp = ld4_mode(in + plane(xa + 2) + rc, mode, np);
-            if (top || bot) hn = ld4_mode(in + p1 + rh, mode, np);
-            if (ze_ld) zen = in[p1 + rz];
-        }
-        rows[buf][w][q] = xc;
-        __syncthreads();
-        float l = __shfl_up(xc.w, 1, 64);
-        float r = __shfl_down(xc.x, 1, 64);
-        if (act) {
-            if (zl_ld) l = ze;
-            else if (ze_ld) r = ze;
-            if (zl_ld && nk <= 4)
-                r = in[(xa + g) * m.SX + (j + g) * m.SY + nb_index<PER>(k0 + np, m.E2) + g];
-            const pde_f4 lo = top ? h : rows[buf][w > 0 ? w - 1 : 0][q];
-            const pde_f4 hi = bot ? h : rows[buf][w + 1 < LB_TB ? w + 1 : w][q];
-            float *o = out + (xa + g) * m.SX + rc;
-            if (mode == 2) {
-                pde_f4 v;
-                v.x = lap7(xm.x, xc.x, xp.x, lo.x, hi.x, l, xc.y, wm, wc, wp, alpha);
-                v.y = lap7(xm.y, xc.y, xp.y, lo.y, hi.y, xc.x, xc.z, wm, wc, wp, alpha);
-                v.z = lap7(xm.z, xc.z, xp.z, lo.z, hi.z, xc.y, xc.w, wm, wc, wp, alpha);
-                v.w = lap7(xm.w, xc.w, xp.w, lo.w, hi.w, xc.z, r, wm, wc, wp, alpha);
-                st4_nt(o, v);
-            } else {
-                const float c[4] = {xc.x, xc.y, xc.z, xc.w};
-                const float a[4] = {xm.x, xm.y, xm.z, xm.w};
-                const float p[4] = {xp.x, xp.y, xp.z, xp.w};
-                const float lv[4] = {lo.x, lo.y, lo.z, lo.w};
-                const float hv[4] = {hi.x, hi.y, hi.z, hi.w};
-                for (int t = 0; t < np; ++t) {
-                    const float zm = t == 0 ? l : c[t > 0 ? t - 1 : 0];
-                    const float zp = t == np - 1 ? r : c[t + 1 < 4 ? t + 1 : 3];
-                    o[t] = lap7(a[t], c[t], p[t], lv[t], hv[t], zm, zp, wm, wc, wp, alpha);
-                }
+            const int64_t p1 = plane(xa + 1), p2 = plane(xa + 2);
+            if (SC) {
+                xpp = ld4_mode(s + p2 + rc, mode, np);
+                if (top || bot) hn = ld4_mode(s + p1 + rh, mode, np);
+                if (ze_ld) zen = s[p1 + rz];
+            }
+            if (VE) {
+                ld12_mode(v + 3 * (p2 + rc), mode, np, vqa, vqb, vqc);
+                if (top || bot) vhn = ld_comp_y(v + 3 * (p1 + rh), mode, np);
+                if (ze_ld) vzen = v[3 * (p1 + rz) + 2];
             }
         }
-        xm = xc;
-        xc = xp;
-        xp = xpp;
-        h = hn;
-        ze = zen;
+        const pde_f4 vy = comp_y(vca, vcb, vcc), vz = comp_z(vca, vcb, vcc);
+        if (SC) rows[buf][w][q] = xc;
+        if (VE) rowsv[buf][w][q] = vy;
+        __syncthreads();
+        float l = 0.0f, r = 0.0f, vl = 0.0f, vr = 0.0f;
+        if (SC) {
+            l = __shfl_up(xc.w, 1, 64);
+            r = __shfl_down(xc.x, 1, 64);
+        }
+        if (VE) {
+            vl = __shfl_up(vz.w, 1, 64);
+            vr = __shfl_down(vz.x, 1, 64);
+        }
+        if (act) {
+            const int64_t pc = (xa + g) * m.SX + rc;   // the lane's first output node
+            const bool late_r = zl_ld && nk <= 4;      // lane 0 holding the row's last point
+            const int64_t late = (xa + g) * m.SX + (j + g) * m.SY + nb_index<PER>(k0 + np, m.E2) + g;
+            if (SC) {
+                if (zl_ld) l = ze;
+                else if (ze_ld) r = ze;
+                if (late_r) r = s[late];
+                const pde_f4 lo = top ? h : rows[buf][w > 0 ? w - 1 : 0][q];
+                const pde_f4 hi = bot ? h : rows[buf][w + 1 < LB_TB ? w + 1 : w][q];
+                float o[4], gx[4], gy[4], gz[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float c = el4(xc, t);
+                    const float zm = t == 0 ? l : el4(xc, t > 0 ? t - 1 : 0);
+                    const float zp = t == (mode == 2 ? 3 : np - 1) ? r : el4(xc, t < 3 ? t + 1 : 3);
+                    if (OPS & BAND_LAP)
+                        o[t] = lap7(el4(xm, t), c, el4(xp, t), el4(lo, t), el4(hi, t), zm, zp,
+                                    W.lm, W.lc, W.lp, W.la);
+                    if (OPS & BAND_GRAD) {
+                        gx[t] = W.ga * axis3(el4(xm, t), c, el4(xp, t), W.cm, W.cc, W.cp);
+                        gy[t] = W.ga * axis3(el4(lo, t), c, el4(hi, t), W.cm, W.cc, W.cp);
+                        gz[t] = W.ga * axis3(zm, c, zp, W.cm, W.cc, W.cp);
+                    }
+                }
+                if (mode == 2) {
+                    if (OPS & BAND_LAP) st4_nt(lap + pc, pde_f4{o[0], o[1], o[2], o[3]});
+                    if (OPS & BAND_GRAD) {
+                        const pde_f4 g0 = {gx[0], gy[0], gz[0], gx[1]};
+                        const pde_f4 g1 = {gy[1], gz[1], gx[2], gy[2]};
+                        const pde_f4 g2 = {gz[2], gx[3], gy[3], gz[3]};
+                        if (wave_full) {
+                            // through LDS: lane i then stores floats 4i.. of
+                            // each KB of the wave's 3 KB output
+                            pde_f4 *st = gstage[2 * w + (q >> 6)];
+                            st[3 * lane] = g0;
+                            st[3 * lane + 1] = g1;
+                            st[3 * lane + 2] = g2;
+                            __builtin_amdgcn_wave_barrier();
+                            const pde_f4 s0 = st[lane], s1 = st[64 + lane], s2 = st[128 + lane];
+                            float *gp = grad + 3 * (pc - 4 * lane) + 4 * lane;
+                            st4_nt(gp, s0);
+                            st4_nt(gp + 256, s1);
+                            st4_nt(gp + 512, s2);
+                        } else {
+                            // a wave with partial lanes (the row's last z tile)
+                            float *gp = grad + 3 * pc;
+                            st4_nt(gp, g0);
+                            st4_nt(gp + 4, g1);
+                            st4_nt(gp + 8, g2);
+                        }
+                    }
+                } else {
+                    for (int t = 0; t < np; ++t) {
+                        if (OPS & BAND_LAP) lap[pc + t] = o[t];
+                        if (OPS & BAND_GRAD) {
+                            grad[3 * (pc + t)] = gx[t];
+                            grad[3 * (pc + t) + 1] = gy[t];
+                            grad[3 * (pc + t) + 2] = gz[t];
+                        }
+                    }
+                }
+            }
+            if (VE) {
+                if (zl_ld) vl = vze;
+                else if (ze_ld) vr = vze;
+                if (late_r) vr = v[3 * late + 2];
+                const pde_f4 lo = top ? vh : rowsv[buf][w > 0 ? w - 1 : 0][q];
+                const pde_f4 hi = bot ? vh : rowsv[buf][w + 1 < LB_TB ? w + 1 : w][q];
+                const pde_f4 vxc = comp_x(vca, vcb, vcc), vxp = comp_x(vpa, vpb, vpc);
+                float d[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float zm = t == 0 ? vl : el4(vz, t > 0 ? t - 1 : 0);
+                    const float zp = t == (mode == 2 ? 3 : np - 1) ? vr : el4(vz, t < 3 ? t + 1 : 3);
+                    float acc = 0.0f;
+                    acc += W.cm * el4(vxm, t);
+                    acc += W.cc * el4(vxc, t);
+                    acc += W.cp * el4(vxp, t);
+                    acc += W.cm * el4(lo, t);
+                    acc += W.cc * el4(vy, t);
+                    acc += W.cp * el4(hi, t);
+                    acc += W.cm * zm;
+                    acc += W.cc * el4(vz, t);
+                    acc += W.cp * zp;
+                    d[t] = W.da * acc;
+                }
+                if (mode == 2)
+                    st4_nt(div + pc, pde_f4{d[0], d[1], d[2], d[3]});
+                else
+                    for (int t = 0; t < np; ++t) div[pc + t] = d[t];
+            }
+        }
+        if (SC) {
+            xm = xc;
+            xc = xp;
+            xp = xpp;
+            h = hn;
+            ze = zen;
+        }
+        if (VE) {
+            vxm = comp_x(vca, vcb, vcc);
+            vca = vpa;
+            vcb = vpb;
+            vcc = vpc;
+            vpa = vqa;
+            vpb = vqb;
+            vpc = vqc;
+            vh = vhn;
+            vze = vzen;
+        }
         buf ^= 1;
     }
 }
@@ -551,9 +750,10 @@ namespace pde {
 // rows); smaller grids take the generic kernels.
 static bool lap3d_band_fits(const Geom &G) { return G.E[1] >= LB_TB; }
 
-template <bool PER>
-static int lap3d_band_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
-                             float wc, float wp, float alpha, hipStream_t s)
+template <int OPS, bool PER>
+static int ops3d_band_launch(const float *s_, const float *v, float *lap, float *grad,
+                             float *div, const Geom &G, int ghost, const BandW &W,
+                             hipStream_t s)
 {
     Band3 m;
     m.SX = G.S[0];
@@ -572,9 +772,17 @@ static int lap3d_band_launch(const float *in, float *out, const Geom &G, int gho
     m.ty = (uint32_t)ty;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;
-    hipLaunchKernelGGL(lap3d_band<PER>, dim3((unsigned)nb), dim3(128, LB_TB), 0, s, in, out, m,
-                       wm, wc, wp, alpha);
+    hipLaunchKernelGGL((ops3d_band<OPS, PER>), dim3((unsigned)nb), dim3(128, LB_TB), 0, s, s_, v,
+                       lap, grad, div, m, W);
     return launch_status();
+}
+
+template <bool PER>
+static int lap3d_band_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
+                             float wc, float wp, float alpha, hipStream_t s)
+{
+    BandW W = {wm, wc, wp, alpha, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    return ops3d_band_launch<BAND_LAP, PER>(in, nullptr, out, nullptr, nullptr, G, ghost, W, s);
 }
 
 }  // namespace pde
@@ -661,6 +869,11 @@ extern "C" int pde_grad_f32(const float *in, float *out, int64_t nx, int64_t ny,
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
     hipStream_t s = as_stream(stream);
+    if (G.ndim == 3 && ncomp == 1 && lap3d_band_fits(G)) {
+        const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, alpha, 0.0f};
+        return ops3d_band_launch<BAND_GRAD, false>(in, nullptr, nullptr, out, nullptr, G, ghost,
+                                                   W, s);
+    }
     RowGeom rg = row_geom(G, ncomp);
     dim3 grid = row_grid(rg), block(256);
     switch (G.ndim) {
@@ -681,6 +894,11 @@ extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
     hipStream_t s = as_stream(stream);
+    if (G.ndim == 3 && nrows == 1 && lap3d_band_fits(G)) {
+        const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, 0.0f, alpha};
+        return ops3d_band_launch<BAND_DIV, false>(nullptr, in, nullptr, nullptr, out, G, ghost, W,
+                                                  s);
+    }
     RowGeom rg = row_geom(G, nrows);
     dim3 grid = row_grid(rg), block(256);
     switch (G.ndim) {
@@ -702,6 +920,11 @@ extern "C" int pde_fused_lap_grad_div_f32(const float *s_, const float *v, float
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;
     if (G.ndim != 3) return PDE_EINVAL;
+    if (lap3d_band_fits(G)) {
+        const BandW W = {lm, lc, lp, al, cm, cc, cp, ag, ad};
+        return ops3d_band_launch<BAND_LAP | BAND_GRAD | BAND_DIV, false>(
+            s_, v, lap, grad, div, G, ghost, W, as_stream(stream));
+    }
     constexpr int BY = 4, NXC = 64;
     March3 m;
     m.SX = G.S[0];

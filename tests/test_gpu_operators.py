@@ -128,6 +128,42 @@ def test_periodic_fill_matches_oracle(dev, corc, shape, g):
     assert same_bits(got, want)
 
 
+BAND_SHAPES = [((12, 9, 517), 1), ((7, 13, 1030), 1), ((10, 6, 6), 1), ((10, 6, 4), 1),
+               ((9, 12, 11), 2), ((40, 37, 70), 1), ((6, 4, 9), 1)]
+
+
+@pytest.mark.parametrize("shape,g", BAND_SHAPES)
+def test_band_grad_div_fused_match_oracle(dev, corc, shape, g):
+    """3D scalar Grad, vec3 Divergence and the fused lap+grad+div pass through
+    the band kernels (partial lanes, several z tiles, z tiles of 1..4 points,
+    a shifted last row tile, g = 2) against the C restatement."""
+    print(f"band grad/div/fused shape={shape} g={g}", flush=True)
+    rng = np.random.default_rng(12)
+    dx = 0.029
+    s = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    v = rng.uniform(-1, 1, shape + (3,)).astype(np.float32)
+    grad = Grad(1, shape, dx, g)
+    got_g = grad(to_dev(s, dev), 1.3).cpu().numpy()
+    want_g = corc.grad(s, grad.stencil, np.float32(1.3), g)
+    assert same_bits(got_g, want_g, ghost_free_mask(shape, g, (3,)))
+    div = Divergence(3, shape, dx, g)
+    got_d = div(to_dev(v, dev), -0.8).cpu().numpy()
+    want_d = corc.divergence(v, div.stencil, np.float32(-0.8), g)
+    assert same_bits(got_d, want_d, ghost_free_mask(shape, g, (1,)))
+    lap = Laplacian(1, dx, g)
+    want_l = corc.laplacian(s, lap.stencil, np.float32(0.5), g)
+    sd, vd = to_dev(s, dev), to_dev(v, dev)
+    lo = torch.full_like(sd, float("nan"))
+    go = torch.full_like(vd, float("nan"))
+    do = torch.full_like(sd, float("nan"))
+    F.fused_lap_grad_div(sd, vd, lo, go, do, dx, g, 0.5, 1.3, -0.8)
+    assert same_bits(lo.cpu().numpy(), want_l, ghost_free_mask(shape, g, (1,)))
+    assert same_bits(go.cpu().numpy(), want_g, ghost_free_mask(shape, g, (3,)))
+    assert same_bits(do.cpu().numpy(), want_d, ghost_free_mask(shape, g, (1,)))
+    m = ghost_free_mask(shape, g, (1,))
+    assert np.all(np.isnan(lo.cpu().numpy()[~m]))
+
+
 PERIODIC_SHAPES = [((10, 11, 12), 1, 1), ((34, 9, 517), 1, 1), ((7, 13, 1030), 1, 1),
                    ((12, 10, 9), 2, 1), ((10, 6, 4), 1, 1), ((5, 3, 4), 1, 1), ((10, 12, 1), 1, 1),
                    ((9, 10, 1), 2, 3), ((1, 6, 5), 1, 2), ((8, 1, 1), 1, 1), ((11, 9, 8), 1, 3)]
