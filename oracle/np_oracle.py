@@ -118,6 +118,30 @@ def forward_euler(cur, k, dt_step):
     return cur + cur.dtype.type(dt_step) * k
 
 
+def rk1_step(f, bc, t, dt, *fields):
+    """Forward Euler over several fields as the reference's ``RungeKatta1``
+    documents it (``pm/time_step/rungeKatta.py:21-25,63-83``): bc, then
+    y + dt * f(t, y).  (The reference's code steps by dt / 2, ``:80``, and
+    never runs: its setup raises, ``:59``.)"""
+    ys = bc(t, *fields)
+    ks = f(t, *ys)
+    return tuple(forward_euler(y, k, dt) for y, k in zip(ys, ks))
+
+
+def rk2_step(f, bc, t, dt, *fields):
+    """Midpoint step of the reference's ``RungeKatta2.forward``
+    (``pm/time_step/rungeKatta.py:145-174``): y0 = bc(t, y) (kept as a copy),
+    k1 = f(t, y0), y1 = bc(t, y0 + dt/2 k1), k2 = f(t + dt/2, y1),
+    y0 + dt k2.  dt / 2 and t + dt / 2 are Python (FP64) arithmetic, rounded
+    to FP32 where the Euler update uses them."""
+    y0 = tuple(np.array(y, copy=True) for y in bc(t, *fields))
+    k1 = f(t, *y0)
+    y1 = tuple(forward_euler(y, k, dt / 2) for y, k in zip(y0, k1))
+    y1 = bc(t, *y1)
+    k2 = f(t + dt / 2, *y1)
+    return tuple(forward_euler(y, k, dt) for y, k in zip(y0, k2))
+
+
 # --------------------------------------------------------------------------- boundary groups
 def no_ghost_shape(shape3, g):
     """Reference ``ExplicitUniformGridStencil.py:134-148``."""

@@ -19,3 +19,9 @@ def ijk_to_global_c(i, j, k, Nx, Ny, Nz):
 def global_to_ijk_c(global_id, Nx, Ny, Nz):
     """(i, j, k) of a flat C-order node index."""
     return global_id // (Ny * Nz), (global_id // Nz) % Ny, global_id % Nz
+
+
+class SignatureMismatchError(Exception):
+    """Raised when two callables that must share a signature do not
+    (reference ``pm/utils`` ``SignatureMismatchError``, raised by the
+    Runge-Kutta integrators)."""
