@@ -449,6 +449,35 @@ int pde_lincomb_f32(const float *x0, const float *x1, const float *x2, const flo
 int pde_affine_f32(const float *x, float *out, int64_t nodes, int32_t ncomp, int32_t comp,
                    float scale, float shift, void *stream);
 
+/* ------------------------------------------------------------ absorbing layers
+ *
+ * Sponge (far-field relaxation) and viscous damping layers over the nodes
+ * within num_layers of an eligible face (reference
+ * pm/FDM/kernel/farfield.py:8-78 and viscousDampingLayer.py:8-76, launched by
+ * pm/FDM/functional/_farfield.py:5-41 and _viscousDampingLayer.py:5-38).  The
+ * host resolves each layer node's distance level d (1..num_layers, measured
+ * along the axis of the nearest face, lowest axis on ties) and the profile
+ * prof[d-1] = fp32((fp32(d) / fp32(num_layers)) ** p), the power taken in FP64
+ * and rounded once. */
+typedef struct pde_layer_point {
+    int64_t node;     /* flat padded node index */
+    int32_t level;    /* 1..nlevels */
+    int32_t reserved;
+} pde_layer_point;
+
+#define PDE_LAYER_MAXC 16   /* most floats per element (vec16 / mat4x4) */
+
+/* For every point: f = scale * prof[level-1];
+ *   out[node][c] = -f * (in[node][c] - farfield[c])   (farfield != NULL, FarField)
+ *   out[node][c] = -f * in[node][c]                   (farfield == NULL,
+ *                                                      ViscousDampingLayer)
+ * farfield is a HOST array of ncomp floats, passed by value to the kernel (so
+ * a captured graph keeps it).  Nodes not in the list are not written.  Every
+ * level must lie in 1..nlevels. */
+int pde_damping_layer_f32(const float *in, float *out, const pde_layer_point *pts, int64_t npts,
+                          const float *profile, int32_t nlevels, int32_t ncomp,
+                          const float *farfield, float scale, void *stream);
+
 /* ------------------------------------------------------------------- library */
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -142,6 +142,37 @@ def rk2_step(f, bc, t, dt, *fields):
     return tuple(forward_euler(y, k, dt) for y, k in zip(y0, k2))
 
 
+def damping_layer(u, grid_shape, num_layers, p, points, scale, farfield=None, out=None):
+    """Sponge / viscous damping layer, node by node as the reference kernels do
+    (``pm/FDM/kernel/farfield.py:36-77``, ``viscousDampingLayer.py:36-74``):
+    axis = argmin over axes of min(i, s - 1 - i) (size-1 axes excluded, first
+    axis on ties); dx from the near face's layer limit; factor =
+    scale * (dx / num_layers) ** p with the quotient in the field's precision
+    and the power in FP64 rounded once; out = -factor * (u - farfield), or
+    -factor * u without a far field.  Other nodes of ``out`` keep their values
+    (zeros for a fresh output, as the modules zero theirs at setup)."""
+    dt = u.dtype.type
+    out = np.zeros_like(u) if out is None else out
+    gs = [int(s) for s in grid_shape]
+    limits = [(num_layers, (s - 1) - num_layers) if s > 1 else (0, 0) for s in gs]
+    for pt in np.asarray(points).reshape(-1, 3):
+        dists = [min(int(pt[i]), gs[i] - int(pt[i]) - 1) if gs[i] > 1 else np.iinfo(np.int32).max
+                 for i in range(3)]
+        axis = int(np.argmin(dists))
+        x = int(pt[axis])
+        if (gs[axis] - 1 - x) < x:
+            dx = x - limits[axis][1]
+        else:
+            dx = limits[axis][0] - x
+        base = dt(dx) / dt(num_layers)
+        prof = dt(np.float64(base) ** np.float64(np.float32(p)))
+        f = dt(scale) * prof
+        idx = tuple(int(i) for i in pt)
+        v = u[idx] if farfield is None else u[idx] - np.asarray(farfield, dtype=u.dtype).reshape(u[idx].shape)
+        out[idx] = (-f) * v
+    return out
+
+
 # --------------------------------------------------------------------------- boundary groups
 def no_ghost_shape(shape3, g):
     """Reference ``ExplicitUniformGridStencil.py:134-148``."""

@@ -1,8 +1,8 @@
 """Uniform-grid finite differences (reference ``pm/FDM/__init__.py:1-13``)."""
-from .module import (Divergence, ExplicitUniformGridStencil, Grad, Laplacian, OuterProduct,
-                     PeriodicLaplacian, scalarVectorMult)
+from .module import (Divergence, ExplicitUniformGridStencil, FarField, Grad, Laplacian,
+                     OuterProduct, PeriodicLaplacian, ViscousDampingLayer, scalarVectorMult)
 from .boundary import GridBoundary, PeriodicBoundary
 from . import functional
 
 __all__ = ["Laplacian", "PeriodicLaplacian", "Grad", "Divergence", "GridBoundary", "PeriodicBoundary",
-           "ExplicitUniformGridStencil", "OuterProduct", "scalarVectorMult", "functional"]
+           "ExplicitUniformGridStencil", "FarField", "ViscousDampingLayer", "OuterProduct", "scalarVectorMult", "functional"]

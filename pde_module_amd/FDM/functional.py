@@ -123,3 +123,27 @@ def linear_combination(output_array, *terms):
     """``output = c0*x0 + c1*x1 + ...`` (up to four ``(c, x)`` terms) in one pass."""
     from ..stencil.element_ops import linear_combination as _lc
     return _lc(output_array, *terms)
+
+
+def farfield(kernel, input_array, warp_outer_points, farfield_condition, sigma_max,
+             output_array, device=None):
+    """Sponge layer (reference ``pm/FDM/functional/_farfield.py:5-41``):
+    ``-sigma_max * prof * (input - farfield_condition)`` on the layer nodes of
+    ``warp_outer_points`` (a device table from ``FDM.layers.point_table``)."""
+    from .layers import launch_layer
+    if kernel.op != "sponge":
+        raise TypeError(f"expected a sponge layer kernel, got {kernel.op}")
+    if farfield_condition is None:
+        raise ValueError("farfield_condition is required")
+    return launch_layer(kernel, input_array, warp_outer_points, farfield_condition, sigma_max,
+                        output_array)
+
+
+def viscousDampingLayer(kernel, du_dt, warp_outer_points, C_max, output_array, device=None):
+    """Viscous damping layer (reference
+    ``pm/FDM/functional/_viscousDampingLayer.py:5-38``): ``-C_max * prof * du_dt``
+    on the layer nodes."""
+    from .layers import launch_layer
+    if kernel.op != "damping":
+        raise TypeError(f"expected a damping layer kernel, got {kernel.op}")
+    return launch_layer(kernel, du_dt, warp_outer_points, None, C_max, output_array)

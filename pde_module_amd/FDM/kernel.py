@@ -75,3 +75,15 @@ def create_Divergence_kernel(input_shape, output_shape, grid_shape, stencil, gho
         raise AssertionError("vector input length must equal the grid dimension")
     return StencilKernel("divergence", _check_stencil(stencil), tuple(input_shape),
                          tuple(output_shape), tuple(grid_shape), ghost_cells)
+
+
+def create_spongeLayer_kernel(input_dtype, num_layers, beta, grid_shape, ghost_cells):
+    """Reference ``pm/FDM/kernel/farfield.py:8-78``."""
+    from .layers import LayerKernel
+    return LayerKernel("sponge", input_dtype, num_layers, beta, grid_shape, ghost_cells)
+
+
+def create_dampingLayer_kernel(input_dtype, num_layers, p, grid_shape, ghost_cells):
+    """Reference ``pm/FDM/kernel/viscousDampingLayer.py:8-76``."""
+    from .layers import LayerKernel
+    return LayerKernel("damping", input_dtype, num_layers, p, grid_shape, ghost_cells)

@@ -67,6 +67,7 @@ SIGNATURES = {
     "pde_wave_step_adjoint_fused_f32": [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F,
                                         _F, _F, _F, _F, _P, _P, _P, _P],
     "pde_scatter_add_f32": [_P, _P, _P, _I64, _F, _P],
+    "pde_damping_layer_f32": [_P, _P, _P, _I64, _P, _I32, _I32, _P, _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],
 }
@@ -89,6 +90,12 @@ POINT_SET_DTYPE = np.dtype([("row_start", "<u8"), ("col", "<u8"), ("slot", "<u8"
                             ("data", "<u8"), ("stride", "<i8"), ("scale", "<f4"),
                             ("reserved", "<i4")])
 assert POINT_SET_DTYPE.itemsize == 48
+
+
+# struct pde_layer_point
+LAYER_POINT_DTYPE = np.dtype([("node", "<i8"), ("level", "<i4"), ("reserved", "<i4")])
+assert LAYER_POINT_DTYPE.itemsize == 16
+LAYER_MAXC = 16
 
 
 class PDEError(RuntimeError):
