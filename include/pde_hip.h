@@ -449,6 +449,36 @@ int pde_lincomb_f32(const float *x0, const float *x1, const float *x2, const flo
 int pde_affine_f32(const float *x, float *out, int64_t nodes, int32_t ncomp, int32_t comp,
                    float scale, float shift, void *stream);
 
+/* --------------------------------------------------------- immersed boundary
+ *
+ * Staircase immersed boundary (reference pm/FDM/boundary/immersedBoundary.py:
+ * create_staircase_boundary_kernel :236-300 and create_fill_solids_inplace
+ * :214-233, launched by ImmersedBoundary.forward :177-211).  One record per
+ * (solid boundary node, component), resolved on the host from the bitmask and
+ * the (node, component) type/value tables:
+ *   kind 1 (Dirichlet): out[node][comp] = value
+ *   kind 2 (Neumann):   running mean, over the fluid neighbours in axis order
+ *                       (eligible axes ascending, side -1 then +1), of
+ *                       in[node + s*stride[d]][comp] - (s * value) * dx:
+ *                       avg += (c - avg) / n; n += 1, starting from avg = 0,
+ *                       n = 1.  Bit 2d + (s > 0) of `mask` marks a fluid
+ *                       neighbour.
+ * Then every interior solid node (all components) takes fill_value.  Reads
+ * touch fluid nodes only and writes solid nodes only, so in == out (in place)
+ * is allowed. */
+typedef struct pde_ib_op {
+    int64_t node;     /* flat padded node index */
+    float value;
+    int32_t comp;
+    int32_t kind;     /* 1 Dirichlet, 2 Neumann, anything else: no write */
+    int32_t mask;     /* fluid neighbours, bit 2*d + (side > 0) */
+} pde_ib_op;
+
+int pde_immersed_apply_f32(const float *in, float *out, const pde_ib_op *ops, int64_t nops,
+                           const int64_t *fill_nodes, int64_t nfill, float fill_value,
+                           int32_t ncomp, int32_t ndim, int64_t stride0, int64_t stride1,
+                           int64_t stride2, float dx, void *stream);
+
 /* ------------------------------------------------------------ absorbing layers
  *
  * Sponge (far-field relaxation) and viscous damping layers over the nodes

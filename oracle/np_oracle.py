@@ -173,6 +173,52 @@ def damping_layer(u, grid_shape, num_layers, p, points, scale, farfield=None, ou
     return out
 
 
+def immersed_boundary(u, bitmask, btype, bvalue, dx, fill_value=0.0):
+    """Staircase immersed boundary, node by node (reference
+    ``pm/FDM/boundary/immersedBoundary.py:177-300`` with the host steps
+    ``:54-152``): solid nodes = nonzero bitmask in C order; a solid node is a
+    boundary node when a neighbour +-1 along an eligible axis is fluid (0);
+    btype / bvalue are the (boundary node, component) tables in that order.
+    Dirichlet: out = v.  Neumann: running mean over the fluid neighbours
+    (axes ascending, side -1 then +1) of u[nb] - (s * v) * dx.  Interior solid
+    nodes take fill_value.  Everything else is u.  Neighbours are flat index
+    +- the axis stride (the reference indexes without a bounds check, so on the
+    outermost layer a row index -1 is the previous row's last node)."""
+    dt = u.dtype.type
+    out = np.array(u, copy=True)
+    shape3 = bitmask.shape
+    elig = [a for a in range(3) if shape3[a] > 1]
+    strides = (shape3[1] * shape3[2], shape3[2], 1)
+    bflat = bitmask.reshape(-1)
+    C = u.shape[3]
+    uflat = u.reshape(-1, C)
+    oflat = out.reshape(-1, C)
+    # neighbours by flat index +- axis stride, the reference's unchecked indexing
+    solid = [int(f) for f in np.flatnonzero(bflat)]
+    bnd, inner = [], []
+    for f in solid:
+        is_b = any(bflat[f + s * strides[a]] == 0 for a in elig for s in (-1, 1))
+        (bnd if is_b else inner).append(f)
+    for b, f in enumerate(bnd):
+        for c in range(C):
+            v = dt(bvalue[b, c])
+            if btype[b, c] == 1:
+                oflat[f, c] = v
+            elif btype[b, c] == 2:
+                avg, n = dt(0.0), dt(1.0)
+                for a in elig:
+                    for s in (-1, 1):
+                        q = f + s * strides[a]
+                        if bflat[q] == 0:
+                            cval = uflat[q, c] - (dt(s) * v) * dt(dx)
+                            avg = avg + (cval - avg) / n
+                            n = n + dt(1.0)
+                oflat[f, c] = avg
+    for f in inner:
+        oflat[f] = dt(fill_value)
+    return out
+
+
 # --------------------------------------------------------------------------- boundary groups
 def no_ghost_shape(shape3, g):
     """Reference ``ExplicitUniformGridStencil.py:134-148``."""

@@ -68,6 +68,8 @@ SIGNATURES = {
                                         _F, _F, _F, _F, _P, _P, _P, _P],
     "pde_scatter_add_f32": [_P, _P, _P, _I64, _F, _P],
     "pde_damping_layer_f32": [_P, _P, _P, _I64, _P, _I32, _I32, _P, _F, _P],
+    "pde_immersed_apply_f32": [_P, _P, _P, _I64, _P, _I64, _F, _I32, _I32, _I64, _I64, _I64,
+                               _F, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],
 }
@@ -96,6 +98,12 @@ assert POINT_SET_DTYPE.itemsize == 48
 LAYER_POINT_DTYPE = np.dtype([("node", "<i8"), ("level", "<i4"), ("reserved", "<i4")])
 assert LAYER_POINT_DTYPE.itemsize == 16
 LAYER_MAXC = 16
+
+
+# struct pde_ib_op
+IB_OP_DTYPE = np.dtype([("node", "<i8"), ("value", "<f4"), ("comp", "<i4"), ("kind", "<i4"),
+                        ("mask", "<i4")])
+assert IB_OP_DTYPE.itemsize == 24
 
 
 class PDEError(RuntimeError):
