@@ -449,6 +449,34 @@ int pde_lincomb_f32(const float *x0, const float *x1, const float *x2, const flo
 int pde_affine_f32(const float *x, float *out, int64_t nodes, int32_t ncomp, int32_t comp,
                    float scale, float shift, void *stream);
 
+/* ------------------------------------------------- projection-method cavity
+ *
+ * Kernels of the incompressible lid-driven cavity by pressure projection (an
+ * explicit predictor from the operators above, a pressure Poisson equation
+ * solved by a fixed number of Jacobi sweeps, then the velocity correction).
+ * None has a reference counterpart: the reference's cavity examples use the
+ * artificial-compressibility method (examples/LDC/LDC_No_Graph.py) instead. */
+
+/* One Jacobi sweep on the interior nodes of a scalar field:
+ *   acc = 0; over eligible axes ascending: acc += p[n-e]; acc += p[n+e];
+ *   out[n] = inv * (acc - h2 * rhs[n])      (inv = 1/(2 ndim), h2 = dx^2).
+ * Ghosts of p are read (apply the pressure boundary first), those of out are
+ * not written.  out must not alias p or rhs. */
+int pde_jacobi_f32(const float *p, const float *rhs, float *out, int64_t nx, int64_t ny,
+                   int64_t nz, int32_t ghost, float h2, float inv, void *stream);
+
+/* Transpose of pde_jacobi_f32: gp[m] = inv * (sum over axes ascending of
+ * gout[m+e] then gout[m-e], each taken when that node is interior) on every
+ * padded node; grhs[n] = -h2 * (inv * gout[n]) on interior nodes, 0 on ghosts
+ * (grhs may be NULL). */
+int pde_jacobi_adjoint_f32(const float *gout, float *gp, float *grhs, int64_t nx, int64_t ny,
+                           int64_t nz, int32_t ghost, float h2, float inv, void *stream);
+
+/* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
+ * pde_matvec_f32 in its vector argument (the convection term's gradient). */
+int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,
+                     int32_t cols, void *stream);
+
 /* --------------------------------------------------------- immersed boundary
  *
  * Staircase immersed boundary (reference pm/FDM/boundary/immersedBoundary.py:

@@ -219,6 +219,42 @@ def immersed_boundary(u, bitmask, btype, bvalue, dx, fill_value=0.0):
     return out
 
 
+def jacobi(p, rhs, g, h2, inv):
+    """One Jacobi sweep on the interior of a scalar field: acc over eligible
+    axes ascending of p[n - e] + p[n + e] (accumulated from 0, left to right),
+    then inv * (acc - h2 * rhs) (restates pde_jacobi_f32; the pressure step of
+    the projection cavity, no reference counterpart)."""
+    dt = p.dtype.type
+    shape3 = p.shape[:3]
+    inner = interior(shape3, g)
+    acc = np.zeros_like(p[inner])
+    for a in eligible_axes(shape3):
+        acc = acc + p[interior(shape3, g, a, -1)]
+        acc = acc + p[interior(shape3, g, a, +1)]
+    out = np.full_like(p, np.nan)
+    out[inner] = dt(inv) * (acc - dt(h2) * rhs[inner])
+    return out
+
+
+def projection_step(u, p, lw, cw, nu, dt_step, h2, inv, sweeps, g, ubc, pbc):
+    """One step of the projection cavity (the composition
+    pde_module_amd.flow.ProjectionCavity.step runs): ubc / pbc map a field to
+    its boundary-applied copy."""
+    dt = u.dtype.type
+    u2 = ubc(u)
+    du = grad(u2, cw, dt(1.0), g)
+    conv = matvec(du, u2)
+    diff = laplacian(u2, lw, dt(nu), g)
+    F = linear_combination((-1, conv), (1, diff))
+    us = forward_euler(u2, F, dt_step)
+    us2 = ubc(us)
+    rhs = divergence(us2, cw, dt(1.0 / dt_step), g)
+    for _ in range(sweeps):
+        p = jacobi(pbc(p), rhs, g, h2, inv)
+    gp = grad(pbc(p), cw, dt(1.0), g)
+    return forward_euler(us2, gp, -dt_step), p
+
+
 # --------------------------------------------------------------------------- boundary groups
 def no_ghost_shape(shape3, g):
     """Reference ``ExplicitUniformGridStencil.py:134-148``."""

@@ -1,0 +1,153 @@
+// Kernels of the projection-method cavity: the Jacobi sweep of the pressure
+// Poisson equation and its transpose, and the transposed node-wise
+// matrix-vector product that the convection term's gradient needs.
+//
+// Jacobi sweep on the interior nodes of a scalar field (1..3 eligible axes):
+//   acc = 0; over eligible axes ascending: acc += p[n - e]; acc += p[n + e];
+//   out[n] = inv * (acc - h2 * rhs[n]),   inv = 1 / (2 * ndim), h2 = dx^2,
+// one Jacobi iteration of (sum of neighbours - 2 ndim p) / dx^2 = rhs.  Ghosts
+// are read (apply the pressure boundary first) and never written.
+#include "pde_common.hpp"
+
+namespace pde {
+
+struct JacGeom {
+    int64_t S[3];          // node strides of canonical axes 0..2
+    int64_t E[3], G[3];    // interior extents and starts
+    int ndim;
+};
+
+static JacGeom jac_geom(const Geom &G)
+{
+    JacGeom j;
+    for (int a = 0; a < 3; ++a) {
+        j.S[a] = G.S[a];
+        j.E[a] = G.E[a];
+        j.G[a] = G.G[a];
+    }
+    j.ndim = G.ndim;
+    return j;
+}
+
+__device__ __forceinline__ bool interior(const JacGeom &g, int64_t i0, int64_t i1, int64_t k)
+{
+    return i0 >= g.G[0] && i0 < g.G[0] + g.E[0] && i1 >= g.G[1] && i1 < g.G[1] + g.E[1] &&
+           k >= g.G[2] && k < g.G[2] + g.E[2];
+}
+
+// thread per interior node: x over the contiguous axis, y/z loops over the others
+__global__ __launch_bounds__(256) void jacobi_kernel(const float *__restrict__ p,
+                                                     const float *__restrict__ rhs,
+                                                     float *__restrict__ out, JacGeom g, float h2,
+                                                     float inv)
+{
+    const int64_t k = g.G[2] + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= g.G[2] + g.E[2]) return;
+    for (int64_t i0 = g.G[0] + blockIdx.z; i0 < g.G[0] + g.E[0]; i0 += gridDim.z)
+        for (int64_t i1 = g.G[1] + blockIdx.y; i1 < g.G[1] + g.E[1]; i1 += gridDim.y) {
+            const int64_t e = i0 * g.S[0] + i1 * g.S[1] + k;
+            float acc = 0.0f;
+            for (int a = 3 - g.ndim; a < 3; ++a) {
+                acc += p[e - g.S[a]];
+                acc += p[e + g.S[a]];
+            }
+            out[e] = inv * (acc - h2 * rhs[e]);
+        }
+}
+
+// Transpose: for every padded node m, acc over axes ascending of g[m + e]
+// (m is the -1 neighbour of m + e) then g[m - e], each when that node is
+// interior; gp[m] = inv * acc.  grhs[n] = -h2 * (inv * g[n]) on interior n,
+// 0 on ghosts.
+__global__ __launch_bounds__(256) void jacobi_adjoint_kernel(const float *__restrict__ gout,
+                                                             float *__restrict__ gp,
+                                                             float *__restrict__ grhs, JacGeom g,
+                                                             int64_t P0, int64_t P1, int64_t P2,
+                                                             float h2, float inv)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    for (int64_t i0 = blockIdx.z; i0 < P0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < P1; i1 += gridDim.y) {
+            const int64_t e = i0 * g.S[0] + i1 * g.S[1] + k;
+            const int64_t c[3] = {i0, i1, k};
+            float acc = 0.0f;
+            for (int a = 3 - g.ndim; a < 3; ++a) {
+                int64_t q[3] = {c[0], c[1], c[2]};
+                q[a] = c[a] + 1;
+                if (interior(g, q[0], q[1], q[2])) acc += gout[e + g.S[a]];
+                q[a] = c[a] - 1;
+                if (interior(g, q[0], q[1], q[2])) acc += gout[e - g.S[a]];
+            }
+            gp[e] = inv * acc;
+            if (grhs) grhs[e] = interior(g, i0, i1, k) ? -h2 * (inv * gout[e]) : 0.0f;
+        }
+}
+
+// out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
+__global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
+                                                       const float *__restrict__ gv,
+                                                       float *__restrict__ out, int64_t total,
+                                                       int rows, int cols)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const int64_t n = t / cols;
+    const int j = (int)(t - n * cols);
+    const float *mn = m + n * rows * cols + j;
+    const float *gn = gv + n * rows;
+    float acc = mn[0] * gn[0];
+    for (int i = 1; i < rows; ++i) acc = acc + mn[i * cols] * gn[i];
+    out[t] = acc;
+}
+
+}  // namespace pde
+
+using namespace pde;
+
+extern "C" int pde_jacobi_f32(const float *p, const float *rhs, float *out, int64_t nx,
+                              int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
+                              void *stream)
+{
+    if (!p || !rhs || !out || out == p || out == rhs) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    const JacGeom g = jac_geom(G);
+    const dim3 grid(clamp_grid((G.E[2] + 255) / 256, 0x7fffffff), clamp_grid(G.E[1], 65535),
+                    clamp_grid(G.E[0], 65535));
+    hipLaunchKernelGGL(jacobi_kernel, grid, dim3(256), 0, as_stream(stream), p, rhs, out, g, h2,
+                       inv);
+    return launch_status();
+}
+
+extern "C" int pde_jacobi_adjoint_f32(const float *gout, float *gp, float *grhs, int64_t nx,
+                                      int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
+                                      void *stream)
+{
+    if (!gout || !gp || gp == gout || grhs == gout || grhs == gp) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    const JacGeom g = jac_geom(G);
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535),
+                    clamp_grid(G.P[0], 65535));
+    hipLaunchKernelGGL(jacobi_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout, gp, grhs,
+                       g, G.P[0], G.P[1], G.P[2], h2, inv);
+    return launch_status();
+}
+
+extern "C" int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes,
+                                int32_t rows, int32_t cols, void *stream)
+{
+    if (!m || !g || !out || nodes < 0 || rows < 1 || cols < 1) return PDE_EINVAL;
+    if (out == m || out == g) return PDE_EINVAL;
+    const int64_t total = nodes * cols;
+    if (total == 0) return PDE_OK;
+    if ((total + 255) / 256 > 0x7fffffff) return PDE_ESHAPE;
+    hipLaunchKernelGGL(matvec_t_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), m, g, out, total, rows, cols);
+    return launch_status();
+}

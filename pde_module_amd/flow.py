@@ -1,0 +1,258 @@
+"""Incompressible lid-driven cavity by pressure projection (2D, collocated).
+
+The reference's cavity examples use the artificial-compressibility method
+(``examples/LDC/LDC_No_Graph.py``, ``LDC_RK2.py``); this is the projection
+variant of the same flow, built from the same operators:
+
+    u2  = B_u(u)                                  no slip, lid u = U on +Y
+    u*  = u2 + dt * (-(grad u2) u2 + nu * lap(u2))
+    u*  = B_u(u*)
+    rhs = div(u*) / dt
+    p   <- J(B_p(p), rhs), sweeps times           Jacobi sweeps, Neumann-0 B_p
+    u   = u* - dt * grad(B_p(p))
+
+with J one Jacobi sweep of lap(p) = rhs (``pde_jacobi_f32``).  ``step`` runs on
+preallocated buffers through the HIP kernels (capturable in a HIP graph);
+``kinetic_energy_rollout`` is the same loop as differentiable functions whose
+backward passes are the transposed kernels, checkpointed every
+``checkpoint_every`` steps, giving dJ/du0 for J = 1/2 sum |u|^2 dx^2 over the
+interior at the final step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+import torch.utils.checkpoint as cp
+
+from . import _native as N
+from . import autograd as A
+from .deprecated import GridBoundary as LegacyGridBoundary
+from .FDM.module.grad import central_weights
+from .FDM.module.laplacian import laplacian_weights
+
+
+def _grid(x):
+    return tuple(int(s) for s in x.shape[:3])
+
+
+class _MatVec(torch.autograd.Function):
+    """out[n] = M[n] v[n]; backward gM = outer(g, v), gv = M^T g (HIP kernels)."""
+
+    @staticmethod
+    def forward(ctx, m, v):
+        out = torch.empty(tuple(m.shape[:4]), dtype=m.dtype, device=m.device)
+        nodes = int(np.prod(m.shape[:3]))
+        N.call("pde_matvec_f32", m.data_ptr(), v.data_ptr(), out.data_ptr(), nodes,
+               int(m.shape[3]), int(m.shape[4]), N.stream_of(m))
+        ctx.save_for_backward(m, v)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        m, v = ctx.saved_tensors
+        g = g.contiguous()
+        nodes = int(np.prod(m.shape[:3]))
+        rows, cols = int(m.shape[3]), int(m.shape[4])
+        gm = torch.empty_like(m)
+        N.call("pde_outer_f32", g.data_ptr(), v.data_ptr(), gm.data_ptr(), nodes, rows, cols,
+               N.stream_of(g))
+        gv = torch.empty_like(v)
+        N.call("pde_matvec_t_f32", m.data_ptr(), g.data_ptr(), gv.data_ptr(), nodes, rows, cols,
+               N.stream_of(g))
+        return gm, gv
+
+
+class _LinComb(torch.autograd.Function):
+    """c0 x0 + c1 x1 (+ ...) left to right (pde_lincomb_f32); backward scales."""
+
+    @staticmethod
+    def forward(ctx, coeffs, *xs):
+        out = torch.empty_like(xs[0])
+        ptrs = [x.data_ptr() for x in xs] + [None] * (4 - len(xs))
+        cs = [np.float32(c) for c in coeffs] + [np.float32(0)] * (4 - len(coeffs))
+        N.call("pde_lincomb_f32", *ptrs, *cs, len(xs), out.data_ptr(), out.numel(),
+               N.stream_of(out))
+        ctx.coeffs = coeffs
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.contiguous()
+        grads = []
+        for c in ctx.coeffs:
+            gx = torch.empty_like(g)
+            N.call("pde_scale_f32", g.data_ptr(), gx.data_ptr(), g.numel(), np.float32(c),
+                   N.stream_of(g))
+            grads.append(gx)
+        return (None, *grads)
+
+
+class _Jacobi(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, p, rhs, g, h2, inv):
+        out = torch.zeros_like(p)
+        N.call("pde_jacobi_f32", p.data_ptr(), rhs.data_ptr(), out.data_ptr(), *_grid(p), g,
+               np.float32(h2), np.float32(inv), N.stream_of(p))
+        ctx.meta = (g, h2, inv)
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        g, h2, inv = ctx.meta
+        gout = gout.contiguous()
+        gp = torch.empty_like(gout)
+        grhs = torch.empty_like(gout)
+        N.call("pde_jacobi_adjoint_f32", gout.data_ptr(), gp.data_ptr(), grhs.data_ptr(),
+               *_grid(gout), g, np.float32(h2), np.float32(inv), N.stream_of(gout))
+        return gp, grhs, None, None, None
+
+
+class _KineticEnergy(torch.autograd.Function):
+    """1/2 sum over interior nodes of |u|^2 dx^2 (torch reduction of the
+    interior squares; the gradient is dx^2 u on the interior, 0 on ghosts)."""
+
+    @staticmethod
+    def forward(ctx, u, mask, area):
+        ctx.save_for_backward(u, mask)
+        ctx.area = area
+        return 0.5 * area * (u * u * mask).sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        u, mask = ctx.saved_tensors
+        return g * ctx.area * u * mask, None, None
+
+
+class ProjectionCavity:
+    """``ProjectionCavity(n, nu=0.01, U=1.0, dt=None, jacobi_sweeps=50,
+    ghost_cells=1, device='cuda')``: n x n nodes on the unit square, dx =
+    1/(n-1), dt = min(0.5 dx / U, 0.25 dx^2 / nu) by default."""
+
+    def __init__(self, n, nu=0.01, U=1.0, dt=None, jacobi_sweeps=50, ghost_cells=1,
+                 device="cuda"):
+        self.n, self.g = int(n), int(ghost_cells)
+        self.dx = 1.0 / (self.n - 1)
+        self.nu, self.U = float(nu), float(U)
+        self.dt = float(dt) if dt is not None else min(0.5 * self.dx / self.U,
+                                                       0.25 * self.dx ** 2 / self.nu)
+        self.sweeps = int(jacobi_sweeps)
+        self.device = torch.device(device)
+        P = self.n + 2 * self.g
+        self.shape = (P, P, 1)
+        z = lambda *e: torch.zeros(self.shape + e, dtype=torch.float32, device=self.device)
+        self.u_bc = LegacyGridBoundary(z(2), self.dx, self.g)
+        self.u_bc.dirichlet_BC("ALL", 0.0)
+        self.u_bc.dirichlet_BC("+Y", float(self.U), 0)
+        self.p_bc = LegacyGridBoundary(z(1), self.dx, self.g)
+        self.p_bc.vonNeumann_BC("ALL", 0.0)
+        self.wl = laplacian_weights(self.dx)
+        self.wc = central_weights(self.dx)
+        self.h2 = np.float32(self.dx * self.dx)
+        self.inv = np.float32(0.25)
+        # work buffers (ghosts of operator outputs stay 0: no kernel writes them)
+        self.b = {k: z(*e) for k, e in (("u2", (2,)), ("du", (2, 2)), ("conv", (2,)),
+                                         ("diff", (2,)), ("F", (2,)), ("us", (2,)),
+                                         ("us2", (2,)), ("rhs", (1,)), ("pf", (1,)),
+                                         ("gp", (2,)))}
+        m = np.zeros(self.shape + (1,), np.float32)
+        m[self.g:P - self.g, self.g:P - self.g] = 1.0
+        self.mask = torch.from_numpy(m).to(self.device)
+
+    # --------------------------------------------------------------- forward
+    def step(self, u, p, u_out, p_out):
+        """One projection step from (u, p) into (u_out, p_out); u_out / p_out may
+        be u / p.  All work on the current stream, no allocation."""
+        b, g, s = self.b, self.g, N.stream_of(u)
+        grid = self.shape
+        self.u_bc._ensure_table(u)
+        self.p_bc._ensure_table(p)
+        self.u_bc.table.apply(u, b["u2"], None)
+        N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
+               np.float32(1.0), s)
+        N.call("pde_matvec_f32", b["du"].data_ptr(), b["u2"].data_ptr(), b["conv"].data_ptr(),
+               int(np.prod(grid)), 2, 2, s)
+        N.call("pde_laplacian_f32", b["u2"].data_ptr(), b["diff"].data_ptr(), *grid, 2, g,
+               *self.wl, np.float32(self.nu), s)
+        N.call("pde_lincomb_f32", b["conv"].data_ptr(), b["diff"].data_ptr(), None, None,
+               np.float32(-1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
+               b["F"].data_ptr(), b["F"].numel(), s)
+        N.call("pde_forward_euler_f32", b["u2"].data_ptr(), b["F"].data_ptr(), b["us"].data_ptr(),
+               b["us"].numel(), np.float32(self.dt), s)
+        self.u_bc.table.apply(b["us"], b["us2"], None)
+        N.call("pde_divergence_f32", b["us2"].data_ptr(), b["rhs"].data_ptr(), *grid, 1, g,
+               *self.wc, np.float32(1.0 / self.dt), s)
+        # each sweep reads the boundary-applied copy pf, so it can write p_out
+        # even when p_out is p
+        cur = p
+        for _ in range(self.sweeps):
+            self.p_bc.table.apply(cur, b["pf"], None)
+            N.call("pde_jacobi_f32", b["pf"].data_ptr(), b["rhs"].data_ptr(), p_out.data_ptr(),
+                   *grid, g, self.h2, self.inv, s)
+            cur = p_out
+        if cur is not p_out:
+            N.call("pde_copy_f32", cur.data_ptr(), p_out.data_ptr(), p_out.numel(), s)
+        self.p_bc.table.apply(p_out, b["pf"], None)
+        N.call("pde_grad_f32", b["pf"].data_ptr(), b["gp"].data_ptr(), *grid, 1, g, *self.wc,
+               np.float32(1.0), s)
+        N.call("pde_forward_euler_f32", b["us2"].data_ptr(), b["gp"].data_ptr(), u_out.data_ptr(),
+               u_out.numel(), np.float32(-self.dt), s)
+        return u_out, p_out
+
+    def run(self, u0, p0, steps, use_graph=True):
+        """``steps`` steps from (u0, p0); returns the final (u, p)."""
+        u, p = u0.clone(), p0.clone()
+        if steps <= 0:
+            return u, p
+        self.step(u, p, u, p)
+        if steps == 1:
+            return u, p
+        if not use_graph:
+            for _ in range(steps - 1):
+                self.step(u, p, u, p)
+            return u, p
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.step(u, p, u, p)
+        for _ in range(steps - 1):
+            graph.replay()
+        return u, p
+
+    def kinetic_energy(self, u):
+        return float(0.5 * self.dx * self.dx * (u * u * self.mask).sum())
+
+    # ---------------------------------------------------------- differentiable
+    def diff_step(self, u, p):
+        """The step as differentiable functions (fresh outputs, zero ghosts)."""
+        u2 = A.boundary(u, self.u_bc)
+        du = A.grad(u2, self.dx, self.g)
+        conv = _MatVec.apply(du, u2)
+        diff = A.laplacian(u2, self.dx, self.g, self.nu)
+        F = _LinComb.apply((-1.0, 1.0), conv, diff)
+        us = A.forward_euler(u2, F, self.dt)
+        us2 = A.boundary(us, self.u_bc)
+        rhs = A.divergence(us2.unsqueeze(3), self.dx, self.g, 1.0 / self.dt)
+        for _ in range(self.sweeps):
+            p = _Jacobi.apply(A.boundary(p, self.p_bc), rhs, self.g, self.h2, self.inv)
+        gp = A.grad(A.boundary(p, self.p_bc), self.dx, self.g).squeeze(3)
+        return A.forward_euler(us2, gp, -self.dt), p
+
+    def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10):
+        """J(u_N) = 1/2 sum |u_N|^2 dx^2 (interior), differentiable in u0 and
+        p0; segments of ``checkpoint_every`` steps are recomputed in the
+        backward pass."""
+        self.u_bc._ensure_table(u0)
+        self.p_bc._ensure_table(p0)
+
+        def segment(u, p, L):
+            for _ in range(L):
+                u, p = self.diff_step(u, p)
+            return u, p
+
+        u, p = u0, p0
+        done = 0
+        while done < steps:
+            L = min(checkpoint_every, steps - done)
+            u, p = cp.checkpoint(segment, u, p, L, use_reentrant=False)
+            done += L
+        return _KineticEnergy.apply(u, self.mask, float(self.dx * self.dx))

@@ -160,3 +160,20 @@ def heat_ensemble_config(seed=3, count=512, n_min=64, n_max=1024, steps=500, gho
     dx = 1.0 / (n - 1)
     dt = 0.2 * dx * dx / alpha
     return HeatEnsembleConfig(ghost_cells, n, dx, alpha, dt, steps)
+
+
+def ns_cavity_u0(seed, n, ghost_cells=1, amplitude=1e-3, corr=8.0):
+    """Initial velocity of the projection cavity workload: a smoothed Gaussian
+    random streamfunction psi (correlation length ``corr`` cells, seeded), its
+    discrete curl u = (d psi / dy, -d psi / dx) by central differences on the
+    padded grid (so the discrete divergence of the interior is 0 up to
+    rounding), scaled to max |u| = amplitude.  Returns (n+2g, n+2g, 1, 2)."""
+    rng = np.random.default_rng(seed)
+    g = ghost_cells
+    P = n + 2 * g
+    psi = smoothed_field(rng, P, corr)
+    u = np.zeros((P, P, 2))
+    u[1:-1, 1:-1, 0] = (psi[1:-1, 2:] - psi[1:-1, :-2]) / 2.0
+    u[1:-1, 1:-1, 1] = -(psi[2:, 1:-1] - psi[:-2, 1:-1]) / 2.0
+    u *= amplitude / np.max(np.abs(u))
+    return u.astype(np.float32).reshape(P, P, 1, 2)

@@ -1,0 +1,122 @@
+"""GPU checks of the projection cavity (pressure projection with Jacobi
+sweeps): each step bit for bit against the oracle's composition, graph replay
+equal to eager, the Jacobi / transposed mat-vec kernels against the oracle,
+and dJ/du0 of the final kinetic energy (checkpointed rollout through the
+transposed kernels) against central differences of the FP64 oracle loop."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import same_bits
+from helpers import ghost_free_mask
+from oracle import np_oracle as O
+import pde_module_amd._native as N
+from pde_module_amd.flow import ProjectionCavity
+from pde_module_amd.workloads import ns_cavity_u0
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_loop(cav, u0, p0, steps, dtype):
+    lw = O.laplacian_weights(cav.dx, dtype)
+    cw = O.central_weights(cav.dx, dtype)
+    lb = lambda bc, a: O.bc_legacy(a, bc.boundary_ijk_indices, bc.boundary_interior,
+                                   bc.boundary_type, bc.boundary_value, cav.dx)
+    u, p = u0.astype(dtype), p0.astype(dtype)
+    h2 = dtype(cav.dx * cav.dx) if dtype == np.float64 else cav.h2
+    for _ in range(steps):
+        u, p = O.projection_step(u, p, lw, cw, cav.nu, cav.dt, h2, dtype(cav.inv),
+                                 cav.sweeps, cav.g, lambda a: lb(cav.u_bc, a),
+                                 lambda a: lb(cav.p_bc, a))
+    return u, p
+
+
+def test_projection_steps_match_oracle(dev):
+    n = 34
+    cav = ProjectionCavity(n, jacobi_sweeps=50, device=dev)
+    u0 = ns_cavity_u0(0, n)
+    p0 = np.zeros(cav.shape + (1,), np.float32)
+    u, p = torch.from_numpy(u0).to(dev), torch.from_numpy(p0).to(dev)
+    for _ in range(3):
+        cav.step(u, p, u, p)
+    ou, op = _oracle_loop(cav, u0, p0, 3, np.float32)
+    assert same_bits(u.cpu().numpy(), ou, ghost_free_mask(cav.shape, 1, (2,)))
+    assert same_bits(p.cpu().numpy(), op, ghost_free_mask(cav.shape, 1, (1,)))
+
+
+def test_graph_replay_equals_eager(dev):
+    n = 24
+    cav = ProjectionCavity(n, jacobi_sweeps=10, device=dev)
+    u0 = torch.from_numpy(ns_cavity_u0(1, n)).to(dev)
+    p0 = torch.zeros(cav.shape + (1,), device=dev)
+    ue, pe = cav.run(u0, p0, 12, use_graph=False)
+    ug, pg = cav.run(u0, p0, 12, use_graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(ue.view(torch.int32), ug.view(torch.int32))
+    assert torch.equal(pe.view(torch.int32), pg.view(torch.int32))
+
+
+@pytest.mark.parametrize("shape", [(11, 9, 1), (7, 6, 8)])
+def test_jacobi_and_transposes_match_oracle(dev, shape):
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    rhs = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    nd = len(O.eligible_axes(shape))
+    h2, inv = np.float32(0.013), np.float32(1.0 / (2 * nd))
+    T = lambda a: torch.from_numpy(a).to(dev)
+    pd, rd = T(p), T(rhs)              # kept alive across the launches
+    out = torch.zeros_like(pd)
+    N.call("pde_jacobi_f32", pd.data_ptr(), rd.data_ptr(), out.data_ptr(), *shape, 1, h2,
+           inv, N.stream_of(out))
+    m = ghost_free_mask(shape, 1, (1,))
+    assert same_bits(out.cpu().numpy(), O.jacobi(p, rhs, 1, h2, inv), m)
+    # transpose: dot-product test in FP64 of the FP32 results
+    y = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    gp, gr = torch.empty_like(out), torch.empty_like(out)
+    yd = T(y)
+    N.call("pde_jacobi_adjoint_f32", yd.data_ptr(), gp.data_ptr(), gr.data_ptr(), *shape, 1,
+           h2, inv, N.stream_of(out))
+    lhs = float(np.sum(np.where(m, out.cpu().numpy(), 0).astype(np.float64) * y))
+    rhs_ = float(np.sum(gp.cpu().numpy().astype(np.float64) * p) +
+                 np.sum(gr.cpu().numpy().astype(np.float64) * rhs))
+    assert abs(lhs - rhs_) <= 1e-5 * (abs(lhs) + 1)
+    # transposed mat-vec, 2 x 3 matrices
+    M = rng.uniform(-1, 1, shape + (2, 3)).astype(np.float32)
+    g = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
+    gv = torch.empty(shape + (3,), device=dev)
+    Md, gd = T(M), T(g)
+    N.call("pde_matvec_t_f32", Md.data_ptr(), gd.data_ptr(), gv.data_ptr(),
+           int(np.prod(shape)), 2, 3, N.stream_of(gv))
+    want = M[..., 0, :] * g[..., 0:1] + M[..., 1, :] * g[..., 1:2]
+    assert same_bits(gv.cpu().numpy(), want)
+
+
+def test_kinetic_energy_gradient_matches_fp64_finite_differences(dev):
+    n, steps = 18, 6
+    cav = ProjectionCavity(n, jacobi_sweeps=20, dt=2e-3, device=dev)
+    u0 = ns_cavity_u0(2, n) * 50.0                  # O(0.05) so convection matters
+    p0 = np.zeros(cav.shape + (1,), np.float32)
+    grads = []
+    for K in (2, steps):
+        x = torch.from_numpy(u0).to(dev).requires_grad_(True)
+        J = cav.kinetic_energy_rollout(x, torch.from_numpy(p0).to(dev), steps, checkpoint_every=K)
+        (gx,) = torch.autograd.grad(J, x)
+        grads.append((float(J), gx.cpu().numpy().astype(np.float64)))
+    assert np.array_equal(grads[0][1], grads[1][1])       # checkpoint-interval invariant
+    J32, g32 = grads[0]
+    mask = ghost_free_mask(cav.shape, 1, (2,))
+
+    def J64(u):
+        uN, _ = _oracle_loop(cav, u, p0, steps, np.float64)
+        return 0.5 * cav.dx * cav.dx * float(np.sum(np.where(mask, uN, 0.0) ** 2))
+
+    base = J64(u0.astype(np.float64))
+    assert abs(J32 - base) <= 1e-5 * abs(base)
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        d = np.where(mask, rng.standard_normal(u0.shape), 0.0)
+        eps = 1e-3 * np.abs(u0).max()
+        fd = (J64(u0 + eps * d) - J64(u0 - eps * d)) / (2 * eps)
+        ad = float(np.sum(g32 * d))
+        print(f"dJ/du0 . d: adjoint {ad:.8e}  fd {fd:.8e}", flush=True)
+        assert abs(ad - fd) <= 1e-4 * abs(fd) + 1e-12
