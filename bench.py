@@ -114,7 +114,7 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     mk = lambda k: WaveRollout(bc, cfg.dx, cfg.g, float(cfg.dt), steps, checkpoint_every=k,
                                src_idx=T(cfg.src_idx), src_wavelet=T(cfg.wavelet),
                                rec_idx=T(cfg.rec_idx), use_graph=True)
-    plain, ro = mk(steps), mk(K)
+    plain, ro, ro_all = mk(steps), mk(K), mk(None)
     _, _, d = plain.forward_states(torch.ones_like(c2), u0, v0)
     d = d.clone()
 
@@ -140,11 +140,22 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
         out["J"], out["g"] = float(J), float(c.grad.abs().max())
 
     t_fa = timed(fwd_adj)
+
+    def fwd_adj_all():
+        c = c2.clone().requires_grad_(True)
+        _, _, r = ro_all(c, u0, v0)
+        ((r - d) ** 2).sum().backward()
+
+    t_all = timed(fwd_adj_all, reps=2)
     return {"grid": [cfg.n, cfg.n], "steps": steps, "checkpoint_every": K,
             "fwd_ms_per_step": round(t_fwd / steps * 1e3, 5),
             "fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
             "ratio": round(t_fa / t_fwd, 3), "J": out["J"], "max_abs_dJ_dc2": out["g"],
-            "note": "graph-replayed 50-step segments; recompute from checkpoints in the backward"}
+            "store_all": {"fwd_adj_ms_per_step": round(t_all / steps * 1e3, 5),
+                          "ratio": round(t_all / t_fwd, 3),
+                          "history_GB": round((steps + 1) * c2.numel() * 4 / 1e9, 1)},
+            "note": "graph-replayed 50-step segments; recompute from checkpoints in the backward "
+                    "(store_all: every state kept, no recompute)"}
 
 
 def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):

@@ -465,6 +465,15 @@ int pde_affine_f32(const float *x, float *out, int64_t nodes, int32_t ncomp, int
 int pde_jacobi_f32(const float *p, const float *rhs, float *out, int64_t nx, int64_t ny,
                    int64_t nz, int32_t ghost, float h2, float inv, void *stream);
 
+/* pde_jacobi_f32 on a field whose ghosts a legacy Neumann-0 boundary on every
+ * face would fill first (ghost = coef * 0 + mirror, coef = -2 dx on '-' faces,
+ * +2 dx on '+' faces): each ghost-layer neighbour is read as that value from
+ * its mirror across the wall, so boundary + sweep is one launch with the same
+ * FP32 results; p's ghosts are neither read nor written.  Every eligible
+ * interior extent must be >= 2. */
+int pde_jacobi_neumann0_f32(const float *p, const float *rhs, float *out, int64_t nx, int64_t ny,
+                            int64_t nz, int32_t ghost, float h2, float inv, void *stream);
+
 /* Transpose of pde_jacobi_f32: gp[m] = inv * (sum over axes ascending of
  * gout[m+e] then gout[m-e], each taken when that node is interior) on every
  * padded node; grhs[n] = -h2 * (inv * gout[n]) on interior nodes, 0 on ghosts

@@ -71,6 +71,7 @@ SIGNATURES = {
     "pde_immersed_apply_f32": [_P, _P, _P, _I64, _P, _I64, _F, _I32, _I32, _I64, _I64, _I64,
                                _F, _P],
     "pde_jacobi_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
+    "pde_jacobi_neumann0_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_adjoint_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_matvec_t_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
     "pde_version": [],

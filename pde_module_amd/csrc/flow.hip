@@ -55,6 +55,33 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const float *__restrict__ p
         }
 }
 
+// The sweep of a field whose ghosts a Neumann-0 legacy boundary would fill
+// first ('ALL', value 0: ghost b - s e = coef * 0 + p[b + s e], coef = -2 dx
+// on '-' faces and +2 dx on '+' faces): a neighbour in the ghost layer is read
+// as (-0 or +0) + its mirror across the wall, the same FP32 value the boundary
+// writes, so one launch replaces boundary + sweep and p's ghosts are not read.
+__global__ __launch_bounds__(256) void jacobi_neumann0_kernel(const float *__restrict__ p,
+                                                              const float *__restrict__ rhs,
+                                                              float *__restrict__ out, JacGeom g,
+                                                              float h2, float inv)
+{
+    const int64_t k = g.G[2] + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= g.G[2] + g.E[2]) return;
+    for (int64_t i0 = g.G[0] + blockIdx.z; i0 < g.G[0] + g.E[0]; i0 += gridDim.z)
+        for (int64_t i1 = g.G[1] + blockIdx.y; i1 < g.G[1] + g.E[1]; i1 += gridDim.y) {
+            const int64_t e = i0 * g.S[0] + i1 * g.S[1] + k;
+            const int64_t c[3] = {i0, i1, k};
+            float acc = 0.0f;
+            for (int a = 3 - g.ndim; a < 3; ++a) {
+                const float lo = c[a] - 1 < g.G[a] ? -0.0f + p[e + g.S[a]] : p[e - g.S[a]];
+                const float hi = c[a] + 1 >= g.G[a] + g.E[a] ? 0.0f + p[e - g.S[a]] : p[e + g.S[a]];
+                acc += lo;
+                acc += hi;
+            }
+            out[e] = inv * (acc - h2 * rhs[e]);
+        }
+}
+
 // Transpose: for every padded node m, acc over axes ascending of g[m + e]
 // (m is the -1 neighbour of m + e) then g[m - e], each when that node is
 // interior; gp[m] = inv * acc.  grhs[n] = -h2 * (inv * g[n]) on interior n,
@@ -119,6 +146,25 @@ extern "C" int pde_jacobi_f32(const float *p, const float *rhs, float *out, int6
                     clamp_grid(G.E[0], 65535));
     hipLaunchKernelGGL(jacobi_kernel, grid, dim3(256), 0, as_stream(stream), p, rhs, out, g, h2,
                        inv);
+    return launch_status();
+}
+
+extern "C" int pde_jacobi_neumann0_f32(const float *p, const float *rhs, float *out, int64_t nx,
+                                       int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
+                                       void *stream)
+{
+    if (!p || !rhs || !out || out == p || out == rhs) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    for (int a = 3 - G.ndim; a < 3; ++a)
+        if (G.E[a] < 2) return PDE_ESHAPE;    // a wall node's mirror must be interior
+    const JacGeom g = jac_geom(G);
+    const dim3 grid(clamp_grid((G.E[2] + 255) / 256, 0x7fffffff), clamp_grid(G.E[1], 65535),
+                    clamp_grid(G.E[0], 65535));
+    hipLaunchKernelGGL(jacobi_neumann0_kernel, grid, dim3(256), 0, as_stream(stream), p, rhs, out,
+                       g, h2, inv);
     return launch_status();
 }
 

@@ -153,7 +153,7 @@ class ProjectionCavity:
         self.b = {k: z(*e) for k, e in (("u2", (2,)), ("du", (2, 2)), ("conv", (2,)),
                                          ("diff", (2,)), ("F", (2,)), ("us", (2,)),
                                          ("us2", (2,)), ("rhs", (1,)), ("pf", (1,)),
-                                         ("gp", (2,)))}
+                                         ("p0", (1,)), ("gp", (2,)))}
         m = np.zeros(self.shape + (1,), np.float32)
         m[self.g:P - self.g, self.g:P - self.g] = 1.0
         self.mask = torch.from_numpy(m).to(self.device)
@@ -181,14 +181,19 @@ class ProjectionCavity:
         self.u_bc.table.apply(b["us"], b["us2"], None)
         N.call("pde_divergence_f32", b["us2"].data_ptr(), b["rhs"].data_ptr(), *grid, 1, g,
                *self.wc, np.float32(1.0 / self.dt), s)
-        # each sweep reads the boundary-applied copy pf, so it can write p_out
-        # even when p_out is p
+        # Neumann-0 walls folded into each sweep (pde_jacobi_neumann0_f32: same
+        # values as applying p_bc and sweeping); sweeps alternate between p_out
+        # and pf so that no sweep writes what it reads, ending in p_out
+        bufs = (p_out, b["pf"]) if self.sweeps % 2 == 1 else (b["pf"], p_out)
         cur = p
-        for _ in range(self.sweeps):
-            self.p_bc.table.apply(cur, b["pf"], None)
-            N.call("pde_jacobi_f32", b["pf"].data_ptr(), b["rhs"].data_ptr(), p_out.data_ptr(),
-                   *grid, g, self.h2, self.inv, s)
-            cur = p_out
+        for k in range(self.sweeps):
+            dst = bufs[k % 2]
+            if dst is cur:                       # p_out is p on the first sweep
+                N.call("pde_copy_f32", cur.data_ptr(), b["p0"].data_ptr(), cur.numel(), s)
+                cur = b["p0"]
+            N.call("pde_jacobi_neumann0_f32", cur.data_ptr(), b["rhs"].data_ptr(),
+                   dst.data_ptr(), *grid, g, self.h2, self.inv, s)
+            cur = dst
         if cur is not p_out:
             N.call("pde_copy_f32", cur.data_ptr(), p_out.data_ptr(), p_out.numel(), s)
         self.p_bc.table.apply(p_out, b["pf"], None)

@@ -120,3 +120,28 @@ def test_kinetic_energy_gradient_matches_fp64_finite_differences(dev):
         ad = float(np.sum(g32 * d))
         print(f"dJ/du0 . d: adjoint {ad:.8e}  fd {fd:.8e}", flush=True)
         assert abs(ad - fd) <= 1e-4 * abs(fd) + 1e-12
+
+
+@pytest.mark.parametrize("shape", [(11, 9, 1), (7, 6, 8), (12, 1, 1)])
+def test_folded_neumann_sweep_equals_boundary_then_sweep(dev, shape):
+    """pde_jacobi_neumann0_f32 == legacy vonNeumann_BC('ALL', 0) + pde_jacobi_f32,
+    bit for bit, including fields holding -0.0 next to the walls."""
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    rng = np.random.default_rng(8)
+    p = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    p[1, ...] = -0.0                      # signed zeros beside the -X wall
+    p[-2, ...] = -0.0                     # and the +X wall
+    rhs = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    nd = len(O.eligible_axes(shape))
+    h2, inv = np.float32(0.021), np.float32(1.0 / (2 * nd))
+    pd, rd = torch.from_numpy(p).to(dev), torch.from_numpy(rhs).to(dev)
+    bc = LegacyGridBoundary(pd, 0.1, 1)
+    bc.vonNeumann_BC("ALL", 0.0)
+    pf = bc(pd)
+    a = torch.zeros_like(pd)
+    b = torch.zeros_like(pd)
+    N.call("pde_jacobi_f32", pf.data_ptr(), rd.data_ptr(), a.data_ptr(), *shape, 1, h2, inv,
+           N.stream_of(a))
+    N.call("pde_jacobi_neumann0_f32", pd.data_ptr(), rd.data_ptr(), b.data_ptr(), *shape, 1, h2,
+           inv, N.stream_of(b))
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
