@@ -226,6 +226,95 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
                     "per-instance: HeatStep per grid (boundary + fused step launches), eager"}
 
 
+def heat_config_timing(dev, steps=1000, cpu_steps=100):
+    """Heat workload (256^2, 8 seeded Gaussian bumps, Dirichlet 0 walls,
+    dt = 0.2 dx^2 / alpha, 1000 steps): the fused HeatStep loop replayed as a
+    HIP graph, the reference-shaped module loop (legacy GridBoundary +
+    Laplacian + ForwardEuler, eager), the C FP32 restatement single-threaded on
+    ``cpu_steps`` steps, and the accumulated error of the GPU state against the
+    FP64 oracle at steps 10, 100 and 1000 (relative to max |u|)."""
+    import torch
+    from oracle import c_oracle
+    from oracle import np_oracle as O
+    from pde_module_amd import ForwardEuler, HeatStep, Laplacian
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    from pde_module_amd.workloads import heat_2d_config
+    cfg = heat_2d_config(seed=0)
+    g = cfg.g
+    u0 = torch.from_numpy(cfg.u0).to(dev)
+    mk_bc = lambda: LegacyGridBoundary(u0, cfg.dx, g)
+    bc = mk_bc()
+    bc.dirichlet_BC("ALL", 0.0)
+    fused = HeatStep(bc, cfg.dx, g, cfg.alpha, cfg.dt)
+    u = fused(fused(u0))                       # both ping-pong buffers set up
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    a = fused._bufs[0]
+    with torch.cuda.graph(graph):
+        fused(fused(a))                        # two steps: back in buffer a
+    torch.cuda.synchronize()
+    a.copy_(u0)
+    snaps = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps // 2):
+        graph.replay()
+        if 2 * (k + 1) in (10, 100, 1000):
+            torch.cuda.synchronize()
+            snaps[2 * (k + 1)] = a.cpu().numpy()
+    torch.cuda.synchronize()
+    t_fused = (time.perf_counter() - t0) / steps
+    # module-by-module (the reference's loop shape), eager, 100 steps
+    bcm = mk_bc()
+    bcm.dirichlet_BC("ALL", 0.0)
+    lap, euler = Laplacian(1, cfg.dx, g), ForwardEuler()
+    um = u0
+    for _ in range(3):
+        u2 = bcm(um)
+        um = euler(u2, lap(u2, cfg.alpha), cfg.dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        u2 = bcm(um)
+        um = euler(u2, lap(u2, cfg.alpha), cfg.dt)
+    torch.cuda.synchronize()
+    t_mod = (time.perf_counter() - t0) / 100
+    # CPU: the C restatement, one thread
+    shape = cfg.u0.shape[:3]
+    bc_fn = lambda x: c_oracle.bc_legacy(x, bc.boundary_ijk_indices, bc.boundary_interior,
+                                         bc.boundary_type, bc.boundary_value, cfg.dx)
+    w = O.laplacian_weights(cfg.dx)
+    uc = cfg.u0.copy()
+    t0 = time.perf_counter()
+    for _ in range(cpu_steps):
+        u2c = bc_fn(uc)
+        uc = c_oracle.forward_euler(u2c, c_oracle.laplacian(u2c, w, np.float32(cfg.alpha), g),
+                                    np.float32(cfg.dt))
+    t_cpu = (time.perf_counter() - t0) / cpu_steps
+    # FP64 oracle: accumulated error
+    ijk, _, inter = O.legacy_boundary_layout(shape, g)
+    bt, bv = np.ones((len(ijk), 1), np.int8), np.zeros((len(ijk), 1))
+    bc64 = lambda x: O.bc_legacy(x, ijk, inter, bt, bv, cfg.dx)
+    w64 = O.laplacian_weights(cfg.dx, np.float64)
+    mask = O.interior_mask(shape, g)[..., None]
+    u64 = cfg.u0.astype(np.float64)
+    errs = {}
+    for step in range(1, steps + 1):
+        u64 = O.heat_step(u64, w64, cfg.alpha, g, cfg.dt, bc64)
+        if step in snaps:
+            s = snaps[step]
+            errs[str(step)] = float(np.max(np.abs(s[mask] - u64[mask])) / np.max(np.abs(u64[mask])))
+    lups = cfg.n * cfg.n
+    return {"grid": [cfg.n, cfg.n], "steps": steps,
+            "fused_graph_us_per_step": round(t_fused * 1e6, 2),
+            "fused_glups": round(lups / t_fused / 1e9, 2),
+            "module_loop_us_per_step": round(t_mod * 1e6, 2),
+            "cpu_1thread_us_per_step": round(t_cpu * 1e6, 1),
+            "cpu_glups": round(lups / t_cpu / 1e9, 4),
+            "rel_error_vs_fp64": errs,
+            "note": "cache-resident (266 KB field): launch- and latency-bound, not an HBM figure"}
+
+
 def projection_cavity_timing(dev, n=1024, steps=20, adj_steps=4, seeds=16):
     """Projection-method cavity workload (1024^2, Re = 100, 50 Jacobi sweeps per
     step, seeded divergence-free u0 of amplitude 1e-3): graph-replayed forward
@@ -332,7 +421,7 @@ def main():
 
     progress(f"operators timed: step {t_step_kernel * 1e6:.1f} us, drop-in laplacian "
              f"{t_lap * 1e6:.1f} us")
-    wave = ensemble = cavity = None
+    wave = ensemble = cavity = heat = None
     if not args.quick:
         wave = wave_adjoint_timing(dev)
         progress("wave forward / forward+adjoint timed")
@@ -340,6 +429,8 @@ def main():
         progress("heat ensemble timed")
         cavity = projection_cavity_timing(dev)
         progress("projection cavity timed")
+        heat = heat_config_timing(dev)
+        progress("heat 256^2 timed")
     traffic = pmc_traffic()
     rec = {
         "metric": "GLUPS & % HBM peak (3D 7-pt FP32); fwd+adjoint vs fwd step time",
@@ -382,6 +473,7 @@ def main():
             "wave2048_fwd_vs_fwd_adjoint": wave,
             "heat_ensemble_512": ensemble,
             "projection_cavity_1024": cavity,
+            "heat_256": heat,
         },
         "fwd_adjoint_vs_fwd": wave["ratio"] if wave else None,
     }
