@@ -1,6 +1,6 @@
 """GPU reverse-mode checks: operator autograd against the transpose kernels and
 dot products; checkpointed heat and wave rollouts against torch autograd of the
-FP64 restatement (<= 1e-4 relative gradient error), and
+FP64 restatement (bound: <= 1e-4 relative gradient error), and
 checkpoint-interval invariance (bit for bit)."""
 import numpy as np
 import pytest
