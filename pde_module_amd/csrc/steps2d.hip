@@ -41,39 +41,64 @@ struct Args2 {
     int64_t step_delta;
 };
 
-// Points of padded row x in columns [k0, k0 + 4): add scale * data[n][slot]
-// to the matching elements of v, in list order.
-__device__ __forceinline__ void pts_add(const pde_point_set &P, int64_t x, int64_t k0, int64_t n,
-                                        v4 &v)
+// Point lists are walked by the whole wave: the row index is the same on
+// every lane, the wave loads up to 64 of the row's points at once (one per
+// lane), then each lane walks them in list order through v_readlane and
+// applies those in its own columns.  A row's cost is one round of loads
+// however many points it holds, and sums at a node keep the list order.
+
+// [a, b): the list range of padded row x (empty outside the field).
+__device__ __forceinline__ void pts_range(const pde_point_set &P, int64_t x, int64_t P1, int &a,
+                                          int &b)
 {
-    const int a = P.row_start[x], b = P.row_start[x + 1];
-    for (int p = a; p < b; ++p) {
-        const int64_t d = (int64_t)P.col[p] - k0;
-        if (d >= 0 && d < 4) {
-            const int i = (int)d;
-            set_el(v, i, el(v, i) + P.scale * P.data[n * P.stride + P.slot[p]]);
+    a = b = 0;
+    if (x < 0 || x >= P1) return;
+    a = __builtin_amdgcn_readfirstlane(P.row_start[x]);
+    b = __builtin_amdgcn_readfirstlane(P.row_start[x + 1]);
+}
+
+// Adds scale * data[n][slot] of the points [a, b) to the elements of v in
+// columns k0 .. k0+3, and to ex when the point's column is kex (kex < 0:
+// none).  Every lane of the wave calls it with the same range.
+__device__ __forceinline__ void pts_add(const pde_point_set &P, int a, int b, int64_t k0,
+                                        int64_t n, int lane, v4 &v, int64_t kex, float &ex)
+{
+    for (int base = a; base < b; base += 64) {
+        const int p = base + lane;
+        int col = -1;
+        float val = 0.0f;
+        if (p < b) {
+            col = P.col[p];
+            val = P.scale * P.data[n * P.stride + P.slot[p]];
+        }
+        const int cnt = b - base < 64 ? b - base : 64;
+        for (int i = 0; i < cnt; ++i) {
+            const int64_t c = __builtin_amdgcn_readlane(col, i);
+            const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(val), i));
+            const int64_t dd = c - k0;
+            if (dd >= 0 && dd < 4) set_el(v, (int)dd, el(v, (int)dd) + d);
+            if (c == kex) ex = ex + d;
         }
     }
 }
 
-// The same for the single node (x, kk).
-__device__ __forceinline__ float pts_add_at(const pde_point_set &P, int64_t x, int64_t kk,
-                                            int64_t n, float v)
+// data[n][slot] = v at the point's column for the points [a, b) whose column
+// lies in the wave's strip [kw0, kw0 + 256) (lane l holds columns
+// kw0 + 4l .. kw0 + 4l + 3).  Every lane of the wave calls it.
+__device__ __forceinline__ void pts_store(const pde_point_set &P, int a, int b, int64_t kw0,
+                                          int64_t n, int lane, const v4 &v)
 {
-    const int a = P.row_start[x], b = P.row_start[x + 1];
-    for (int p = a; p < b; ++p)
-        if ((int64_t)P.col[p] == kk) v = v + P.scale * P.data[n * P.stride + P.slot[p]];
-    return v;
-}
-
-// data[n][slot] = v[element] for the points of row x in the lane's columns.
-__device__ __forceinline__ void pts_store(const pde_point_set &P, int64_t x, int64_t k0, int64_t n,
-                                          const v4 &v)
-{
-    const int a = P.row_start[x], b = P.row_start[x + 1];
-    for (int p = a; p < b; ++p) {
-        const int64_t d = (int64_t)P.col[p] - k0;
-        if (d >= 0 && d < 4) P.data[n * P.stride + P.slot[p]] = el(v, (int)d);
+    for (int base = a; base < b; base += 64) {
+        const int p = base + lane;
+        const int64_t d = p < b ? (int64_t)P.col[p] - kw0 : -1;
+        const bool mine = d >= 0 && d < 256;
+        const int from = mine ? (int)(d >> 2) : lane;
+        const float e0 = __shfl(v.x, from, 64), e1 = __shfl(v.y, from, 64);
+        const float e2 = __shfl(v.z, from, 64), e3 = __shfl(v.w, from, 64);
+        if (mine) {
+            const int e = (int)(d & 3);
+            P.data[n * P.stride + P.slot[p]] = e == 0 ? e0 : (e == 1 ? e1 : (e == 2 ? e2 : e3));
+        }
     }
 }
 
@@ -128,34 +153,44 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         v4 rows[R + 2];
 #pragma unroll
         for (int r = 0; r < R + 2; ++r) rows[r] = ld_row4(A.a, t, x0 - 1 + r, k0);
+        const int64_t kw0 = (int64_t)qt * 256;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int64_t x = x0 + r;
             float left, right;
             col_edges(A.a, t, x, k0, rows[r + 1], lane, 64, left, right);
-            if (!lane_ok || x >= t.P1) continue;
+            if (x >= t.P1) break;                 // the same on every lane
             const bool xin = inside(x, t.g, t.P1);
             v4 un = rows[r + 1];
-            {
-                v4 vo = ld_row4(A.b, t, x, k0);
-                if (has_src) pts_add(A.src, x, k0, n, vo);
-                v4 cc = {1.0f, 1.0f, 1.0f, 1.0f};
-                if (A.c2) cc = ld_row4(A.c2, t, x, k0);
-                v4 vn = vo;
+            v4 vo = ld_row4(A.b, t, x, k0);
+            if (has_src) {
+                int a, b;
+                float none = 0.0f;
+                pts_range(A.src, x, t.P1, a, b);
+                pts_add(A.src, a, b, k0, n, lane, vo, -1, none);
+            }
+            v4 cc = {1.0f, 1.0f, 1.0f, 1.0f};
+            if (A.c2) cc = ld_row4(A.c2, t, x, k0);
+            v4 vn = vo;
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (xin && kin[i]) {
-                        const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i,
-                                               A.wm, A.wc, A.wp);
-                        float rhs = A.alpha * acc;
-                        if (A.c2) rhs = el(cc, i) * rhs;
-                        const float vnew = el(vo, i) + dt * rhs;
-                        set_el(vn, i, vnew);
-                        set_el(un, i, el(rows[r + 1], i) + dt * (first ? el(vo, i) : vnew));
-                    }
+            for (int i = 0; i < 4; ++i)
+                if (xin && kin[i]) {
+                    const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i,
+                                           A.wm, A.wc, A.wp);
+                    float rhs = A.alpha * acc;
+                    if (A.c2) rhs = el(cc, i) * rhs;
+                    const float vnew = el(vo, i) + dt * rhs;
+                    set_el(vn, i, vnew);
+                    set_el(un, i, el(rows[r + 1], i) + dt * (first ? el(vo, i) : vnew));
+                }
+            if (lane_ok) {
                 st_row4(A.o2, t, x, k0, vn);
                 st_row4(A.o1, t, x, k0, un);
-                if (has_rec) pts_store(A.rec, x, k0, n, un);
+            }
+            if (has_rec) {
+                int a, b;
+                pts_range(A.rec, x, t.P1, a, b);
+                pts_store(A.rec, a, b, kw0, n, lane, un);
             }
         }
         return;
@@ -193,15 +228,31 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
     // Rolling window over rows x-1, x, x+1: each input row is loaded once.
     struct RowIn {
         v4 gn, gv, h, u;
+        float he;   // lanes 0 and 63: h at the column just outside the strip
     };
     auto load_row = [&](int64_t x) {
         RowIn R_;
         R_.gn = ld_row4(A.a, t, x, k0);
-        if (has_rec && x >= 0 && x < t.P1) pts_add(A.rec, x, k0, n, R_.gn);
         R_.gv = ld_row4(A.b, t, x, k0);
         v4 cc = {1.0f, 1.0f, 1.0f, 1.0f};
         if (A.c2) cc = ld_row4(A.c2, t, x, k0);
         const bool xin = inside(x, t.g, t.P1);
+        // the outer column of the strip's edge lanes (0 outside the interior)
+        const int64_t kk = lane == 0 ? k0 - 1 : k0 + 4;
+        const bool e_ok = (lane == 0 || lane == 63) && kk >= 0 && kk < t.P2 && xin &&
+                          inside(kk, t.g, t.P2);
+        float ea = 0.0f, eb = 0.0f, ec = 1.0f;
+        if (e_ok) {
+            const int64_t e = x * t.P2 + kk;
+            ea = A.a[e];
+            eb = A.b[e];
+            if (A.c2) ec = A.c2[e];
+        }
+        if (has_rec) {
+            int a, b;
+            pts_range(A.rec, x, t.P1, a, b);
+            pts_add(A.rec, a, b, k0, n, lane, R_.gn, e_ok ? kk : -1, ea);
+        }
         v4 h = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -210,16 +261,13 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
                 set_el(h, i, A.c2 ? el(cc, i) * tv : tv);
             }
         R_.h = h;
+        R_.he = 0.0f;
+        if (e_ok) {
+            const float tv = first ? eb : eb + dt * ea;
+            R_.he = A.c2 ? ec * tv : tv;
+        }
         R_.u = A.gc2 ? ld_row4(A.u2, t, x, k0) : v4{0.0f, 0.0f, 0.0f, 0.0f};
         return R_;
-    };
-    // h at a column outside the lane's strip (lanes 0 and 63)
-    auto h_at = [&](int64_t x, int64_t kk) {
-        if (kk < 0 || kk >= t.P2 || !inside(x, t.g, t.P1) || !inside(kk, t.g, t.P2)) return 0.0f;
-        const int64_t e = x * t.P2 + kk;
-        const float gnv = has_rec ? pts_add_at(A.rec, x, kk, n, A.a[e]) : A.a[e];
-        const float tv = first ? A.b[e] : A.b[e] + dt * gnv;
-        return A.c2 ? A.c2[e] * tv : tv;
     };
     v4 hup = load_row(x0 - 1).h;
     v4 uup = A.gc2 ? ld_row4(A.u2, t, x0 - 1, k0) : v4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -231,11 +279,8 @@ __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
         float hl, hrt, ul = 0.0f, urt = 0.0f;
         col_edges(nullptr, t, -1, k0, mid.h, lane, 64, hl, hrt);
         if (A.gc2) col_edges(A.u2, t, x, k0, mid.u, lane, 64, ul, urt);
-        if (lane == 0 || lane == 63) {   // one load per array for both edges
-            const float hv = h_at(x, lane == 0 ? k0 - 1 : k0 + 4);
-            if (lane == 0) hl = hv;
-            else hrt = hv;
-        }
+        if (lane == 0) hl = mid.he;
+        else if (lane == 63) hrt = mid.he;
         if (lane_ok && x < t.P1) {
             const bool xin = inside(x, t.g, t.P1);
             const bool upin = inside(x - 1, t.g, t.P1), dnin = inside(x + 1, t.g, t.P1);

@@ -111,17 +111,33 @@ def _point_set(idx, p2, data, scale):
     return rows, cols, order
 
 
-@pytest.mark.parametrize("shape", [(40, 37, 1), (1, 70, 300)])
-def test_fused_wave_steps_match_separate_kernels(dev, shape):
+def _points(layout, p1, p2):
+    """Source and receiver flat indices: a few scattered points ("spread"), or
+    rows holding more than 64 points each, duplicates included ("dense_row":
+    the fused kernels walk a row's list 64 points at a time)."""
+    if layout == "spread":
+        src = np.array([5 * p2 + 7, 2 * p2 + 3, 5 * p2 + 7, (p1 - 3) * p2 + p2 - 4])
+        rec = np.array([1 * p2 + 1, 6 * p2 + 9, 3 * p2 + 4, (p1 - 2) * p2 + 2])
+        return src, rec
+    cols = np.arange(70) * (p2 - 1) // 69
+    src = np.concatenate([5 * p2 + cols, [5 * p2 + 7, 5 * p2 + 7, 2 * p2 + 3]])
+    # columns 255 / 256 straddle the first 256-column strip edge
+    rec = np.concatenate([(p1 - 2) * p2 + cols[::-1],
+                          (p1 - 2) * p2 + np.array([256, 256, 255]), [6 * p2 + 1]])
+    return src, rec
+
+
+@pytest.mark.parametrize("shape,layout", [((40, 37, 1), "spread"), ((1, 70, 300), "spread"),
+                                          ((40, 300, 1), "dense_row")])
+def test_fused_wave_steps_match_separate_kernels(dev, shape, layout):
     """pde_wave_step_fused_f32 / pde_wave_step_adjoint_fused_f32 against the
     separate source, step, receiver and counter kernels, over 5 steps and back."""
-    print(f"fused wave steps {shape}", flush=True)
+    print(f"fused wave steps {shape} {layout}", flush=True)
     p1, p2 = [s for s in shape if s > 1]
     rng = np.random.default_rng(9)
     u, v, c2 = (T(x, dev) for x in _fields(shape, 3, 11))
     steps, g = 5, 1
-    src = np.array([5 * p2 + 7, 2 * p2 + 3, 5 * p2 + 7, (p1 - 3) * p2 + p2 - 4])
-    rec = np.array([1 * p2 + 1, 6 * p2 + 9, 3 * p2 + 4, (p1 - 2) * p2 + 2])
+    src, rec = _points(layout, p1, p2)
     wav = T(rng.uniform(-1, 1, (steps, len(src))).astype(np.float32), dev)
     w = O.laplacian_weights(0.1)
     alpha, dt = np.float32(0.7), np.float32(0.05)
@@ -188,8 +204,9 @@ def test_fused_wave_steps_match_separate_kernels(dev, shape):
            alpha, dt, ctr.data_ptr() + 8, ctr.data_ptr(), ps_grec, st)
     c1.fill_(steps - 1)
     gun2 = gun.clone()
-    N.call("pde_scatter_add_step_f32", gun2.data_ptr(), ridx.data_ptr(), grec.data_ptr(),
-           len(rec), len(rec), np.float32(1.0), c1.data_ptr(), st)
+    for j in range(len(rec)):     # one point per call: duplicates summed in list order
+        N.call("pde_scatter_add_step_f32", gun2.data_ptr(), ridx[j:].data_ptr(),
+               grec[:, j:].data_ptr(), 1, len(rec), np.float32(1.0), c1.data_ptr(), st)
     N.call("pde_wave_step_adjoint_ctr_f32", gun2.data_ptr(), gvn.data_ptr(), c2.data_ptr(),
            u2.data_ptr(), outs[2].data_ptr(), outs[3].data_ptr(), gc_s.data_ptr(), *shape, g, *w,
            alpha, dt, c1.data_ptr(), st)
