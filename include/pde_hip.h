@@ -481,6 +481,20 @@ int pde_jacobi_neumann0_f32(const float *p, const float *rhs, float *out, int64_
 int pde_jacobi_adjoint_f32(const float *gout, float *gp, float *grhs, int64_t nx, int64_t ny,
                            int64_t nz, int32_t ghost, float h2, float inv, void *stream);
 
+/* Transpose of pde_jacobi_neumann0_f32 as a linear map of (p's interior, rhs):
+ * gp = 0 on ghosts; on interior m, inv * (sum over axes ascending of g[m + e]
+ * then g[m - e], each when interior), then per axis ascending + inv * g[m - e]
+ * when m - e is the low-wall node (m - 2e a ghost) and + inv * g[m + e] when
+ * m + e is the high-wall node (the folded ghost mirrors); grhs[m] +=
+ * -h2 * (inv * g[m]) on interior m (accumulated; ghosts untouched).  The
+ * same values as pde_jacobi_adjoint_f32 followed by the Neumann-0 boundary's
+ * transpose, in one launch.  Every eligible axis needs 2+ interior nodes.  No
+ * reference counterpart (the reference has no adjoints); it serves the
+ * gradient of the projection cavity. */
+int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, float *grhs, int64_t nx,
+                                    int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
+                                    void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -111,6 +111,45 @@ __global__ __launch_bounds__(256) void jacobi_adjoint_kernel(const float *__rest
         }
 }
 
+// Transpose of the folded sweep (jacobi_neumann0_kernel) as a map of p's
+// interior and rhs: thread per padded node m.  On interior m, acc over axes
+// ascending of g[m + e] then g[m - e] (each when interior), gp[m] = inv * acc,
+// then per axis ascending the wall mirrors: + inv * g[m - e] when m - e is
+// interior and m - 2e a ghost (the low wall's ghost of m - e mirrors m), then
+// + inv * g[m + e] when m + e is interior and m + 2e a ghost.  That is the
+// order of the unfolded transpose (pde_jacobi_adjoint_f32, whose ghost
+// entries inv * g[wall node] the boundary's transpose then adds to the
+// mirror).  gp = 0 on ghosts; grhs[m] += -h2 * (inv * g[m]) on interior m.
+__global__ __launch_bounds__(256) void jacobi_neumann0_adjoint_kernel(
+    const float *__restrict__ gout, float *__restrict__ gp, float *__restrict__ grhs, JacGeom g,
+    int64_t P0, int64_t P1, int64_t P2, float h2, float inv)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    for (int64_t i0 = blockIdx.z; i0 < P0; i0 += gridDim.z)
+        for (int64_t i1 = blockIdx.y; i1 < P1; i1 += gridDim.y) {
+            const int64_t e = i0 * g.S[0] + i1 * g.S[1] + k;
+            if (!interior(g, i0, i1, k)) {
+                gp[e] = 0.0f;
+                continue;
+            }
+            const int64_t c[3] = {i0, i1, k};
+            float acc = 0.0f;
+            for (int a = 3 - g.ndim; a < 3; ++a) {
+                if (c[a] + 1 < g.G[a] + g.E[a]) acc += gout[e + g.S[a]];
+                if (c[a] - 1 >= g.G[a]) acc += gout[e - g.S[a]];
+            }
+            float v = inv * acc;
+            for (int a = 3 - g.ndim; a < 3; ++a) {
+                if (c[a] - 1 >= g.G[a] && c[a] - 2 < g.G[a]) v = v + inv * gout[e - g.S[a]];
+                if (c[a] + 1 < g.G[a] + g.E[a] && c[a] + 2 >= g.G[a] + g.E[a])
+                    v = v + inv * gout[e + g.S[a]];
+            }
+            gp[e] = v;
+            grhs[e] = grhs[e] + -h2 * (inv * gout[e]);
+        }
+}
+
 // out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
 __global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
                                                        const float *__restrict__ gv,
@@ -182,6 +221,25 @@ extern "C" int pde_jacobi_adjoint_f32(const float *gout, float *gp, float *grhs,
                     clamp_grid(G.P[0], 65535));
     hipLaunchKernelGGL(jacobi_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout, gp, grhs,
                        g, G.P[0], G.P[1], G.P[2], h2, inv);
+    return launch_status();
+}
+
+extern "C" int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, float *grhs,
+                                               int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                               float h2, float inv, void *stream)
+{
+    if (!gout || !gp || !grhs || gp == gout || grhs == gout || grhs == gp) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0) return PDE_ESHAPE;
+    for (int a = 3 - G.ndim; a < 3; ++a)
+        if (G.E[a] < 2) return PDE_ESHAPE;
+    const JacGeom g = jac_geom(G);
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535),
+                    clamp_grid(G.P[0], 65535));
+    hipLaunchKernelGGL(jacobi_neumann0_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout,
+                       gp, grhs, g, G.P[0], G.P[1], G.P[2], h2, inv);
     return launch_status();
 }
 

@@ -12,11 +12,16 @@ variant of the same flow, built from the same operators:
     u   = u* - dt * grad(B_p(p))
 
 with J one Jacobi sweep of lap(p) = rhs (``pde_jacobi_f32``).  ``step`` runs on
-preallocated buffers through the HIP kernels (capturable in a HIP graph);
-``kinetic_energy_rollout`` is the same loop as differentiable functions whose
-backward passes are the transposed kernels, checkpointed every
-``checkpoint_every`` steps, giving dJ/du0 for J = 1/2 sum |u|^2 dx^2 over the
-interior at the final step.
+preallocated buffers through the HIP kernels (capturable in a HIP graph).
+``step_adjoint`` is its transpose: the same kernels' transposes in reverse
+order on preallocated buffers (also capturable); the step is nonlinear only
+through u (the convection term), so the transpose needs just the step's input
+velocity.  ``kinetic_energy_rollout`` gives dJ/du0 (and dJ/dp0) for J = 1/2
+sum |u|^2 dx^2 over the interior at the final step through ``CavityRollout``
+(every step's velocity kept, or checkpoints every ``checkpoint_every`` steps
+and a recomputed segment); ``kinetic_energy_rollout_autograd`` is the same
+gradient through torch autograd over the differentiable operators
+(``diff_step``), the cross-check.
 """
 from __future__ import annotations
 
@@ -242,10 +247,94 @@ class ProjectionCavity:
         gp = A.grad(A.boundary(p, self.p_bc), self.dx, self.g).squeeze(3)
         return A.forward_euler(us2, gp, -self.dt), p
 
-    def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10):
+    # ------------------------------------------------------------ transpose
+    def _adjoint_buffers(self):
+        if getattr(self, "ga", None) is None:
+            z = lambda *e: torch.zeros(self.shape + e, dtype=torch.float32, device=self.device)
+            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("gpa", (1,)),
+                                             ("gpb", (1,)), ("grhs", (1,)), ("gdiv", (2,)),
+                                             ("gus2", (2,)), ("gus", (2,)), ("gF", (2,)),
+                                             ("gconv", (2,)), ("glap", (2,)), ("gdu", (2, 2)),
+                                             ("gmv", (2,)), ("ggr", (2,)), ("gu2", (2,)))}
+        return self.ga
+
+    def prepare_adjoint(self, like_u, like_p):
+        """Boundary tables and their transposes on the device (call before
+        capturing ``step_adjoint`` in a graph)."""
+        self.u_bc._ensure_table(like_u)
+        self.p_bc._ensure_table(like_p)
+        self.u_bc.table.prepare_adjoint()
+        self.p_bc.table.prepare_adjoint()
+        self._adjoint_buffers()
+
+    def step_adjoint(self, u, gu_out, gp_out, gu, gp):
+        """Transpose of ``step`` at the state u (the step's input velocity):
+        from the gradients (gu_out, gp_out) of the step's outputs, the
+        gradients (gu, gp) of its inputs (u, p).  gu / gp must not alias
+        gu_out / gp_out.  Preallocated buffers only, current stream."""
+        b, a, g, s = self.b, self._adjoint_buffers(), self.g, N.stream_of(u)
+        grid = self.shape
+        nodes = int(np.prod(grid))
+        self.prepare_adjoint(u, gp)
+        # the two forward values the transpose reads: u2 = B_u(u), grad(u2)
+        self.u_bc.table.apply(u, b["u2"], None)
+        N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
+               np.float32(1.0), s)
+        # u_out = us2 - dt * grad(B_p(p_out))
+        N.call("pde_scale_f32", gu_out.data_ptr(), a["ggp"].data_ptr(), gu_out.numel(),
+               np.float32(-self.dt), s)
+        N.call("pde_grad_adjoint_f32", a["ggp"].data_ptr(), a["gpf"].data_ptr(), *grid, 1, g,
+               *self.wc, np.float32(1.0), s)
+        self.p_bc.table.adjoint(a["gpf"], a["gpa"])
+        N.call("pde_lincomb_f32", a["gpa"].data_ptr(), gp_out.data_ptr(), None, None,
+               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
+               a["gpb"].data_ptr(), a["gpb"].numel(), s)
+        # the sweeps p <- J(B_p(p), rhs), last to first; rhs's gradient accumulates
+        a["grhs"].zero_()
+        cur, nxt = a["gpb"], a["gpa"]
+        for _ in range(self.sweeps):
+            N.call("pde_jacobi_neumann0_adjoint_f32", cur.data_ptr(), nxt.data_ptr(),
+                   a["grhs"].data_ptr(), *grid, g, self.h2, self.inv, s)
+            cur, nxt = nxt, cur
+        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), s)
+        # rhs = div(us2) / dt; us2 = B_u(us)
+        N.call("pde_divergence_adjoint_f32", a["grhs"].data_ptr(), a["gdiv"].data_ptr(), *grid, 1,
+               g, *self.wc, np.float32(1.0 / self.dt), s)
+        N.call("pde_lincomb_f32", gu_out.data_ptr(), a["gdiv"].data_ptr(), None, None,
+               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
+               a["gus2"].data_ptr(), a["gus2"].numel(), s)
+        self.u_bc.table.adjoint(a["gus2"], a["gus"])
+        # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2)
+        N.call("pde_scale_f32", a["gus"].data_ptr(), a["gF"].data_ptr(), a["gF"].numel(),
+               np.float32(self.dt), s)
+        N.call("pde_scale_f32", a["gF"].data_ptr(), a["gconv"].data_ptr(), a["gF"].numel(),
+               np.float32(-1.0), s)
+        N.call("pde_laplacian_adjoint_f32", a["gF"].data_ptr(), a["glap"].data_ptr(), *grid, 2, g,
+               *self.wl, np.float32(self.nu), s)
+        N.call("pde_outer_f32", a["gconv"].data_ptr(), b["u2"].data_ptr(), a["gdu"].data_ptr(),
+               nodes, 2, 2, s)
+        N.call("pde_matvec_t_f32", b["du"].data_ptr(), a["gconv"].data_ptr(), a["gmv"].data_ptr(),
+               nodes, 2, 2, s)
+        N.call("pde_grad_adjoint_f32", a["gdu"].data_ptr(), a["ggr"].data_ptr(), *grid, 2, g,
+               *self.wc, np.float32(1.0), s)
+        N.call("pde_lincomb_f32", a["gus"].data_ptr(), a["glap"].data_ptr(), a["gmv"].data_ptr(),
+               a["ggr"].data_ptr(), np.float32(1.0), np.float32(1.0), np.float32(1.0),
+               np.float32(1.0), 4, a["gu2"].data_ptr(), a["gu2"].numel(), s)
+        # u2 = B_u(u)
+        self.u_bc.table.adjoint(a["gu2"], gu)
+        return gu, gp
+
+    def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
         """J(u_N) = 1/2 sum |u_N|^2 dx^2 (interior), differentiable in u0 and
-        p0; segments of ``checkpoint_every`` steps are recomputed in the
-        backward pass."""
+        p0 through ``CavityRollout`` (``checkpoint_every=None``: every step's
+        velocity kept, no recompute)."""
+        ro = CavityRollout(self, steps, checkpoint_every, use_graph)
+        uN, _ = _CavityRolloutFn.apply(u0, p0, ro)
+        return _KineticEnergy.apply(uN, self.mask, float(self.dx * self.dx))
+
+    def kinetic_energy_rollout_autograd(self, u0, p0, steps, checkpoint_every=10):
+        """The same J through torch autograd over ``diff_step`` (segments of
+        ``checkpoint_every`` steps recomputed in the backward pass)."""
         self.u_bc._ensure_table(u0)
         self.p_bc._ensure_table(p0)
 
@@ -261,3 +350,109 @@ class ProjectionCavity:
             u, p = cp.checkpoint(segment, u, p, L, use_reentrant=False)
             done += L
         return _KineticEnergy.apply(u, self.mask, float(self.dx * self.dx))
+
+
+class CavityRollout:
+    """``steps`` projection steps and their transpose on preallocated buffers.
+
+    ``checkpoint_every=None`` keeps the input velocity of every step (steps
+    fields; the transpose needs nothing else); K > 0 keeps (u, p) every K steps
+    and recomputes a segment's velocities before its transposed steps.  With
+    ``use_graph`` the forward step and the transposed step are each captured
+    once in a HIP graph (on fixed buffers) and replayed for every step."""
+
+    def __init__(self, cav, steps, checkpoint_every=None, use_graph=False):
+        self.cav = cav
+        self.steps = int(steps)
+        self.store_all = checkpoint_every is None
+        self.K = self.steps if self.store_all else max(1, min(int(checkpoint_every), self.steps))
+        self.use_graph = bool(use_graph)
+        self._graphs = {}
+
+    def _prepare(self, u0, p0):
+        cav = self.cav
+        cav.prepare_adjoint(u0, p0)
+        E = lambda like, *n: torch.empty(n + tuple(like.shape), dtype=like.dtype,
+                                         device=like.device)
+        self.U, self.P = E(u0), E(p0)
+        self.AU = E(u0)                                    # the transposed step's state
+        self.gU, self.gP, self.gU2, self.gP2 = E(u0), E(p0), E(u0), E(p0)
+        nseg = (self.steps + self.K - 1) // self.K
+        if self.store_all:
+            self.hist = E(u0, max(self.steps, 1))
+        else:
+            self.hist = E(u0, self.K)
+            self.ck_u, self.ck_p = E(u0, nseg), E(p0, nseg)
+
+    def _fwd(self):
+        self._run("fwd", lambda: self.cav.step(self.U, self.P, self.U, self.P))
+
+    def _adj(self):
+        def body():
+            self.cav.step_adjoint(self.AU, self.gU, self.gP, self.gU2, self.gP2)
+            st = N.stream_of(self.gU)
+            N.call("pde_copy_f32", self.gU2.data_ptr(), self.gU.data_ptr(), self.gU.numel(), st)
+            N.call("pde_copy_f32", self.gP2.data_ptr(), self.gP.data_ptr(), self.gP.numel(), st)
+        self._run("adj", body)
+
+    def _run(self, key, fn):
+        if not self.use_graph:
+            fn()
+            return
+        gr = self._graphs.get(key)
+        if gr is None:
+            torch.cuda.synchronize(self.U.device)
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr):
+                fn()
+            self._graphs[key] = gr
+        gr.replay()
+
+    def forward_states(self, u0, p0):
+        N.check_field(u0, "u0")
+        N.check_field(p0, "p0")
+        self._prepare(u0, p0)
+        self.U.copy_(u0)
+        self.P.copy_(p0)
+        for n in range(self.steps):
+            if self.store_all:
+                self.hist[n].copy_(self.U)
+            elif n % self.K == 0:
+                self.ck_u[n // self.K].copy_(self.U)
+                self.ck_p[n // self.K].copy_(self.P)
+            self._fwd()
+        return self.U.clone(), self.P.clone()
+
+    def backward_states(self, guN, gpN):
+        self.gU.copy_(guN)
+        self.gP.copy_(gpN)
+        nseg = (self.steps + self.K - 1) // self.K
+        for sg in reversed(range(nseg)):
+            n0 = sg * self.K
+            L = min(self.K, self.steps - n0)
+            if not self.store_all:
+                self.U.copy_(self.ck_u[sg])
+                self.P.copy_(self.ck_p[sg])
+                for i in range(L):
+                    self.hist[i].copy_(self.U)
+                    self._fwd()
+            for i in reversed(range(L)):
+                self.AU.copy_(self.hist[n0 + i] if self.store_all else self.hist[i])
+                self._adj()
+        return self.gU.clone(), self.gP.clone()
+
+
+class _CavityRolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, p0, ro):
+        uN, pN = ro.forward_states(u0.detach().contiguous(), p0.detach().contiguous())
+        ctx.ro = ro
+        return uN, pN
+
+    @staticmethod
+    def backward(ctx, guN, gpN):
+        ro = ctx.ro
+        guN = torch.zeros_like(ro.U) if guN is None else guN.contiguous()
+        gpN = torch.zeros_like(ro.P) if gpN is None else gpN.contiguous()
+        gu0, gp0 = ctx.ro.backward_states(guN, gpN)
+        return gu0, gp0, None

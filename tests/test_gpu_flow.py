@@ -145,3 +145,84 @@ def test_folded_neumann_sweep_equals_boundary_then_sweep(dev, shape):
     N.call("pde_jacobi_neumann0_f32", pd.data_ptr(), rd.data_ptr(), b.data_ptr(), *shape, 1, h2,
            inv, N.stream_of(b))
     assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+
+
+@pytest.mark.parametrize("shape", [(11, 9, 1), (7, 6, 8), (12, 1, 1), (5, 5, 1)])
+def test_folded_neumann_sweep_transpose(dev, shape):
+    """pde_jacobi_neumann0_adjoint_f32 against the unfolded transpose
+    (pde_jacobi_adjoint_f32, then the Neumann-0 boundary's transpose, grhs
+    summed), and the dot-product identity <J(p, rhs), y> = <p, J^T_p y> +
+    <rhs, J^T_rhs y> over interior p and rhs."""
+    print(f"folded Jacobi transpose {shape}", flush=True)
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    rng = np.random.default_rng(21)
+    nd = len(O.eligible_axes(shape))
+    h2, inv = np.float32(0.021), np.float32(1.0 / (2 * nd))
+    mask = ghost_free_mask(shape, 1, (1,))
+    y = np.where(mask, rng.uniform(-1, 1, shape + (1,)), 0.0).astype(np.float32)
+    g0 = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)     # grhs accumulates onto g0
+    yd = torch.from_numpy(y).to(dev)
+    gp = torch.zeros_like(yd)
+    grhs = torch.from_numpy(g0.copy()).to(dev)
+    N.call("pde_jacobi_neumann0_adjoint_f32", yd.data_ptr(), gp.data_ptr(), grhs.data_ptr(),
+           *shape, 1, h2, inv, N.stream_of(yd))
+    # unfolded
+    bc = LegacyGridBoundary(yd.clone(), 0.1, 1)
+    bc.vonNeumann_BC("ALL", 0.0)
+    bc._ensure_table(yd)
+    a, ra = torch.empty_like(yd), torch.empty_like(yd)
+    N.call("pde_jacobi_adjoint_f32", yd.data_ptr(), a.data_ptr(), ra.data_ptr(), *shape, 1, h2,
+           inv, N.stream_of(yd))
+    b = torch.empty_like(yd)
+    bc.table.adjoint(a, b)
+    got, want = gp.cpu().numpy(), b.cpu().numpy()
+    assert np.all(got[~mask] == 0)
+    np.testing.assert_allclose(got[mask], want[mask], rtol=2e-7, atol=2e-7)
+    want_rhs = np.where(mask, g0 + ra.cpu().numpy(), g0)
+    assert np.array_equal(grhs.cpu().numpy(), want_rhs.astype(np.float32))
+    # dot products (FP64 sums of the FP32 kernels' outputs)
+    p = np.where(mask, rng.uniform(-1, 1, shape + (1,)), 0.0).astype(np.float32)
+    r = np.where(mask, rng.uniform(-1, 1, shape + (1,)), 0.0).astype(np.float32)
+    out = torch.zeros_like(yd)
+    pd, rd = torch.from_numpy(p).to(dev), torch.from_numpy(r).to(dev)   # alive past the launch
+    N.call("pde_jacobi_neumann0_f32", pd.data_ptr(), rd.data_ptr(), out.data_ptr(), *shape, 1,
+           h2, inv, N.stream_of(out))
+    torch.cuda.synchronize()
+    lhs = float(np.sum(out.cpu().numpy().astype(np.float64) * y))
+    gr = grhs.cpu().numpy().astype(np.float64) - g0
+    rhs_ = float(np.sum(p.astype(np.float64) * got) + np.sum(np.where(mask, r, 0.0) * gr))
+    assert abs(lhs - rhs_) <= 1e-5 * max(1.0, abs(lhs))
+
+
+def test_cavity_step_transpose_matches_autograd(dev):
+    """The hand-written transposed rollout (CavityRollout) against torch
+    autograd over the differentiable operators, for dJ/du0 and dJ/dp0; every
+    storage mode and graph replay give the same bits."""
+    n, steps = 18, 5
+    cav = ProjectionCavity(n, jacobi_sweeps=12, dt=2e-3, device=dev)
+    u0 = torch.from_numpy(ns_cavity_u0(4, n) * 50.0).to(dev)
+    rng = np.random.default_rng(5)
+    p0 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, cav.shape + (1,)).astype(np.float32)).to(dev)
+
+    def grads(fn, **kw):
+        x, q = u0.clone().requires_grad_(True), p0.clone().requires_grad_(True)
+        J = fn(x, q, steps, **kw)
+        gx, gq = torch.autograd.grad(J, (x, q))
+        return float(J), gx.cpu().numpy(), gq.cpu().numpy()
+
+    Ja, gua, gpa = grads(cav.kinetic_energy_rollout_autograd, checkpoint_every=2)
+    runs = [grads(cav.kinetic_energy_rollout, checkpoint_every=None),
+            grads(cav.kinetic_energy_rollout, checkpoint_every=2),
+            grads(cav.kinetic_energy_rollout, checkpoint_every=None, use_graph=True),
+            grads(cav.kinetic_energy_rollout, checkpoint_every=3, use_graph=True)]
+    J, gu, gp = runs[0]
+    for Jr, gur, gpr in runs[1:]:
+        assert Jr == J
+        assert same_bits(gur, gu) and same_bits(gpr, gp)
+    assert J == Ja
+    mu = ghost_free_mask(cav.shape, 1, (2,))
+    mp = ghost_free_mask(cav.shape, 1, (1,))
+    scale_u, scale_p = np.abs(gua[mu]).max(), np.abs(gpa[mp]).max()
+    print(f"|dJ/du0| {scale_u:.3e}  |dJ/dp0| {scale_p:.3e}", flush=True)
+    np.testing.assert_allclose(gu, gua, rtol=0, atol=1e-5 * scale_u)
+    np.testing.assert_allclose(gp, gpa, rtol=0, atol=1e-5 * max(scale_p, 1e-30))
