@@ -315,36 +315,60 @@ def heat_config_timing(dev, steps=1000, cpu_steps=100):
             "note": "cache-resident (266 KB field): launch- and latency-bound, not an HBM figure"}
 
 
-def projection_cavity_timing(dev, n=1024, steps=20, adj_steps=4, seeds=16):
+def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=10):
     """Projection-method cavity workload (1024^2, Re = 100, 50 Jacobi sweeps per
-    step, seeded divergence-free u0 of amplitude 1e-3): graph-replayed forward
-    ms/step of one instance (the 16-seed batch is 16 such instances), and one
-    kinetic-energy gradient through ``adj_steps`` steps (checkpointed rollout
-    through the transposed kernels) against the same steps forward."""
+    step, seeded divergence-free u0 of amplitude 1e-3).  One instance: the
+    graph-replayed forward, then the kinetic-energy gradient dJ/du0 through the
+    transposed steps (CavityRollout: every step's velocity kept, and
+    checkpoints every ``checkpoint_every`` steps with recompute), each timed
+    after a warm-up run that captures the graphs.  Batch: ``seeds`` instances
+    (seeds 0..15) stepped by one captured graph, timed per step."""
     import torch
-    from pde_module_amd.flow import ProjectionCavity
+    from pde_module_amd.flow import CavityRollout, ProjectionCavity
     from pde_module_amd.workloads import ns_cavity_u0
     cav = ProjectionCavity(n, jacobi_sweeps=50, device=dev)
     u0 = torch.from_numpy(ns_cavity_u0(0, n)).to(dev)
     p0 = torch.zeros(cav.shape + (1,), device=dev)
-    cav.run(u0, p0, 3)
+    area = float(cav.dx * cav.dx)
+    out = {"grid": [n, n], "jacobi_sweeps": 50, "dt": cav.dt, "steps": steps}
+    for name, K in (("store_all", None), ("checkpointed", checkpoint_every)):
+        ro = CavityRollout(cav, steps, checkpoint_every=K, use_graph=True)
+        for rep in range(2):                     # first run captures the graphs
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            uN, _ = ro.forward_states(u0, p0)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            gu, _ = ro.backward_states(area * uN * cav.mask, torch.zeros_like(p0))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+        if K is None:
+            out["fwd_ms_per_step"] = round(fwd * 1e3, 3)
+        out[name] = {"checkpoint_every": K, "fwd_plus_dKE_du0_ms_per_step": round((fwd + bwd) * 1e3, 3),
+                     "ratio_to_fwd": round((fwd + bwd) / out["fwd_ms_per_step"] / 1e-3, 2)}
+        del ro
+    # the batch: one cavity's work buffers, seeds (u, p) pairs, one graph
+    us = [torch.from_numpy(ns_cavity_u0(sd, n)).to(dev) for sd in range(seeds)]
+    ps = [torch.zeros(cav.shape + (1,), device=dev) for _ in range(seeds)]
+    for u, p in zip(us, ps):
+        cav.step(u, p, u, p)
+    torch.cuda.synchronize()
+    gr = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gr):
+        for u, p in zip(us, ps):
+            cav.step(u, p, u, p)
+    gr.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    cav.run(u0, p0, steps + 1)
+    for _ in range(steps):
+        gr.replay()
     torch.cuda.synchronize()
-    fwd = (time.perf_counter() - t0) / steps
-    x = u0.clone().requires_grad_(True)
-    t0 = time.perf_counter()
-    J = cav.kinetic_energy_rollout(x, p0, adj_steps, checkpoint_every=2)
-    J.backward()
-    torch.cuda.synchronize()
-    fa = (time.perf_counter() - t0) / adj_steps
-    return {"grid": [n, n], "jacobi_sweeps": 50, "dt": cav.dt,
-            "fwd_ms_per_step": round(fwd * 1e3, 3),
-            "batch_of_16_ms_per_step": round(seeds * fwd * 1e3, 2),
-            "fwd_plus_dKE_du0_ms_per_step": round(fa * 1e3, 3),
-            "note": "forward graph-replayed; gradient eager (autograd over the transposed "
-                    "kernels, checkpoints every 2 steps)"}
+    out["batch_of_16_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    out["note"] = ("graph-replayed forward and transposed steps; gradient = dJ/du0 of the "
+                   "interior kinetic energy after `steps` steps; batch = 16 seeded instances "
+                   "stepped in one graph, measured")
+    return out
 
 
 def main():

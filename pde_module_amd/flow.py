@@ -370,8 +370,15 @@ class CavityRollout:
         self._graphs = {}
 
     def _prepare(self, u0, p0):
+        """Buffers, kept (with the graphs that point at them) while the shapes
+        and device stay the same."""
         cav = self.cav
         cav.prepare_adjoint(u0, p0)
+        key = (tuple(u0.shape), tuple(p0.shape), u0.device)
+        if getattr(self, "_key", None) == key:
+            return
+        self._key = key
+        self._graphs = {}
         E = lambda like, *n: torch.empty(n + tuple(like.shape), dtype=like.dtype,
                                          device=like.device)
         self.U, self.P = E(u0), E(p0)

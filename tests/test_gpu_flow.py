@@ -220,6 +220,13 @@ def test_cavity_step_transpose_matches_autograd(dev):
         assert Jr == J
         assert same_bits(gur, gu) and same_bits(gpr, gp)
     assert J == Ja
+    # one rollout object run twice, graphs replayed on the buffers it kept
+    from pde_module_amd.flow import CavityRollout
+    ro = CavityRollout(cav, steps, checkpoint_every=2, use_graph=True)
+    for _ in range(2):
+        uN, _ = ro.forward_states(u0, p0)
+        g_u, g_p = ro.backward_states(float(cav.dx * cav.dx) * uN * cav.mask, torch.zeros_like(p0))
+        assert same_bits(g_u.cpu().numpy(), gu) and same_bits(g_p.cpu().numpy(), gp)
     mu = ghost_free_mask(cav.shape, 1, (2,))
     mp = ghost_free_mask(cav.shape, 1, (1,))
     scale_u, scale_p = np.abs(gua[mu]).max(), np.abs(gpa[mp]).max()
