@@ -322,9 +322,10 @@ def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=1
     transposed steps (CavityRollout: every step's velocity kept, and
     checkpoints every ``checkpoint_every`` steps with recompute), each timed
     after a warm-up run that captures the graphs.  Batch: ``seeds`` instances
-    (seeds 0..15) stepped by one captured graph, timed per step."""
+    (seeds 0..15) as 16 instances in one captured graph, and as one
+    ProjectionCavityBatch (forward and batched gradient), timed per step."""
     import torch
-    from pde_module_amd.flow import CavityRollout, ProjectionCavity
+    from pde_module_amd.flow import CavityRollout, ProjectionCavity, ProjectionCavityBatch
     from pde_module_amd.workloads import ns_cavity_u0
     cav = ProjectionCavity(n, jacobi_sweeps=50, device=dev)
     u0 = torch.from_numpy(ns_cavity_u0(0, n)).to(dev)
@@ -348,7 +349,7 @@ def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=1
         out[name] = {"checkpoint_every": K, "fwd_plus_dKE_du0_ms_per_step": round((fwd + bwd) * 1e3, 3),
                      "ratio_to_fwd": round((fwd + bwd) / out["fwd_ms_per_step"] / 1e-3, 2)}
         del ro
-    # the batch: one cavity's work buffers, seeds (u, p) pairs, one graph
+    # 16 seeds as 16 separate instances in one graph (one cavity's work buffers)
     us = [torch.from_numpy(ns_cavity_u0(sd, n)).to(dev) for sd in range(seeds)]
     ps = [torch.zeros(cav.shape + (1,), device=dev) for _ in range(seeds)]
     for u, p in zip(us, ps):
@@ -364,10 +365,31 @@ def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=1
     for _ in range(steps):
         gr.replay()
     torch.cuda.synchronize()
-    out["batch_of_16_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    out["16_instances_one_graph_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    del gr, us, ps
+    # the same 16 seeds as one ProjectionCavityBatch (every instance's sweeps in
+    # one launch per sweep), forward and the batched gradient
+    bat = ProjectionCavityBatch(n, seeds, jacobi_sweeps=50, device=dev)
+    ub = torch.from_numpy(np.stack([ns_cavity_u0(sd, n) for sd in range(seeds)])).to(dev)
+    pb = torch.zeros((seeds,) + cav.shape + (1,), device=dev)
+    ro = CavityRollout(bat, steps, checkpoint_every=None, use_graph=True)
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        uN, _ = ro.forward_states(ub, pb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ro.backward_states(area * uN * cav.mask, torch.zeros_like(pb))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    out["batch_of_16_ms_per_step"] = round(fwd * 1e3, 3)
+    out["batch_of_16_fwd_plus_dKE_du0_ms_per_step"] = round((fwd + bwd) * 1e3, 3)
+    out["batch_speedup_vs_16_instances"] = round(out["16_instances_one_graph_ms_per_step"] /
+                                                 out["batch_of_16_ms_per_step"], 2)
     out["note"] = ("graph-replayed forward and transposed steps; gradient = dJ/du0 of the "
-                   "interior kinetic energy after `steps` steps; batch = 16 seeded instances "
-                   "stepped in one graph, measured")
+                   "interior kinetic energy after `steps` steps; batch = 16 seeds "
+                   "(ProjectionCavityBatch, store-all rollout), measured")
     return out
 
 

@@ -495,6 +495,19 @@ int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, float *grhs, i
                                     int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
                                     void *stream);
 
+/* pde_jacobi_neumann0_f32 / pde_jacobi_neumann0_adjoint_f32 over `batch`
+ * instances of the same 1D or 2D grid stored one after another (instance b
+ * at offset b * nx * ny * nz in every array): one launch for all, each
+ * instance the same values as swept alone.  PDE_ESHAPE for 3D grids.  No
+ * reference counterpart; it serves the projection cavity stepped over many
+ * seeds at once. */
+int pde_jacobi_neumann0_batch_f32(const float *p, const float *rhs, float *out, int64_t batch,
+                                  int64_t nx, int64_t ny, int64_t nz, int32_t ghost, float h2,
+                                  float inv, void *stream);
+int pde_jacobi_neumann0_adjoint_batch_f32(const float *gout, float *gp, float *grhs,
+                                          int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                          int32_t ghost, float h2, float inv, void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -74,6 +74,9 @@ SIGNATURES = {
     "pde_jacobi_neumann0_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_adjoint_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_neumann0_adjoint_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
+    "pde_jacobi_neumann0_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],
+    "pde_jacobi_neumann0_adjoint_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F,
+                                              _P],
     "pde_matvec_t_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
     "pde_version": [],
     "pde_status_string": [C.c_int],

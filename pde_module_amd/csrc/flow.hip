@@ -243,6 +243,64 @@ extern "C" int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, flo
     return launch_status();
 }
 
+// Batches of 1D/2D instances stored one after another (stride = the padded
+// node count of one instance): the batch index takes the canonical axis 0,
+// which the folded geometry of such an instance leaves at size 1 and which
+// no stencil term reads, so each instance is swept exactly as alone.
+static int batch_geom(int64_t batch, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                      JacGeom *out, int64_t P[3])
+{
+    if (batch < 1) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0 || G.ndim == 3 || G.P[0] != 1) return PDE_ESHAPE;
+    for (int a = 3 - G.ndim; a < 3; ++a)
+        if (G.E[a] < 2) return PDE_ESHAPE;
+    JacGeom g = jac_geom(G);
+    g.S[0] = nx * ny * nz;
+    g.G[0] = 0;
+    g.E[0] = batch;
+    *out = g;
+    P[0] = batch;
+    P[1] = G.P[1];
+    P[2] = G.P[2];
+    return PDE_OK;
+}
+
+extern "C" int pde_jacobi_neumann0_batch_f32(const float *p, const float *rhs, float *out,
+                                             int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                             int32_t ghost, float h2, float inv, void *stream)
+{
+    if (!p || !rhs || !out || out == p || out == rhs) return PDE_EINVAL;
+    JacGeom g;
+    int64_t P[3];
+    int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
+    if (st) return st;
+    const dim3 grid(clamp_grid((g.E[2] + 255) / 256, 0x7fffffff), clamp_grid(g.E[1], 65535),
+                    clamp_grid(batch, 65535));
+    hipLaunchKernelGGL(jacobi_neumann0_kernel, grid, dim3(256), 0, as_stream(stream), p, rhs, out,
+                       g, h2, inv);
+    return launch_status();
+}
+
+extern "C" int pde_jacobi_neumann0_adjoint_batch_f32(const float *gout, float *gp, float *grhs,
+                                                     int64_t batch, int64_t nx, int64_t ny,
+                                                     int64_t nz, int32_t ghost, float h2,
+                                                     float inv, void *stream)
+{
+    if (!gout || !gp || !grhs || gp == gout || grhs == gout || grhs == gp) return PDE_EINVAL;
+    JacGeom g;
+    int64_t P[3];
+    int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
+    if (st) return st;
+    const dim3 grid(clamp_grid((P[2] + 255) / 256, 0x7fffffff), clamp_grid(P[1], 65535),
+                    clamp_grid(P[0], 65535));
+    hipLaunchKernelGGL(jacobi_neumann0_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout,
+                       gp, grhs, g, P[0], P[1], P[2], h2, inv);
+    return launch_status();
+}
+
 extern "C" int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes,
                                 int32_t rows, int32_t cols, void *stream)
 {

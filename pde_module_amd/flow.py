@@ -167,10 +167,17 @@ class ProjectionCavity:
     def step(self, u, p, u_out, p_out):
         """One projection step from (u, p) into (u_out, p_out); u_out / p_out may
         be u / p.  All work on the current stream, no allocation."""
+        b = self.b
+        self._predict(u, b["us2"], b["rhs"])
+        self._sweeps(p, b["rhs"], p_out, b["pf"], b["p0"], 1)
+        self._correct(p_out, b["us2"], u_out)
+        return u_out, p_out
+
+    def _predict(self, u, us2, rhs):
+        """u -> the boundary-fixed predictor us2 and the pressure source rhs."""
         b, g, s = self.b, self.g, N.stream_of(u)
         grid = self.shape
         self.u_bc._ensure_table(u)
-        self.p_bc._ensure_table(p)
         self.u_bc.table.apply(u, b["u2"], None)
         N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
                np.float32(1.0), s)
@@ -183,30 +190,43 @@ class ProjectionCavity:
                b["F"].data_ptr(), b["F"].numel(), s)
         N.call("pde_forward_euler_f32", b["u2"].data_ptr(), b["F"].data_ptr(), b["us"].data_ptr(),
                b["us"].numel(), np.float32(self.dt), s)
-        self.u_bc.table.apply(b["us"], b["us2"], None)
-        N.call("pde_divergence_f32", b["us2"].data_ptr(), b["rhs"].data_ptr(), *grid, 1, g,
-               *self.wc, np.float32(1.0 / self.dt), s)
-        # Neumann-0 walls folded into each sweep (pde_jacobi_neumann0_f32: same
-        # values as applying p_bc and sweeping); sweeps alternate between p_out
-        # and pf so that no sweep writes what it reads, ending in p_out
-        bufs = (p_out, b["pf"]) if self.sweeps % 2 == 1 else (b["pf"], p_out)
+        self.u_bc.table.apply(b["us"], us2, None)
+        N.call("pde_divergence_f32", us2.data_ptr(), rhs.data_ptr(), *grid, 1, g, *self.wc,
+               np.float32(1.0 / self.dt), s)
+
+    def _sweeps(self, p, rhs, p_out, pf, p0, batch):
+        """The Jacobi sweeps from p into p_out (``batch`` instances stored one
+        after another).  Neumann-0 walls are folded into each sweep
+        (pde_jacobi_neumann0_f32: the same values as applying p_bc, then
+        sweeping); sweeps alternate between p_out and pf so that no sweep
+        writes what it reads, ending in p_out."""
+        s = N.stream_of(p)
+        bufs = (p_out, pf) if self.sweeps % 2 == 1 else (pf, p_out)
         cur = p
         for k in range(self.sweeps):
             dst = bufs[k % 2]
             if dst is cur:                       # p_out is p on the first sweep
-                N.call("pde_copy_f32", cur.data_ptr(), b["p0"].data_ptr(), cur.numel(), s)
-                cur = b["p0"]
-            N.call("pde_jacobi_neumann0_f32", cur.data_ptr(), b["rhs"].data_ptr(),
-                   dst.data_ptr(), *grid, g, self.h2, self.inv, s)
+                N.call("pde_copy_f32", cur.data_ptr(), p0.data_ptr(), cur.numel(), s)
+                cur = p0
+            if batch == 1:
+                N.call("pde_jacobi_neumann0_f32", cur.data_ptr(), rhs.data_ptr(), dst.data_ptr(),
+                       *self.shape, self.g, self.h2, self.inv, s)
+            else:
+                N.call("pde_jacobi_neumann0_batch_f32", cur.data_ptr(), rhs.data_ptr(),
+                       dst.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
             cur = dst
         if cur is not p_out:
             N.call("pde_copy_f32", cur.data_ptr(), p_out.data_ptr(), p_out.numel(), s)
+
+    def _correct(self, p_out, us2, u_out):
+        """u_out = us2 - dt * grad(B_p(p_out))."""
+        b, g, s = self.b, self.g, N.stream_of(p_out)
+        self.p_bc._ensure_table(p_out)
         self.p_bc.table.apply(p_out, b["pf"], None)
-        N.call("pde_grad_f32", b["pf"].data_ptr(), b["gp"].data_ptr(), *grid, 1, g, *self.wc,
-               np.float32(1.0), s)
-        N.call("pde_forward_euler_f32", b["us2"].data_ptr(), b["gp"].data_ptr(), u_out.data_ptr(),
+        N.call("pde_grad_f32", b["pf"].data_ptr(), b["gp"].data_ptr(), *self.shape, 1, g,
+               *self.wc, np.float32(1.0), s)
+        N.call("pde_forward_euler_f32", us2.data_ptr(), b["gp"].data_ptr(), u_out.data_ptr(),
                u_out.numel(), np.float32(-self.dt), s)
-        return u_out, p_out
 
     def run(self, u0, p0, steps, use_graph=True):
         """``steps`` steps from (u0, p0); returns the final (u, p)."""
@@ -251,7 +271,8 @@ class ProjectionCavity:
     def _adjoint_buffers(self):
         if getattr(self, "ga", None) is None:
             z = lambda *e: torch.zeros(self.shape + e, dtype=torch.float32, device=self.device)
-            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("gpa", (1,)),
+            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("ghp", (1,)),
+                                             ("gpa", (1,)),
                                              ("gpb", (1,)), ("grhs", (1,)), ("gdiv", (2,)),
                                              ("gus2", (2,)), ("gus", (2,)), ("gF", (2,)),
                                              ("gconv", (2,)), ("glap", (2,)), ("gdu", (2, 2)),
@@ -272,34 +293,57 @@ class ProjectionCavity:
         from the gradients (gu_out, gp_out) of the step's outputs, the
         gradients (gu, gp) of its inputs (u, p).  gu / gp must not alias
         gu_out / gp_out.  Preallocated buffers only, current stream."""
-        b, a, g, s = self.b, self._adjoint_buffers(), self.g, N.stream_of(u)
+        a = self._adjoint_buffers()
+        self.prepare_adjoint(u, gp)
+        self._adj_correct(gu_out, gp_out, a["gpb"])
+        a["grhs"].zero_()
+        cur = self._adj_sweeps(a["gpb"], a["gpa"], a["grhs"], 1)
+        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), N.stream_of(gp))
+        self._adj_predict(u, gu_out, a["grhs"], gu)
+        return gu, gp
+
+    def _adj_correct(self, gu_out, gp_out, gpb):
+        """Transpose of the correction u_out = us2 - dt * grad(B_p(p_out)) in
+        p_out: gpb = B_p^T(grad^T(-dt * gu_out)) + gp_out."""
+        a, g, s = self.ga, self.g, N.stream_of(gu_out)
+        N.call("pde_scale_f32", gu_out.data_ptr(), a["ggp"].data_ptr(), gu_out.numel(),
+               np.float32(-self.dt), s)
+        N.call("pde_grad_adjoint_f32", a["ggp"].data_ptr(), a["gpf"].data_ptr(), *self.shape, 1,
+               g, *self.wc, np.float32(1.0), s)
+        self.p_bc.table.adjoint(a["gpf"], a["ghp"])
+        N.call("pde_lincomb_f32", a["ghp"].data_ptr(), gp_out.data_ptr(), None, None,
+               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
+               gpb.data_ptr(), gpb.numel(), s)
+
+    def _adj_sweeps(self, g, spare, grhs, batch):
+        """The sweeps p <- J(B_p(p), rhs), transposed last to first, from g
+        (``batch`` instances one after another); rhs's gradient accumulates in
+        grhs.  Returns the buffer (g or spare) holding p's gradient."""
+        s = N.stream_of(g)
+        cur, nxt = g, spare
+        for _ in range(self.sweeps):
+            if batch == 1:
+                N.call("pde_jacobi_neumann0_adjoint_f32", cur.data_ptr(), nxt.data_ptr(),
+                       grhs.data_ptr(), *self.shape, self.g, self.h2, self.inv, s)
+            else:
+                N.call("pde_jacobi_neumann0_adjoint_batch_f32", cur.data_ptr(), nxt.data_ptr(),
+                       grhs.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
+            cur, nxt = nxt, cur
+        return cur
+
+    def _adj_predict(self, u, gu_out, grhs, gu):
+        """Transpose of the predictor and the correction in us2: from rhs's
+        gradient grhs and gu_out, u's gradient gu (u is the step's input)."""
+        b, a, g, s = self.b, self.ga, self.g, N.stream_of(u)
         grid = self.shape
         nodes = int(np.prod(grid))
-        self.prepare_adjoint(u, gp)
-        # the two forward values the transpose reads: u2 = B_u(u), grad(u2)
+        # the two forward values this transpose reads: u2 = B_u(u), grad(u2)
         self.u_bc.table.apply(u, b["u2"], None)
         N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
                np.float32(1.0), s)
-        # u_out = us2 - dt * grad(B_p(p_out))
-        N.call("pde_scale_f32", gu_out.data_ptr(), a["ggp"].data_ptr(), gu_out.numel(),
-               np.float32(-self.dt), s)
-        N.call("pde_grad_adjoint_f32", a["ggp"].data_ptr(), a["gpf"].data_ptr(), *grid, 1, g,
-               *self.wc, np.float32(1.0), s)
-        self.p_bc.table.adjoint(a["gpf"], a["gpa"])
-        N.call("pde_lincomb_f32", a["gpa"].data_ptr(), gp_out.data_ptr(), None, None,
-               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
-               a["gpb"].data_ptr(), a["gpb"].numel(), s)
-        # the sweeps p <- J(B_p(p), rhs), last to first; rhs's gradient accumulates
-        a["grhs"].zero_()
-        cur, nxt = a["gpb"], a["gpa"]
-        for _ in range(self.sweeps):
-            N.call("pde_jacobi_neumann0_adjoint_f32", cur.data_ptr(), nxt.data_ptr(),
-                   a["grhs"].data_ptr(), *grid, g, self.h2, self.inv, s)
-            cur, nxt = nxt, cur
-        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), s)
-        # rhs = div(us2) / dt; us2 = B_u(us)
-        N.call("pde_divergence_adjoint_f32", a["grhs"].data_ptr(), a["gdiv"].data_ptr(), *grid, 1,
-               g, *self.wc, np.float32(1.0 / self.dt), s)
+        # rhs = div(us2) / dt; u_out = us2 - ...; us2 = B_u(us)
+        N.call("pde_divergence_adjoint_f32", grhs.data_ptr(), a["gdiv"].data_ptr(), *grid, 1, g,
+               *self.wc, np.float32(1.0 / self.dt), s)
         N.call("pde_lincomb_f32", gu_out.data_ptr(), a["gdiv"].data_ptr(), None, None,
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
                a["gus2"].data_ptr(), a["gus2"].numel(), s)
@@ -322,7 +366,6 @@ class ProjectionCavity:
                np.float32(1.0), 4, a["gu2"].data_ptr(), a["gu2"].numel(), s)
         # u2 = B_u(u)
         self.u_bc.table.adjoint(a["gu2"], gu)
-        return gu, gp
 
     def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
         """J(u_N) = 1/2 sum |u_N|^2 dx^2 (interior), differentiable in u0 and
@@ -355,8 +398,9 @@ class ProjectionCavity:
 class CavityRollout:
     """``steps`` projection steps and their transpose on preallocated buffers.
 
-    ``checkpoint_every=None`` keeps the input velocity of every step (steps
-    fields; the transpose needs nothing else); K > 0 keeps (u, p) every K steps
+    ``cav`` is a ProjectionCavity or a ProjectionCavityBatch (fields with a
+    leading batch axis).  ``checkpoint_every=None`` keeps the input velocity
+    of every step (steps fields; the transpose needs nothing else); K > 0 keeps (u, p) every K steps
     and recomputes a segment's velocities before its transposed steps.  With
     ``use_graph`` the forward step and the transposed step are each captured
     once in a HIP graph (on fixed buffers) and replayed for every step."""
@@ -463,3 +507,85 @@ class _CavityRolloutFn(torch.autograd.Function):
         gpN = torch.zeros_like(ro.P) if gpN is None else gpN.contiguous()
         gu0, gp0 = ctx.ro.backward_states(guN, gpN)
         return gu0, gp0, None
+
+
+class ProjectionCavityBatch:
+    """``batch`` cavities of one grid and one set of parameters stepped
+    together (one per seed): u (batch, X, Y, 1, 2) and p (batch, X, Y, 1, 1)
+    with the instances one after another.  The Jacobi sweeps of every instance
+    run as one launch per sweep (pde_jacobi_neumann0_batch_f32); the predictor
+    and the correction run per instance on shared work buffers.  Each
+    instance gets the same bits as ``ProjectionCavity.step`` alone."""
+
+    def __init__(self, n, batch, **kw):
+        self.cav = ProjectionCavity(n, **kw)
+        self.batch = int(batch)
+        if self.batch < 1:
+            raise ValueError("batch must be >= 1")
+        z = lambda *e: torch.zeros((self.batch,) + self.cav.shape + e, dtype=torch.float32,
+                                   device=self.cav.device)
+        self.us2, self.rhs, self.pf, self.p0 = z(2), z(1), z(1), z(1)
+
+    def step(self, u, p, u_out, p_out):
+        """One step of every instance from (u, p) into (u_out, p_out) (which
+        may be u / p)."""
+        cav = self.cav
+        for t in (u, p, u_out, p_out):
+            N.check_field(t, "field")
+        if u.shape[0] != self.batch or tuple(u.shape[1:4]) != cav.shape:
+            raise AssertionError("u must be (batch, X, Y, 1, 2) of the cavity's grid")
+        for i in range(self.batch):
+            cav._predict(u[i], self.us2[i], self.rhs[i])
+        cav._sweeps(p, self.rhs, p_out, self.pf, self.p0, self.batch)
+        for i in range(self.batch):
+            cav._correct(p_out[i], self.us2[i], u_out[i])
+        return u_out, p_out
+
+    def prepare_adjoint(self, like_u, like_p):
+        self.cav.prepare_adjoint(like_u[0], like_p[0])
+        if getattr(self, "gpb", None) is None:
+            z = lambda *e: torch.zeros((self.batch,) + self.cav.shape + e, dtype=torch.float32,
+                                       device=self.cav.device)
+            self.gpb, self.gpa, self.grhs = z(1), z(1), z(1)
+
+    def step_adjoint(self, u, gu_out, gp_out, gu, gp):
+        """Transpose of ``step`` for every instance (the instances' transposed
+        sweeps as one launch per sweep); same arguments as
+        ``ProjectionCavity.step_adjoint`` with a leading batch axis."""
+        cav = self.cav
+        self.prepare_adjoint(u, gp)
+        for i in range(self.batch):
+            cav._adj_correct(gu_out[i], gp_out[i], self.gpb[i])
+        self.grhs.zero_()
+        cur = cav._adj_sweeps(self.gpb, self.gpa, self.grhs, self.batch)
+        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), N.stream_of(gp))
+        for i in range(self.batch):
+            cav._adj_predict(u[i], gu_out[i], self.grhs[i], gu[i])
+        return gu, gp
+
+    def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
+        """Sum over the instances of J_i = 1/2 sum |u_i,N|^2 dx^2 (interior),
+        differentiable in u0 and p0 (each instance's gradient is its own)."""
+        ro = CavityRollout(self, steps, checkpoint_every, use_graph)
+        uN, _ = _CavityRolloutFn.apply(u0, p0, ro)
+        return _KineticEnergy.apply(uN, self.cav.mask, float(self.cav.dx * self.cav.dx))
+
+    def run(self, u0, p0, steps, use_graph=True):
+        """``steps`` steps of every instance; returns the final (u, p)."""
+        u, p = u0.clone(), p0.clone()
+        if steps <= 0:
+            return u, p
+        self.step(u, p, u, p)
+        if steps == 1:
+            return u, p
+        if not use_graph:
+            for _ in range(steps - 1):
+                self.step(u, p, u, p)
+            return u, p
+        torch.cuda.synchronize(self.cav.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.step(u, p, u, p)
+        for _ in range(steps - 1):
+            graph.replay()
+        return u, p

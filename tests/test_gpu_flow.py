@@ -233,3 +233,35 @@ def test_cavity_step_transpose_matches_autograd(dev):
     print(f"|dJ/du0| {scale_u:.3e}  |dJ/dp0| {scale_p:.3e}", flush=True)
     np.testing.assert_allclose(gu, gua, rtol=0, atol=1e-5 * scale_u)
     np.testing.assert_allclose(gp, gpa, rtol=0, atol=1e-5 * max(scale_p, 1e-30))
+
+
+@pytest.mark.parametrize("sweeps", [12, 13])
+def test_cavity_batch_matches_single_instances(dev, sweeps):
+    """ProjectionCavityBatch (sweeps of every instance in one launch) against
+    each instance stepped alone: the same bits, forward (eager and graph) and
+    dJ/du0, dJ/dp0 of the summed kinetic energy."""
+    from pde_module_amd.flow import ProjectionCavityBatch
+    print(f"cavity batch sweeps={sweeps}", flush=True)
+    n, B, steps = 18, 3, 4
+    kw = dict(jacobi_sweeps=sweeps, dt=2e-3, device=dev)
+    bat = ProjectionCavityBatch(n, B, **kw)
+    one = ProjectionCavity(n, **kw)
+    rng = np.random.default_rng(17)
+    u0 = torch.from_numpy(np.stack([ns_cavity_u0(s, n) * 50.0 for s in range(B)])).to(dev)
+    p0 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, (B,) + one.shape + (1,)).astype(np.float32)).to(dev)
+    ue, pe = bat.run(u0, p0, steps, use_graph=False)
+    ug, pg = bat.run(u0, p0, steps, use_graph=True)
+    for i in range(B):
+        ui, pi = one.run(u0[i], p0[i], steps, use_graph=False)
+        for got in ((ue, pe), (ug, pg)):
+            assert same_bits(got[0][i].cpu().numpy(), ui.cpu().numpy())
+            assert same_bits(got[1][i].cpu().numpy(), pi.cpu().numpy())
+    x, q = u0.clone().requires_grad_(True), p0.clone().requires_grad_(True)
+    J = bat.kinetic_energy_rollout(x, q, steps, checkpoint_every=3, use_graph=True)
+    gx, gq = torch.autograd.grad(J, (x, q))
+    for i in range(B):
+        xi, qi = u0[i].clone().requires_grad_(True), p0[i].clone().requires_grad_(True)
+        Ji = one.kinetic_energy_rollout(xi, qi, steps, checkpoint_every=None)
+        gxi, gqi = torch.autograd.grad(Ji, (xi, qi))
+        assert same_bits(gx[i].cpu().numpy(), gxi.cpu().numpy())
+        assert same_bits(gq[i].cpu().numpy(), gqi.cpu().numpy())
