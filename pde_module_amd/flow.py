@@ -164,9 +164,20 @@ class ProjectionCavity:
         self.mask = torch.from_numpy(m).to(self.device)
 
     # --------------------------------------------------------------- forward
+    def check_fields(self, vel=(), pres=(), lead=()):
+        """Velocity fields must be ``lead + (X, Y, 1, 2)`` and pressure fields
+        ``lead + (X, Y, 1, 1)`` of this grid (the kernels take their extents
+        from the grid, not from the tensors), contiguous FP32 on the GPU."""
+        for t, c in [(t, 2) for t in vel] + [(t, 1) for t in pres]:
+            N.check_field(t, "field")
+            want = tuple(lead) + self.shape + (c,)
+            if tuple(t.shape) != want:
+                raise AssertionError(f"field of shape {tuple(t.shape)}; expected {want}")
+
     def step(self, u, p, u_out, p_out):
         """One projection step from (u, p) into (u_out, p_out); u_out / p_out may
         be u / p.  All work on the current stream, no allocation."""
+        self.check_fields((u, u_out), (p, p_out))
         b = self.b
         self._predict(u, b["us2"], b["rhs"])
         self._sweeps(p, b["rhs"], p_out, b["pf"], b["p0"], 1)
@@ -293,6 +304,7 @@ class ProjectionCavity:
         from the gradients (gu_out, gp_out) of the step's outputs, the
         gradients (gu, gp) of its inputs (u, p).  gu / gp must not alias
         gu_out / gp_out.  Preallocated buffers only, current stream."""
+        self.check_fields((u, gu_out, gu), (gp_out, gp))
         a = self._adjoint_buffers()
         self.prepare_adjoint(u, gp)
         self._adj_correct(gu_out, gp_out, a["gpb"])
@@ -400,10 +412,12 @@ class CavityRollout:
 
     ``cav`` is a ProjectionCavity or a ProjectionCavityBatch (fields with a
     leading batch axis).  ``checkpoint_every=None`` keeps the input velocity
-    of every step (steps fields; the transpose needs nothing else); K > 0 keeps (u, p) every K steps
-    and recomputes a segment's velocities before its transposed steps.  With
-    ``use_graph`` the forward step and the transposed step are each captured
-    once in a HIP graph (on fixed buffers) and replayed for every step."""
+    of every step (steps fields; the transpose needs nothing else); K > 0
+    keeps (u, p) every K steps and recomputes a segment's velocities before
+    its transposed steps.  With ``use_graph`` the forward step and the
+    transposed step are each captured once in a HIP graph (on fixed buffers)
+    and replayed for every step.  One object holds the states of its last
+    ``forward_states`` call: ``backward_states`` transposes that rollout."""
 
     def __init__(self, cav, steps, checkpoint_every=None, use_graph=False):
         self.cav = cav
@@ -475,6 +489,13 @@ class CavityRollout:
         return self.U.clone(), self.P.clone()
 
     def backward_states(self, guN, gpN):
+        if getattr(self, "_key", None) is None:
+            raise RuntimeError("backward_states needs a forward_states call first")
+        for g, ref in ((guN, self.U), (gpN, self.P)):
+            N.check_field(g, "gradient")
+            if g.shape != ref.shape:
+                raise AssertionError(f"gradient of shape {tuple(g.shape)}; the rollout's "
+                                     f"state is {tuple(ref.shape)}")
         self.gU.copy_(guN)
         self.gP.copy_(gpN)
         nseg = (self.steps + self.K - 1) // self.K
@@ -530,10 +551,7 @@ class ProjectionCavityBatch:
         """One step of every instance from (u, p) into (u_out, p_out) (which
         may be u / p)."""
         cav = self.cav
-        for t in (u, p, u_out, p_out):
-            N.check_field(t, "field")
-        if u.shape[0] != self.batch or tuple(u.shape[1:4]) != cav.shape:
-            raise AssertionError("u must be (batch, X, Y, 1, 2) of the cavity's grid")
+        cav.check_fields((u, u_out), (p, p_out), (self.batch,))
         for i in range(self.batch):
             cav._predict(u[i], self.us2[i], self.rhs[i])
         cav._sweeps(p, self.rhs, p_out, self.pf, self.p0, self.batch)
@@ -553,6 +571,7 @@ class ProjectionCavityBatch:
         sweeps as one launch per sweep); same arguments as
         ``ProjectionCavity.step_adjoint`` with a leading batch axis."""
         cav = self.cav
+        cav.check_fields((u, gu_out, gu), (gp_out, gp), (self.batch,))
         self.prepare_adjoint(u, gp)
         for i in range(self.batch):
             cav._adj_correct(gu_out[i], gp_out[i], self.gpb[i])

@@ -265,3 +265,27 @@ def test_cavity_batch_matches_single_instances(dev, sweeps):
         gxi, gqi = torch.autograd.grad(Ji, (xi, qi))
         assert same_bits(gx[i].cpu().numpy(), gxi.cpu().numpy())
         assert same_bits(gq[i].cpu().numpy(), gqi.cpu().numpy())
+
+
+def test_cavity_rejects_fields_of_another_shape(dev):
+    """The kernels take their extents from the cavity's grid: a field of any
+    other shape raises on the host before a launch."""
+    from pde_module_amd.flow import CavityRollout, ProjectionCavityBatch
+    cav = ProjectionCavity(10, jacobi_sweeps=4, dt=1e-3, device=dev)
+    u = torch.zeros(cav.shape + (2,), device=dev)
+    p = torch.zeros(cav.shape + (1,), device=dev)
+    with pytest.raises(AssertionError):
+        cav.step(u, torch.zeros(cav.shape + (2,), device=dev), u, p)
+    with pytest.raises(AssertionError):
+        cav.step(u[:-1], p[:-1], u[:-1], p[:-1])
+    with pytest.raises(AssertionError):
+        cav.step_adjoint(u, u, p, u, torch.zeros(8, 12, 1, 1, device=dev))
+    bat = ProjectionCavityBatch(10, 2, jacobi_sweeps=4, dt=1e-3, device=dev)
+    with pytest.raises(AssertionError):
+        bat.step(u, p, u, p)                      # no batch axis
+    ro = CavityRollout(cav, 2)
+    with pytest.raises(RuntimeError):
+        ro.backward_states(u, p)                  # no forward yet
+    ro.forward_states(u, p)
+    with pytest.raises(AssertionError):
+        ro.backward_states(p, p)                  # would broadcast
