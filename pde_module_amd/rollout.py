@@ -187,6 +187,16 @@ class WaveRollout:
         # only by the generic path)
         self._ctr = torch.zeros(2, dtype=torch.int64, device=dev)
         self._fused = sum(1 for s_ in shape[:3] if s_ > 1) == 2 and shape[3] == 1
+        numel = int(np.prod(shape))
+        for name, idx in (("src_idx", self.src_idx), ("rec_idx", self.rec_idx)):
+            if idx is not None and (idx.dtype != torch.int64 or idx.device != dev
+                                    or not idx.is_contiguous() or idx.dim() != 1):
+                raise TypeError(f"{name} must be a contiguous 1-D int64 tensor on {dev}")
+            if idx is not None and idx.numel():
+                lo, hi = int(idx.min()), int(idx.max())
+                if lo < 0 or hi >= numel:
+                    raise IndexError(f"{name} holds {lo}..{hi}, outside a field of {numel} "
+                                     "values")
         if self._fused:
             self._build_point_sets(shape)
         self._graphs = {}
@@ -421,6 +431,9 @@ class _WaveFn(torch.autograd.Function):
         N.check_field(v0, "v0")
         if c2 is not None:
             N.check_field(c2, "c2")
+        for name, t in (("v0", v0), ("c2", c2)):
+            if t is not None and t.shape != u0.shape:
+                raise AssertionError(f"{name} has shape {tuple(t.shape)}, u0 {tuple(u0.shape)}")
         c2d = c2.detach() if c2 is not None else None
         uN, vN, rec = ro.forward_states(c2d, u0.detach(), v0.detach())
         ctx.ro, ctx.c2 = ro, c2d

@@ -181,3 +181,35 @@ def test_wave_rollout_graph_replay_matches_eager(dev):
         for call in range(2):
             for a, b in zip(base[call], outs[key][call]):
                 assert torch.equal(a.view(torch.int32), b.view(torch.int32)), key
+
+
+def test_wave_rollout_rejects_bad_points_and_shapes(dev):
+    """Point indices outside the field, non-int64 index tensors and fields of
+    another shape raise on the host before any launch (2D fused path and the
+    generic 3D path)."""
+    shape, dx, dt, c2, u0, v0, src, wav, rec = _wave_setup(dev, n=12, steps=4)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    bc = LegacyGridBoundary(T(u0), dx, 1)
+    numel = int(np.prod(u0.shape))
+    for bad_rec, err in ((np.array([numel], np.int64), IndexError),
+                         (np.array([-1], np.int64), IndexError),
+                         (rec.astype(np.int32), TypeError)):
+        ro = WaveRollout(bc, dx, 1, float(dt), 4, checkpoint_every=2, src_idx=T(src),
+                         src_wavelet=T(wav), rec_idx=T(bad_rec))
+        with pytest.raises(err):
+            ro(T(c2), T(u0), T(v0))
+    ro = WaveRollout(bc, dx, 1, float(dt), 4, checkpoint_every=2, src_idx=T(src),
+                     src_wavelet=T(wav), rec_idx=T(rec))
+    with pytest.raises(AssertionError):
+        ro(T(c2), T(u0), T(v0[:-1]))
+    with pytest.raises(AssertionError):
+        ro(T(c2[:-1]), T(u0), T(v0))
+    # generic path (3D field): out-of-range source
+    shape3 = (6, 7, 8)
+    z = np.zeros(shape3 + (1,), np.float32)
+    bc3 = LegacyGridBoundary(T(z), 0.1, 1)
+    ro3 = WaveRollout(bc3, 0.1, 1, 0.01, 3, checkpoint_every=None,
+                      src_idx=T(np.array([z.size + 5], np.int64)),
+                      src_wavelet=T(np.ones((3, 1), np.float32)))
+    with pytest.raises(IndexError):
+        ro3(T(z), T(z), T(z))
