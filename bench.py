@@ -69,6 +69,21 @@ def pmc_traffic():
         return None
 
 
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and the cores this process may
+    run on (sched_getaffinity)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "affinity_cores": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(seconds):
     """The C FP32 restatement (single thread, the serial reference-equivalent
     CPU path) on slabs of the 512^3 workload: 512 x 512 x 512 planes of the
@@ -534,7 +549,8 @@ def main():
                                "sample": "C FP32 restatement of the reference Laplacian "
                                          "(oracle/pde_oracle.c, -O2 -ffp-contract=off), "
                                          "32-plane slabs of the 514^3 grid, single thread",
-                               "multicore": {"value": round(cpu[threads], 4), "cores": threads}}
+                               "multicore": {"value": round(cpu[threads], 4), "cores": threads},
+                               "host_cpu": host_cpu_model()}
     print(json.dumps(rec), flush=True)
 
 
