@@ -7,7 +7,7 @@
 //   out[n] = inv * (acc - h2 * rhs[n]),   inv = 1 / (2 * ndim), h2 = dx^2,
 // one Jacobi iteration of (sum of neighbours - 2 ndim p) / dx^2 = rhs.  Ghosts
 // are read (apply the pressure boundary first) and never written.
-#include "pde_common.hpp"
+#include "tile2d.hpp"
 
 namespace pde {
 
@@ -150,6 +150,174 @@ __global__ __launch_bounds__(256) void jacobi_neumann0_adjoint_kernel(
         }
 }
 
+// 2D grids (and batches of them, one instance per blockIdx.y): the folded
+// sweep and its transpose as row tiles.  A lane owns 4 consecutive columns
+// (16-byte accesses) of JR rows; a block stacks JBY waves along the rows and
+// covers a 256-column strip; the rows above and below come from the lane's
+// own loads, the columns beside its 4 from its neighbours by shuffles (the
+// strip's edge lanes load theirs).  Tile order as in the fused 2D steps
+// (vertically adjacent tiles on one XCD).  Same arithmetic, in the same
+// order, as jacobi_neumann0_kernel / jacobi_neumann0_adjoint_kernel.
+constexpr int JR = 2;
+constexpr int JBY = 4;
+
+struct JTile {
+    Tile2 t;
+    int64_t stride;   // floats per instance
+};
+
+__device__ __forceinline__ uint32_t jtile_index()
+{
+    const uint32_t per = gridDim.x >> 3;
+    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+__global__ __launch_bounds__(64 * JBY) void jacobi_n0_rows_kernel(const float *__restrict__ p,
+                                                                  const float *__restrict__ rhs,
+                                                                  float *__restrict__ out,
+                                                                  JTile J, float h2, float inv)
+{
+    const Tile2 &t = J.t;
+    const uint32_t L = jtile_index();
+    if (L >= t.tq * t.tr) return;
+    const int64_t off = (int64_t)blockIdx.y * J.stride;
+    p += off;
+    rhs += off;
+    out += off;
+    const uint32_t qt = L / t.tr, rt = L % t.tr;
+    const int lane = threadIdx.x;
+    const int64_t q = (int64_t)qt * 64 + lane;
+    const bool lane_ok = q < t.nq;
+    const int64_t k0 = (lane_ok ? q : t.nq - 1) * 4;
+    const int64_t x0 = ((int64_t)rt * JBY + threadIdx.y) * JR;
+    const int64_t lo1 = t.g, hi1 = t.P1 - t.g, lo2 = t.g, hi2 = t.P2 - t.g;
+    v4 rows[JR + 2];
+#pragma unroll
+    for (int r = 0; r < JR + 2; ++r) rows[r] = ld_row4(p, t, x0 - 1 + r, k0);
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+        const int64_t x = x0 + r;
+        float left, right;
+        col_edges(p, t, x, k0, rows[r + 1], lane, 64, left, right);
+        if (!lane_ok || x < lo1 || x >= hi1) continue;
+        const v4 rh = ld_row4(rhs, t, x, k0);
+        v4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t k = k0 + i;
+            const float up = el(rows[r], i), dn = el(rows[r + 2], i);
+            const float zl = i == 0 ? left : el(rows[r + 1], i > 0 ? i - 1 : 0);
+            const float zr = i == 3 ? right : el(rows[r + 1], i < 3 ? i + 1 : 3);
+            const float a_lo = x - 1 < lo1 ? -0.0f + dn : up;
+            const float a_hi = x + 1 >= hi1 ? 0.0f + up : dn;
+            const float b_lo = k - 1 < lo2 ? -0.0f + zr : zl;
+            const float b_hi = k + 1 >= hi2 ? 0.0f + zl : zr;
+            float acc = 0.0f;
+            acc += a_lo;
+            acc += a_hi;
+            acc += b_lo;
+            acc += b_hi;
+            set_el(o, i, inv * (acc - h2 * el(rh, i)));
+        }
+        if (k0 >= lo2 && k0 + 4 <= hi2) {
+            st_row4(out, t, x, k0, o);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (k0 + i >= lo2 && k0 + i < hi2) out[x * t.P2 + k0 + i] = el(o, i);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * JBY) void jacobi_n0_adj_rows_kernel(
+    const float *__restrict__ g, float *__restrict__ gp, float *__restrict__ grhs, JTile J,
+    float h2, float inv)
+{
+    const Tile2 &t = J.t;
+    const uint32_t L = jtile_index();
+    if (L >= t.tq * t.tr) return;
+    const int64_t off = (int64_t)blockIdx.y * J.stride;
+    g += off;
+    gp += off;
+    grhs += off;
+    const uint32_t qt = L / t.tr, rt = L % t.tr;
+    const int lane = threadIdx.x;
+    const int64_t q = (int64_t)qt * 64 + lane;
+    const bool lane_ok = q < t.nq;
+    const int64_t k0 = (lane_ok ? q : t.nq - 1) * 4;
+    const int64_t x0 = ((int64_t)rt * JBY + threadIdx.y) * JR;
+    const int64_t lo1 = t.g, hi1 = t.P1 - t.g, lo2 = t.g, hi2 = t.P2 - t.g;
+    v4 rows[JR + 2];
+#pragma unroll
+    for (int r = 0; r < JR + 2; ++r) rows[r] = ld_row4(g, t, x0 - 1 + r, k0);
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+        const int64_t x = x0 + r;
+        float left, right;
+        col_edges(g, t, x, k0, rows[r + 1], lane, 64, left, right);
+        if (!lane_ok || x >= t.P1) continue;
+        const bool xin = x >= lo1 && x < hi1;
+        v4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+        v4 gr = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (xin) gr = ld_row4(grhs, t, x, k0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t k = k0 + i;
+            if (!xin || k < lo2 || k >= hi2) continue;
+            const float up = el(rows[r], i), mid = el(rows[r + 1], i), dn = el(rows[r + 2], i);
+            const float zl = i == 0 ? left : el(rows[r + 1], i > 0 ? i - 1 : 0);
+            const float zr = i == 3 ? right : el(rows[r + 1], i < 3 ? i + 1 : 3);
+            float acc = 0.0f;
+            if (x + 1 < hi1) acc += dn;
+            if (x - 1 >= lo1) acc += up;
+            if (k + 1 < hi2) acc += zr;
+            if (k - 1 >= lo2) acc += zl;
+            float v = inv * acc;
+            if (x - 1 >= lo1 && x - 2 < lo1) v = v + inv * up;
+            if (x + 1 < hi1 && x + 2 >= hi1) v = v + inv * dn;
+            if (k - 1 >= lo2 && k - 2 < lo2) v = v + inv * zl;
+            if (k + 1 < hi2 && k + 2 >= hi2) v = v + inv * zr;
+            set_el(o, i, v);
+            set_el(gr, i, el(gr, i) + -h2 * (inv * mid));
+        }
+        st_row4(gp, t, x, k0, o);
+        if (xin) {
+            if (k0 >= lo2 && k0 + 4 <= hi2) {
+                st_row4(grhs, t, x, k0, gr);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (k0 + i >= lo2 && k0 + i < hi2) grhs[x * t.P2 + k0 + i] = el(gr, i);
+            }
+        }
+    }
+}
+
+// Launch of either row-tile kernel over `batch` instances of a 2D grid
+// (padded extents P1 x P2); PDE_ESHAPE beyond the grid limits.
+template <bool ADJ>
+static int jacobi_rows_launch(const float *a, float *b, float *c, const float *rhs,
+                              int64_t batch, int64_t P1, int64_t P2, int64_t ghost, float h2,
+                              float inv, hipStream_t s)
+{
+    JTile J;
+    J.t.P1 = P1;
+    J.t.P2 = P2;
+    J.t.g = ghost;
+    J.t.nq = (P2 + 3) / 4;
+    J.t.tq = (uint32_t)((J.t.nq + 63) / 64);
+    J.t.tr = (uint32_t)((P1 + JBY * JR - 1) / (JBY * JR));
+    J.stride = P1 * P2;
+    const int64_t nb = ((int64_t)J.t.tq * J.t.tr + 7) / 8 * 8;
+    if (nb > 0x7fffffff || batch > 65535) return PDE_ESHAPE;
+    const dim3 grid((unsigned)nb, (unsigned)batch), block(64, JBY);
+    if (ADJ)
+        hipLaunchKernelGGL(jacobi_n0_adj_rows_kernel, grid, block, 0, s, a, b, c, J, h2, inv);
+    else
+        hipLaunchKernelGGL(jacobi_n0_rows_kernel, grid, block, 0, s, a, rhs, b, J, h2, inv);
+    return launch_status();
+}
+
 // out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
 __global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
                                                        const float *__restrict__ gv,
@@ -199,6 +367,9 @@ extern "C" int pde_jacobi_neumann0_f32(const float *p, const float *rhs, float *
     if (G.ndim == 0) return PDE_ESHAPE;
     for (int a = 3 - G.ndim; a < 3; ++a)
         if (G.E[a] < 2) return PDE_ESHAPE;    // a wall node's mirror must be interior
+    if (G.ndim == 2)
+        return jacobi_rows_launch<false>(p, out, nullptr, rhs, 1, G.P[1], G.P[2], ghost, h2, inv,
+                                         as_stream(stream));
     const JacGeom g = jac_geom(G);
     const dim3 grid(clamp_grid((G.E[2] + 255) / 256, 0x7fffffff), clamp_grid(G.E[1], 65535),
                     clamp_grid(G.E[0], 65535));
@@ -235,6 +406,9 @@ extern "C" int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, flo
     if (G.ndim == 0) return PDE_ESHAPE;
     for (int a = 3 - G.ndim; a < 3; ++a)
         if (G.E[a] < 2) return PDE_ESHAPE;
+    if (G.ndim == 2)
+        return jacobi_rows_launch<true>(gout, gp, grhs, nullptr, 1, G.P[1], G.P[2], ghost, h2, inv,
+                                        as_stream(stream));
     const JacGeom g = jac_geom(G);
     const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535),
                     clamp_grid(G.P[0], 65535));
@@ -250,6 +424,7 @@ extern "C" int pde_jacobi_neumann0_adjoint_f32(const float *gout, float *gp, flo
 static int batch_geom(int64_t batch, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
                       JacGeom *out, int64_t P[3])
 {
+    if (batch > 65535) return PDE_ESHAPE;
     if (batch < 1) return PDE_EINVAL;
     Geom G;
     int st = make_geom(nx, ny, nz, ghost, &G);
@@ -277,6 +452,9 @@ extern "C" int pde_jacobi_neumann0_batch_f32(const float *p, const float *rhs, f
     int64_t P[3];
     int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
     if (st) return st;
+    if (g.ndim == 2)
+        return jacobi_rows_launch<false>(p, out, nullptr, rhs, batch, P[1], P[2], ghost, h2, inv,
+                                         as_stream(stream));
     const dim3 grid(clamp_grid((g.E[2] + 255) / 256, 0x7fffffff), clamp_grid(g.E[1], 65535),
                     clamp_grid(batch, 65535));
     hipLaunchKernelGGL(jacobi_neumann0_kernel, grid, dim3(256), 0, as_stream(stream), p, rhs, out,
@@ -294,6 +472,9 @@ extern "C" int pde_jacobi_neumann0_adjoint_batch_f32(const float *gout, float *g
     int64_t P[3];
     int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
     if (st) return st;
+    if (g.ndim == 2)
+        return jacobi_rows_launch<true>(gout, gp, grhs, nullptr, batch, P[1], P[2], ghost, h2, inv,
+                                        as_stream(stream));
     const dim3 grid(clamp_grid((P[2] + 255) / 256, 0x7fffffff), clamp_grid(P[1], 65535),
                     clamp_grid(P[0], 65535));
     hipLaunchKernelGGL(jacobi_neumann0_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout,

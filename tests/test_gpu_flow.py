@@ -289,3 +289,99 @@ def test_cavity_rejects_fields_of_another_shape(dev):
     ro.forward_states(u, p)
     with pytest.raises(AssertionError):
         ro.backward_states(p, p)                  # would broadcast
+
+
+def _folded_sweep_np(p, rhs, g, h2, inv):
+    """FP32 restatement of the folded Neumann-0 sweep on a 2D (X, Y) grid of
+    ghost width g, in the kernels' operation order (rows, then columns; low
+    neighbour, then high)."""
+    X, Y = p.shape
+    out = np.zeros_like(p)
+    f = np.float32
+    for x in range(g, X - g):
+        for k in range(g, Y - g):
+            up, dn = p[x - 1, k], p[x + 1, k]
+            zl, zr = p[x, k - 1], p[x, k + 1]
+            a_lo = f(-0.0) + dn if x - 1 < g else up
+            a_hi = f(0.0) + up if x + 1 >= X - g else dn
+            b_lo = f(-0.0) + zr if k - 1 < g else zl
+            b_hi = f(0.0) + zl if k + 1 >= Y - g else zr
+            acc = f(0.0)
+            for v in (a_lo, a_hi, b_lo, b_hi):
+                acc = f(acc + v)
+            out[x, k] = f(inv * f(acc - f(h2 * rhs[x, k])))
+    return out
+
+
+def _folded_sweep_adjoint_np(gout, grhs, g, h2, inv):
+    X, Y = gout.shape
+    gp = np.zeros_like(gout)
+    grhs = grhs.copy()
+    f = np.float32
+    for x in range(g, X - g):
+        for k in range(g, Y - g):
+            acc = f(0.0)
+            if x + 1 < X - g:
+                acc = f(acc + gout[x + 1, k])
+            if x - 1 >= g:
+                acc = f(acc + gout[x - 1, k])
+            if k + 1 < Y - g:
+                acc = f(acc + gout[x, k + 1])
+            if k - 1 >= g:
+                acc = f(acc + gout[x, k - 1])
+            v = f(inv * acc)
+            if x - 1 >= g and x - 2 < g:
+                v = f(v + f(inv * gout[x - 1, k]))
+            if x + 1 < X - g and x + 2 >= X - g:
+                v = f(v + f(inv * gout[x + 1, k]))
+            if k - 1 >= g and k - 2 < g:
+                v = f(v + f(inv * gout[x, k - 1]))
+            if k + 1 < Y - g and k + 2 >= Y - g:
+                v = f(v + f(inv * gout[x, k + 1]))
+            gp[x, k] = v
+            grhs[x, k] = f(grhs[x, k] + f(-h2 * f(inv * gout[x, k])))
+    return gp, grhs
+
+
+@pytest.mark.parametrize("shape,g,batch", [((70, 300, 1), 1, 1), ((1, 33, 261), 1, 1),
+                                           ((23, 41, 1), 2, 1), ((19, 70, 1), 1, 3),
+                                           ((4, 5, 1), 1, 2)])
+def test_folded_sweeps_2d_tiles_match_restatement(dev, shape, g, batch):
+    """The 2D row-tile sweep and transpose (single and batched entries) against
+    the FP32 restatement above, bit for bit: multi-strip rows, a partial last
+    chunk, ghost width 2, 2 interior nodes per axis; ghosts of the sweep's
+    output untouched, the transpose's gp 0 on ghosts."""
+    print(f"folded 2D tiles {shape} g={g} batch={batch}", flush=True)
+    rng = np.random.default_rng(31)
+    ext = [s for s in shape if s > 1]
+    full = (batch,) + shape + (1,)
+    h2, inv = np.float32(0.013), np.float32(0.25)
+    p = rng.uniform(-1, 1, full).astype(np.float32)
+    rhs = rng.uniform(-1, 1, full).astype(np.float32)
+    sentinel = rng.uniform(5, 6, full).astype(np.float32)
+    pd, rd = torch.from_numpy(p).to(dev), torch.from_numpy(rhs).to(dev)
+    out = torch.from_numpy(sentinel.copy()).to(dev)
+    gp = torch.empty_like(pd)
+    grhs = torch.from_numpy(sentinel.copy()).to(dev)
+    st = N.stream_of(pd)
+    if batch == 1:
+        N.call("pde_jacobi_neumann0_f32", pd.data_ptr(), rd.data_ptr(), out.data_ptr(), *shape, g,
+               h2, inv, st)
+        N.call("pde_jacobi_neumann0_adjoint_f32", pd.data_ptr(), gp.data_ptr(), grhs.data_ptr(),
+               *shape, g, h2, inv, st)
+    else:
+        N.call("pde_jacobi_neumann0_batch_f32", pd.data_ptr(), rd.data_ptr(), out.data_ptr(),
+               batch, *shape, g, h2, inv, st)
+        N.call("pde_jacobi_neumann0_adjoint_batch_f32", pd.data_ptr(), gp.data_ptr(),
+               grhs.data_ptr(), batch, *shape, g, h2, inv, st)
+    o, gpo, gro = (t.cpu().numpy().reshape((batch,) + tuple(ext)) for t in (out, gp, grhs))
+    p2, r2, s2 = (a.reshape((batch,) + tuple(ext)) for a in (p, rhs, sentinel))
+    for b in range(batch):
+        want = _folded_sweep_np(p2[b], r2[b], g, h2, inv)
+        inner = np.zeros(tuple(ext), bool)
+        inner[g:-g, g:-g] = True
+        want = np.where(inner, want, s2[b])
+        assert same_bits(o[b], want)
+        wgp, wgr = _folded_sweep_adjoint_np(p2[b], s2[b], g, h2, inv)
+        assert same_bits(gpo[b], wgp)
+        assert same_bits(gro[b], wgr)
