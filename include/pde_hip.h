@@ -508,6 +508,23 @@ int pde_jacobi_neumann0_adjoint_batch_f32(const float *gout, float *gp, float *g
                                           int64_t batch, int64_t nx, int64_t ny, int64_t nz,
                                           int32_t ghost, float h2, float inv, void *stream);
 
+/* Two consecutive pde_jacobi_neumann0_f32 sweeps in one launch, over `batch`
+ * instances of a 2D grid stored one after another: out = J(J(p, rhs), rhs)
+ * with the same values as two launches (the intermediate sweep never leaves
+ * registers); out's ghosts untouched.  PDE_ESHAPE unless the grid is 2D with
+ * 2+ interior nodes per axis.  No reference counterpart; it serves the
+ * projection cavity's pressure sweeps. */
+int pde_jacobi_neumann0_x2_f32(const float *p, const float *rhs, float *out, int64_t batch,
+                               int64_t nx, int64_t ny, int64_t nz, int32_t ghost, float h2,
+                               float inv, void *stream);
+
+/* Two consecutive pde_jacobi_neumann0_adjoint_f32 sweeps in one launch (same
+ * batching and limits): gp = T(T(gout)), grhs accumulated with the first
+ * sweep's contribution, then the second's, as two launches would. */
+int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, float *grhs, int64_t batch,
+                                       int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                       float h2, float inv, void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -293,6 +293,221 @@ __global__ __launch_bounds__(64 * JBY) void jacobi_n0_adj_rows_kernel(
     }
 }
 
+// Two folded sweeps in one pass: out = J(J(p)), the same values as two
+// launches.  The intermediate sweep lives in registers: a lane computes it on
+// JR + 2 rows (its own and one above and below) of its 4 columns, and the
+// strips overlap by one lane on each side (lanes 0 and 63 compute only the
+// intermediate), so every column neighbour of the second sweep comes from a
+// shuffle.  A strip covers 62 * 4 = 248 output columns.
+constexpr int J2_COLS = 248;
+
+__device__ __forceinline__ v4 j2_load(const float *__restrict__ f, const Tile2 &t, int64_t x,
+                                      int64_t k0)
+{
+    const v4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    return (k0 < 0 || k0 >= t.P2) ? z : ld_row4(f, t, x, k0);
+}
+
+// One folded sweep at the 4 columns of a lane (rows up / mid / dn, column
+// neighbours zl of element 0 and zr of element 3), interior nodes only.
+__device__ __forceinline__ v4 j_sweep4(const v4 &up, const v4 &mid, const v4 &dn, float zl0,
+                                       float zr3, const v4 &rh, int64_t x, int64_t k0,
+                                       int64_t lo1, int64_t hi1, int64_t lo2, int64_t hi2,
+                                       float h2, float inv)
+{
+    v4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t k = k0 + i;
+        if (x < lo1 || x >= hi1 || k < lo2 || k >= hi2) continue;
+        const float u = el(up, i), d = el(dn, i);
+        const float zl = i == 0 ? zl0 : el(mid, i > 0 ? i - 1 : 0);
+        const float zr = i == 3 ? zr3 : el(mid, i < 3 ? i + 1 : 3);
+        const float a_lo = x - 1 < lo1 ? -0.0f + d : u;
+        const float a_hi = x + 1 >= hi1 ? 0.0f + u : d;
+        const float b_lo = k - 1 < lo2 ? -0.0f + zr : zl;
+        const float b_hi = k + 1 >= hi2 ? 0.0f + zl : zr;
+        float acc = 0.0f;
+        acc += a_lo;
+        acc += a_hi;
+        acc += b_lo;
+        acc += b_hi;
+        set_el(o, i, inv * (acc - h2 * el(rh, i)));
+    }
+    return o;
+}
+
+__global__ __launch_bounds__(64 * JBY) void jacobi_n0_rows2_kernel(const float *__restrict__ p,
+                                                                   const float *__restrict__ rhs,
+                                                                   float *__restrict__ out,
+                                                                   JTile J, float h2, float inv)
+{
+    const Tile2 &t = J.t;
+    const uint32_t L = jtile_index();
+    if (L >= t.tq * t.tr) return;
+    const int64_t off = (int64_t)blockIdx.y * J.stride;
+    p += off;
+    rhs += off;
+    out += off;
+    const uint32_t qt = L / t.tr, rt = L % t.tr;
+    const int lane = threadIdx.x;
+    const int64_t k0 = (int64_t)qt * J2_COLS + 4 * (int64_t)(lane - 1);
+    const int64_t x0 = ((int64_t)rt * JBY + threadIdx.y) * JR;
+    const int64_t lo1 = t.g, hi1 = t.P1 - t.g, lo2 = t.g, hi2 = t.P2 - t.g;
+    v4 pr[JR + 4], rh[JR + 2];
+#pragma unroll
+    for (int r = 0; r < JR + 4; ++r) pr[r] = j2_load(p, t, x0 - 2 + r, k0);
+#pragma unroll
+    for (int r = 0; r < JR + 2; ++r) rh[r] = j2_load(rhs, t, x0 - 1 + r, k0);
+    // the intermediate sweep on rows x0-1 .. x0+JR; lane 0's left and lane
+    // 63's right column neighbour from memory (0 outside the field)
+    v4 q1[JR + 2];
+#pragma unroll
+    for (int r = 0; r < JR + 2; ++r) {
+        const int64_t x = x0 - 1 + r;
+        float zl = __shfl_up(pr[r + 1].w, 1, 64), zr = __shfl_down(pr[r + 1].x, 1, 64);
+        if (lane == 0 || lane == 63) {
+            const int64_t k = lane == 0 ? k0 - 1 : k0 + 4;
+            const float v = (k >= 0 && k < t.P2 && x >= 0 && x < t.P1) ? p[x * t.P2 + k] : 0.0f;
+            if (lane == 0) zl = v;
+            else zr = v;
+        }
+        q1[r] = j_sweep4(pr[r], pr[r + 1], pr[r + 2], zl, zr, rh[r], x, k0, lo1, hi1, lo2, hi2,
+                         h2, inv);
+    }
+    // the second sweep on rows x0 .. x0+JR-1, lanes 1 .. 62
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+        const int64_t x = x0 + r;
+        const float zl = __shfl_up(q1[r + 1].w, 1, 64), zr = __shfl_down(q1[r + 1].x, 1, 64);
+        const v4 o = j_sweep4(q1[r], q1[r + 1], q1[r + 2], zl, zr, rh[r + 1], x, k0, lo1, hi1,
+                              lo2, hi2, h2, inv);
+        if (lane == 0 || lane == 63 || x < lo1 || x >= hi1 || k0 >= t.P2) continue;
+        if (k0 >= lo2 && k0 + 4 <= hi2) {
+            st_row4(out, t, x, k0, o);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (k0 + i >= lo2 && k0 + i < hi2) out[x * t.P2 + k0 + i] = el(o, i);
+        }
+    }
+}
+
+// One transposed folded sweep at the 4 columns of a lane (gp; 0 off the
+// interior), as jacobi_neumann0_adjoint_kernel computes it.
+__device__ __forceinline__ v4 j_adj4(const v4 &up, const v4 &mid, const v4 &dn, float zl0,
+                                     float zr3, int64_t x, int64_t k0, int64_t lo1, int64_t hi1,
+                                     int64_t lo2, int64_t hi2, float inv)
+{
+    v4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t k = k0 + i;
+        if (x < lo1 || x >= hi1 || k < lo2 || k >= hi2) continue;
+        const float u = el(up, i), d = el(dn, i);
+        const float zl = i == 0 ? zl0 : el(mid, i > 0 ? i - 1 : 0);
+        const float zr = i == 3 ? zr3 : el(mid, i < 3 ? i + 1 : 3);
+        float acc = 0.0f;
+        if (x + 1 < hi1) acc += d;
+        if (x - 1 >= lo1) acc += u;
+        if (k + 1 < hi2) acc += zr;
+        if (k - 1 >= lo2) acc += zl;
+        float v = inv * acc;
+        if (x - 1 >= lo1 && x - 2 < lo1) v = v + inv * u;
+        if (x + 1 < hi1 && x + 2 >= hi1) v = v + inv * d;
+        if (k - 1 >= lo2 && k - 2 < lo2) v = v + inv * zl;
+        if (k + 1 < hi2 && k + 2 >= hi2) v = v + inv * zr;
+        set_el(o, i, v);
+    }
+    return o;
+}
+
+// Two transposed sweeps in one pass (the same overlapping strips as
+// jacobi_n0_rows2_kernel): gp = T(T(g)), grhs += (rhs part of T at g), then
+// += (that at T(g)), in that order: the values of two launches.
+__global__ __launch_bounds__(64 * JBY) void jacobi_n0_adj_rows2_kernel(
+    const float *__restrict__ g, float *__restrict__ gp, float *__restrict__ grhs, JTile J,
+    float h2, float inv)
+{
+    const Tile2 &t = J.t;
+    const uint32_t L = jtile_index();
+    if (L >= t.tq * t.tr) return;
+    const int64_t off = (int64_t)blockIdx.y * J.stride;
+    g += off;
+    gp += off;
+    grhs += off;
+    const uint32_t qt = L / t.tr, rt = L % t.tr;
+    const int lane = threadIdx.x;
+    const int64_t k0 = (int64_t)qt * J2_COLS + 4 * (int64_t)(lane - 1);
+    const int64_t x0 = ((int64_t)rt * JBY + threadIdx.y) * JR;
+    const int64_t lo1 = t.g, hi1 = t.P1 - t.g, lo2 = t.g, hi2 = t.P2 - t.g;
+    v4 gr[JR + 4];
+#pragma unroll
+    for (int r = 0; r < JR + 4; ++r) gr[r] = j2_load(g, t, x0 - 2 + r, k0);
+    v4 t1[JR + 2];
+#pragma unroll
+    for (int r = 0; r < JR + 2; ++r) {
+        const int64_t x = x0 - 1 + r;
+        float zl = __shfl_up(gr[r + 1].w, 1, 64), zr = __shfl_down(gr[r + 1].x, 1, 64);
+        if (lane == 0 || lane == 63) {
+            const int64_t k = lane == 0 ? k0 - 1 : k0 + 4;
+            const float v = (k >= 0 && k < t.P2 && x >= 0 && x < t.P1) ? g[x * t.P2 + k] : 0.0f;
+            if (lane == 0) zl = v;
+            else zr = v;
+        }
+        t1[r] = j_adj4(gr[r], gr[r + 1], gr[r + 2], zl, zr, x, k0, lo1, hi1, lo2, hi2, inv);
+    }
+#pragma unroll
+    for (int r = 0; r < JR; ++r) {
+        const int64_t x = x0 + r;
+        const float zl = __shfl_up(t1[r + 1].w, 1, 64), zr = __shfl_down(t1[r + 1].x, 1, 64);
+        const v4 o = j_adj4(t1[r], t1[r + 1], t1[r + 2], zl, zr, x, k0, lo1, hi1, lo2, hi2, inv);
+        if (lane == 0 || lane == 63 || x >= t.P1 || k0 >= t.P2) continue;
+        st_row4(gp, t, x, k0, o);
+        if (x < lo1 || x >= hi1) continue;
+        v4 acc = ld_row4(grhs, t, x, k0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t k = k0 + i;
+            if (k < lo2 || k >= hi2) continue;
+            float a = el(acc, i) + -h2 * (inv * el(gr[r + 2], i));
+            a = a + -h2 * (inv * el(t1[r + 1], i));
+            set_el(acc, i, a);
+        }
+        if (k0 >= lo2 && k0 + 4 <= hi2) {
+            st_row4(grhs, t, x, k0, acc);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (k0 + i >= lo2 && k0 + i < hi2) grhs[x * t.P2 + k0 + i] = el(acc, i);
+        }
+    }
+}
+
+template <bool ADJ>
+static int jacobi_rows2_launch(const float *p, const float *rhs, float *out, float *grhs,
+                               int64_t batch, int64_t P1, int64_t P2, int64_t ghost, float h2,
+                               float inv, hipStream_t s)
+{
+    JTile J;
+    J.t.P1 = P1;
+    J.t.P2 = P2;
+    J.t.g = ghost;
+    J.t.nq = (P2 + 3) / 4;
+    J.t.tq = (uint32_t)((P2 + J2_COLS - 1) / J2_COLS);
+    J.t.tr = (uint32_t)((P1 + JBY * JR - 1) / (JBY * JR));
+    J.stride = P1 * P2;
+    const int64_t nb = ((int64_t)J.t.tq * J.t.tr + 7) / 8 * 8;
+    if (nb > 0x7fffffff || batch > 65535) return PDE_ESHAPE;
+    if (ADJ)
+        hipLaunchKernelGGL(jacobi_n0_adj_rows2_kernel, dim3((unsigned)nb, (unsigned)batch),
+                           dim3(64, JBY), 0, s, p, out, grhs, J, h2, inv);
+    else
+        hipLaunchKernelGGL(jacobi_n0_rows2_kernel, dim3((unsigned)nb, (unsigned)batch),
+                           dim3(64, JBY), 0, s, p, rhs, out, J, h2, inv);
+    return launch_status();
+}
+
 // Launch of either row-tile kernel over `batch` instances of a 2D grid
 // (padded extents P1 x P2); PDE_ESHAPE beyond the grid limits.
 template <bool ADJ>
@@ -480,6 +695,35 @@ extern "C" int pde_jacobi_neumann0_adjoint_batch_f32(const float *gout, float *g
     hipLaunchKernelGGL(jacobi_neumann0_adjoint_kernel, grid, dim3(256), 0, as_stream(stream), gout,
                        gp, grhs, g, P[0], P[1], P[2], h2, inv);
     return launch_status();
+}
+
+extern "C" int pde_jacobi_neumann0_x2_f32(const float *p, const float *rhs, float *out,
+                                          int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                          int32_t ghost, float h2, float inv, void *stream)
+{
+    if (!p || !rhs || !out || out == p || out == rhs) return PDE_EINVAL;
+    JacGeom g;
+    int64_t P[3];
+    int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
+    if (st) return st;
+    if (g.ndim != 2) return PDE_ESHAPE;
+    return jacobi_rows2_launch<false>(p, rhs, out, nullptr, batch, P[1], P[2], ghost, h2, inv,
+                                      as_stream(stream));
+}
+
+extern "C" int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, float *grhs,
+                                                  int64_t batch, int64_t nx, int64_t ny,
+                                                  int64_t nz, int32_t ghost, float h2, float inv,
+                                                  void *stream)
+{
+    if (!gout || !gp || !grhs || gp == gout || grhs == gout || grhs == gp) return PDE_EINVAL;
+    JacGeom g;
+    int64_t P[3];
+    int st = batch_geom(batch, nx, ny, nz, ghost, &g, P);
+    if (st) return st;
+    if (g.ndim != 2) return PDE_ESHAPE;
+    return jacobi_rows2_launch<true>(gout, nullptr, gp, grhs, batch, P[1], P[2], ghost, h2, inv,
+                                     as_stream(stream));
 }
 
 extern "C" int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes,

@@ -209,17 +209,22 @@ class ProjectionCavity:
         """The Jacobi sweeps from p into p_out (``batch`` instances stored one
         after another).  Neumann-0 walls are folded into each sweep
         (pde_jacobi_neumann0_f32: the same values as applying p_bc, then
-        sweeping); sweeps alternate between p_out and pf so that no sweep
-        writes what it reads, ending in p_out."""
+        sweeping), two sweeps per launch (pde_jacobi_neumann0_x2_f32, the
+        same values again) and a single one when the count is odd.  Launches
+        alternate between p_out and pf so that none writes what it reads and
+        the last one writes p_out."""
         s = N.stream_of(p)
-        bufs = (p_out, pf) if self.sweeps % 2 == 1 else (pf, p_out)
+        plan = [2] * (self.sweeps // 2) + [1] * (self.sweeps % 2)
         cur = p
-        for k in range(self.sweeps):
-            dst = bufs[k % 2]
-            if dst is cur:                       # p_out is p on the first sweep
+        for j, m in enumerate(plan):
+            dst = p_out if (len(plan) - 1 - j) % 2 == 0 else pf
+            if dst is cur:                       # p_out is p on the first launch
                 N.call("pde_copy_f32", cur.data_ptr(), p0.data_ptr(), cur.numel(), s)
                 cur = p0
-            if batch == 1:
+            if m == 2:
+                N.call("pde_jacobi_neumann0_x2_f32", cur.data_ptr(), rhs.data_ptr(),
+                       dst.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
+            elif batch == 1:
                 N.call("pde_jacobi_neumann0_f32", cur.data_ptr(), rhs.data_ptr(), dst.data_ptr(),
                        *self.shape, self.g, self.h2, self.inv, s)
             else:
@@ -329,11 +334,16 @@ class ProjectionCavity:
 
     def _adj_sweeps(self, g, spare, grhs, batch):
         """The sweeps p <- J(B_p(p), rhs), transposed last to first, from g
-        (``batch`` instances one after another); rhs's gradient accumulates in
-        grhs.  Returns the buffer (g or spare) holding p's gradient."""
+        (``batch`` instances one after another), two per launch; rhs's
+        gradient accumulates in grhs.  Returns the buffer (g or spare) holding
+        p's gradient."""
         s = N.stream_of(g)
         cur, nxt = g, spare
-        for _ in range(self.sweeps):
+        for _ in range(self.sweeps // 2):          # two transposed sweeps per launch
+            N.call("pde_jacobi_neumann0_adjoint_x2_f32", cur.data_ptr(), nxt.data_ptr(),
+                   grhs.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
+            cur, nxt = nxt, cur
+        for _ in range(self.sweeps % 2):
             if batch == 1:
                 N.call("pde_jacobi_neumann0_adjoint_f32", cur.data_ptr(), nxt.data_ptr(),
                        grhs.data_ptr(), *self.shape, self.g, self.h2, self.inv, s)

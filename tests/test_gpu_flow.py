@@ -345,7 +345,8 @@ def _folded_sweep_adjoint_np(gout, grhs, g, h2, inv):
 
 @pytest.mark.parametrize("shape,g,batch", [((70, 300, 1), 1, 1), ((1, 33, 261), 1, 1),
                                            ((23, 41, 1), 2, 1), ((19, 70, 1), 1, 3),
-                                           ((4, 5, 1), 1, 2)])
+                                           ((4, 5, 1), 1, 2), ((9, 500, 1), 1, 2),
+                                           ((250, 249, 1), 3, 1)])
 def test_folded_sweeps_2d_tiles_match_restatement(dev, shape, g, batch):
     """The 2D row-tile sweep and transpose (single and batched entries) against
     the FP32 restatement above, bit for bit: multi-strip rows, a partial last
@@ -374,6 +375,28 @@ def test_folded_sweeps_2d_tiles_match_restatement(dev, shape, g, batch):
                batch, *shape, g, h2, inv, st)
         N.call("pde_jacobi_neumann0_adjoint_batch_f32", pd.data_ptr(), gp.data_ptr(),
                grhs.data_ptr(), batch, *shape, g, h2, inv, st)
+    # two sweeps in one launch == two single sweeps (ghosts of out untouched)
+    two = torch.from_numpy(sentinel.copy()).to(dev)
+    mid = torch.from_numpy(sentinel.copy()).to(dev)
+    ref = torch.from_numpy(sentinel.copy()).to(dev)
+    N.call("pde_jacobi_neumann0_x2_f32", pd.data_ptr(), rd.data_ptr(), two.data_ptr(), batch,
+           *shape, g, h2, inv, st)
+    N.call("pde_jacobi_neumann0_batch_f32", pd.data_ptr(), rd.data_ptr(), mid.data_ptr(), batch,
+           *shape, g, h2, inv, st)
+    N.call("pde_jacobi_neumann0_batch_f32", mid.data_ptr(), rd.data_ptr(), ref.data_ptr(), batch,
+           *shape, g, h2, inv, st)
+    assert same_bits(two.cpu().numpy(), ref.cpu().numpy())
+    # two transposed sweeps in one launch == two single ones (gp and grhs)
+    gp2, gr2 = torch.empty_like(pd), torch.from_numpy(sentinel.copy()).to(dev)
+    gm, gp1, gr1 = torch.empty_like(pd), torch.empty_like(pd), torch.from_numpy(sentinel.copy()).to(dev)
+    N.call("pde_jacobi_neumann0_adjoint_x2_f32", pd.data_ptr(), gp2.data_ptr(), gr2.data_ptr(),
+           batch, *shape, g, h2, inv, st)
+    N.call("pde_jacobi_neumann0_adjoint_batch_f32", pd.data_ptr(), gm.data_ptr(), gr1.data_ptr(),
+           batch, *shape, g, h2, inv, st)
+    N.call("pde_jacobi_neumann0_adjoint_batch_f32", gm.data_ptr(), gp1.data_ptr(), gr1.data_ptr(),
+           batch, *shape, g, h2, inv, st)
+    assert same_bits(gp2.cpu().numpy(), gp1.cpu().numpy())
+    assert same_bits(gr2.cpu().numpy(), gr1.cpu().numpy())
     o, gpo, gro = (t.cpu().numpy().reshape((batch,) + tuple(ext)) for t in (out, gp, grhs))
     p2, r2, s2 = (a.reshape((batch,) + tuple(ext)) for a in (p, rhs, sentinel))
     for b in range(batch):
