@@ -525,6 +525,18 @@ int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, float *grhs
                                        int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
                                        float h2, float inv, void *stream);
 
+/* The projection cavity's predictor on a 2D vec2 field in one launch:
+ * us = u2 + dt * (-(grad u2) u2 + nu * lap(u2)) with the values of
+ * pde_grad_f32 (alpha 1) + pde_matvec_f32 + pde_laplacian_f32 (alpha nu) +
+ * pde_lincomb_f32 (-1, +1) + pde_forward_euler_f32 on zero-ghost work
+ * buffers, every padded node written.  u2's ghosts are read.  No reference
+ * counterpart (the reference's cavities use glue-operator chains,
+ * examples/LDC/LDC_No_Graph.py:87,90). */
+int pde_cavity_predict_f32(const float *u2, float *us, int64_t nx, int64_t ny, int64_t nz,
+                           int32_t ghost, float lap_wm, float lap_wc, float lap_wp,
+                           float grad_wm, float grad_wc, float grad_wp, float nu, float dt,
+                           void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -533,6 +533,63 @@ static int jacobi_rows_launch(const float *a, float *b, float *c, const float *r
     return launch_status();
 }
 
+// The cavity's predictor in one pass over a 2D vec2 field (thread per padded
+// node): us = u2 + dt * (-(grad u2) u2 + nu * lap(u2)), each piece with the
+// operation order of its own kernel (grad_generic: per component, the row
+// axis then the column axis, acc = 0, += wm u[n-e], += wc u[n], += wp u[n+e],
+// times 1; matvec_kernel: m0 v0 then + m1 v1; lap_generic<2>: six terms then
+// times nu; lincomb_kernel: (-1) conv then + 1 diff; euler_kernel: u2 + dt F).
+// Off the interior those kernels leave grad and lap at 0, so there
+// us = u2 + dt * ((-1) (0 u0 + 0 u1) + 1 * 0), computed the same way.
+__global__ __launch_bounds__(256) void cavity_predict_kernel(
+    const float *__restrict__ u2, float *__restrict__ us, int64_t P1, int64_t P2, int64_t g,
+    float lm, float lc, float lp, float gm, float gc, float gp, float nu, float dt)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    const int64_t S1 = 2 * P2, S2 = 2;
+    for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
+        const int64_t e = (x * P2 + k) * 2;
+        const float v0 = u2[e], v1 = u2[e + 1];
+        const bool in = x >= g && x < P1 - g && k >= g && k < P2 - g;
+        float du[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, diff[2] = {0.0f, 0.0f};
+        if (in) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float ctr = u2[e + c];
+                const float xm = u2[e + c - S1], xp = u2[e + c + S1];
+                const float ym = u2[e + c - S2], yp = u2[e + c + S2];
+                float a = 0.0f;
+                a += gm * xm;
+                a += gc * ctr;
+                a += gp * xp;
+                du[c][0] = 1.0f * a;
+                a = 0.0f;
+                a += gm * ym;
+                a += gc * ctr;
+                a += gp * yp;
+                du[c][1] = 1.0f * a;
+                a = 0.0f;
+                a += lm * xm;
+                a += lc * ctr;
+                a += lp * xp;
+                a += lm * ym;
+                a += lc * ctr;
+                a += lp * yp;
+                diff[c] = nu * a;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            float conv = du[c][0] * v0;
+            conv = conv + du[c][1] * v1;
+            float F = -1.0f * conv;
+            F = F + 1.0f * diff[c];
+            us[e + c] = u2[e + c] + dt * F;
+        }
+    }
+}
+
 // out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
 __global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
                                                        const float *__restrict__ gv,
@@ -724,6 +781,23 @@ extern "C" int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, 
     if (g.ndim != 2) return PDE_ESHAPE;
     return jacobi_rows2_launch<true>(gout, nullptr, gp, grhs, batch, P[1], P[2], ghost, h2, inv,
                                      as_stream(stream));
+}
+
+extern "C" int pde_cavity_predict_f32(const float *u2, float *us, int64_t nx, int64_t ny,
+                                      int64_t nz, int32_t ghost, float lap_wm, float lap_wc,
+                                      float lap_wp, float grad_wm, float grad_wc, float grad_wp,
+                                      float nu, float dt, void *stream)
+{
+    if (!u2 || !us || u2 == us) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 2) return PDE_ESHAPE;
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
+    hipLaunchKernelGGL(cavity_predict_kernel, grid, dim3(256), 0, as_stream(stream), u2, us,
+                       G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm, grad_wc,
+                       grad_wp, nu, dt);
+    return launch_status();
 }
 
 extern "C" int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes,

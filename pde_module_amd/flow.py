@@ -190,17 +190,10 @@ class ProjectionCavity:
         grid = self.shape
         self.u_bc._ensure_table(u)
         self.u_bc.table.apply(u, b["u2"], None)
-        N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
-               np.float32(1.0), s)
-        N.call("pde_matvec_f32", b["du"].data_ptr(), b["u2"].data_ptr(), b["conv"].data_ptr(),
-               int(np.prod(grid)), 2, 2, s)
-        N.call("pde_laplacian_f32", b["u2"].data_ptr(), b["diff"].data_ptr(), *grid, 2, g,
-               *self.wl, np.float32(self.nu), s)
-        N.call("pde_lincomb_f32", b["conv"].data_ptr(), b["diff"].data_ptr(), None, None,
-               np.float32(-1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
-               b["F"].data_ptr(), b["F"].numel(), s)
-        N.call("pde_forward_euler_f32", b["u2"].data_ptr(), b["F"].data_ptr(), b["us"].data_ptr(),
-               b["us"].numel(), np.float32(self.dt), s)
+        # grad, convection product, diffusion, F and the Euler update in one
+        # launch (the values of the five separate kernels)
+        N.call("pde_cavity_predict_f32", b["u2"].data_ptr(), b["us"].data_ptr(), *grid, g,
+               *self.wl, *self.wc, np.float32(self.nu), np.float32(self.dt), s)
         self.u_bc.table.apply(b["us"], us2, None)
         N.call("pde_divergence_f32", us2.data_ptr(), rhs.data_ptr(), *grid, 1, g, *self.wc,
                np.float32(1.0 / self.dt), s)

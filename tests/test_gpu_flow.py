@@ -408,3 +408,32 @@ def test_folded_sweeps_2d_tiles_match_restatement(dev, shape, g, batch):
         wgp, wgr = _folded_sweep_adjoint_np(p2[b], s2[b], g, h2, inv)
         assert same_bits(gpo[b], wgp)
         assert same_bits(gro[b], wgr)
+
+
+@pytest.mark.parametrize("shape,g", [((20, 33, 1), 1), ((1, 70, 300), 1), ((15, 12, 1), 2)])
+def test_cavity_predictor_fused_equals_separate_kernels(dev, shape, g):
+    """pde_cavity_predict_f32 against grad + matvec + laplacian + lincomb +
+    Euler on zero-ghost work buffers, every padded node bit for bit (signed
+    zeros on the ghosts included)."""
+    rng = np.random.default_rng(41)
+    u = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
+    u[0, ...] = -0.0                                   # signed zeros in a ghost layer
+    ud = torch.from_numpy(u).to(dev)
+    dx = 0.05
+    from pde_module_amd.FDM.module.grad import central_weights
+    wl, wc = O.laplacian_weights(dx), central_weights(dx)
+    nu, dt = np.float32(0.01), np.float32(1e-3)
+    z = lambda *e: torch.zeros(shape + e, device=dev)
+    du, conv, diff, F, us = z(2, 2), z(2), z(2), z(2), z(2)
+    st = N.stream_of(ud)
+    N.call("pde_grad_f32", ud.data_ptr(), du.data_ptr(), *shape, 2, g, *wc, np.float32(1.0), st)
+    N.call("pde_matvec_f32", du.data_ptr(), ud.data_ptr(), conv.data_ptr(),
+           int(np.prod(shape)), 2, 2, st)
+    N.call("pde_laplacian_f32", ud.data_ptr(), diff.data_ptr(), *shape, 2, g, *wl, nu, st)
+    N.call("pde_lincomb_f32", conv.data_ptr(), diff.data_ptr(), None, None, np.float32(-1.0),
+           np.float32(1.0), np.float32(0), np.float32(0), 2, F.data_ptr(), F.numel(), st)
+    N.call("pde_forward_euler_f32", ud.data_ptr(), F.data_ptr(), us.data_ptr(), us.numel(), dt, st)
+    fused = torch.full_like(us, 7.0)
+    N.call("pde_cavity_predict_f32", ud.data_ptr(), fused.data_ptr(), *shape, g, *wl, *wc, nu, dt,
+           st)
+    assert same_bits(fused.cpu().numpy(), us.cpu().numpy())
