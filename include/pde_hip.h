@@ -537,6 +537,16 @@ int pde_cavity_predict_f32(const float *u2, float *us, int64_t nx, int64_t ny, i
                            float grad_wm, float grad_wc, float grad_wp, float nu, float dt,
                            void *stream);
 
+/* The projection cavity's correction on a 2D grid in one launch:
+ * u_out = us2 - dt * grad(B_p(p)) (u_out, us2 vec2; p scalar) with the
+ * values of a legacy vonNeumann_BC('ALL', 0) applied to p, pde_grad_f32
+ * (alpha 1) on a zero-ghost buffer and pde_forward_euler_f32 (-dt) over every
+ * node; p's ghosts are not read.  u_out may alias us2, not p.  Every
+ * interior extent must be >= 2.  No reference counterpart. */
+int pde_cavity_correct_f32(const float *p, const float *us2, float *u_out, int64_t nx, int64_t ny,
+                           int64_t nz, int32_t ghost, float grad_wm, float grad_wc,
+                           float grad_wp, float dt, void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pde_jacobi_neumann0_x2_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_cavity_predict_f32": [_P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F, _F, _F, _F,
                                _P],
+    "pde_cavity_correct_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _P],
     "pde_jacobi_neumann0_adjoint_x2_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_neumann0_adjoint_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F,
                                               _P],

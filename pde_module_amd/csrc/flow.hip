@@ -590,6 +590,44 @@ __global__ __launch_bounds__(256) void cavity_predict_kernel(
     }
 }
 
+// The cavity's correction in one pass (thread per padded node):
+// u_out = us2 + (-dt) * grad(B_p(p)) with B_p the legacy Neumann-0 boundary
+// on every face folded in (a face-ghost neighbour of an interior node reads
+// (-0 or +0) + its mirror, the value the boundary writes; see
+// jacobi_neumann0_kernel), grad with grad_generic's order (per axis, acc = 0,
+// += wm p[n-e], += wc p[n], += wp p[n+e], times 1) and 0 off the interior,
+// and the update with euler_kernel's (us2 + (-dt) * gp) on every node.
+__global__ __launch_bounds__(256) void cavity_correct_kernel(
+    const float *__restrict__ p, const float *__restrict__ us2, float *__restrict__ u_out,
+    int64_t P1, int64_t P2, int64_t g, float gm, float gc, float gp, float ndt)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
+        const int64_t n = x * P2 + k;
+        float d[2] = {0.0f, 0.0f};
+        if (x >= g && x < P1 - g && k >= g && k < P2 - g) {
+            const float pc = p[n];
+            const float xm = x - 1 < g ? -0.0f + p[n + P2] : p[n - P2];
+            const float xp = x + 1 >= P1 - g ? 0.0f + p[n - P2] : p[n + P2];
+            const float ym = k - 1 < g ? -0.0f + p[n + 1] : p[n - 1];
+            const float yp = k + 1 >= P2 - g ? 0.0f + p[n - 1] : p[n + 1];
+            float a = 0.0f;
+            a += gm * xm;
+            a += gc * pc;
+            a += gp * xp;
+            d[0] = 1.0f * a;
+            a = 0.0f;
+            a += gm * ym;
+            a += gc * pc;
+            a += gp * yp;
+            d[1] = 1.0f * a;
+        }
+        u_out[2 * n] = us2[2 * n] + ndt * d[0];
+        u_out[2 * n + 1] = us2[2 * n + 1] + ndt * d[1];
+    }
+}
+
 // out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
 __global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
                                                        const float *__restrict__ gv,
@@ -797,6 +835,23 @@ extern "C" int pde_cavity_predict_f32(const float *u2, float *us, int64_t nx, in
     hipLaunchKernelGGL(cavity_predict_kernel, grid, dim3(256), 0, as_stream(stream), u2, us,
                        G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm, grad_wc,
                        grad_wp, nu, dt);
+    return launch_status();
+}
+
+extern "C" int pde_cavity_correct_f32(const float *p, const float *us2, float *u_out,
+                                      int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                      float grad_wm, float grad_wc, float grad_wp, float dt,
+                                      void *stream)
+{
+    if (!p || !us2 || !u_out || u_out == p) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 2) return PDE_ESHAPE;
+    if (G.E[1] < 2 || G.E[2] < 2) return PDE_ESHAPE;    // a wall node's mirror must be interior
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
+    hipLaunchKernelGGL(cavity_correct_kernel, grid, dim3(256), 0, as_stream(stream), p, us2,
+                       u_out, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
     return launch_status();
 }
 

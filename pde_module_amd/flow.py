@@ -229,13 +229,10 @@ class ProjectionCavity:
 
     def _correct(self, p_out, us2, u_out):
         """u_out = us2 - dt * grad(B_p(p_out))."""
-        b, g, s = self.b, self.g, N.stream_of(p_out)
-        self.p_bc._ensure_table(p_out)
-        self.p_bc.table.apply(p_out, b["pf"], None)
-        N.call("pde_grad_f32", b["pf"].data_ptr(), b["gp"].data_ptr(), *self.shape, 1, g,
-               *self.wc, np.float32(1.0), s)
-        N.call("pde_forward_euler_f32", us2.data_ptr(), b["gp"].data_ptr(), u_out.data_ptr(),
-               u_out.numel(), np.float32(-self.dt), s)
+        # B_p (Neumann 0 on every face) folded into the gradient: one launch
+        # with the values of boundary + grad + Euler
+        N.call("pde_cavity_correct_f32", p_out.data_ptr(), us2.data_ptr(), u_out.data_ptr(),
+               *self.shape, self.g, *self.wc, np.float32(self.dt), N.stream_of(p_out))
 
     def run(self, u0, p0, steps, use_graph=True):
         """``steps`` steps from (u0, p0); returns the final (u, p)."""

@@ -437,3 +437,37 @@ def test_cavity_predictor_fused_equals_separate_kernels(dev, shape, g):
     N.call("pde_cavity_predict_f32", ud.data_ptr(), fused.data_ptr(), *shape, g, *wl, *wc, nu, dt,
            st)
     assert same_bits(fused.cpu().numpy(), us.cpu().numpy())
+
+
+@pytest.mark.parametrize("shape,g", [((20, 33, 1), 1), ((1, 70, 300), 1), ((15, 12, 1), 2)])
+def test_cavity_correction_fused_equals_boundary_grad_euler(dev, shape, g):
+    """pde_cavity_correct_f32 against legacy vonNeumann_BC('ALL', 0) on p, then
+    pde_grad_f32 into a zero-ghost buffer, then Euler with -dt over every node:
+    the same bits everywhere, signed zeros beside the walls included."""
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    from pde_module_amd.FDM.module.grad import central_weights
+    rng = np.random.default_rng(43)
+    ext = [s for s in shape if s > 1]
+    p = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    pv = p.reshape(ext)
+    pv[g + 1, :] = -0.0                    # mirrors of the -X wall's ghosts
+    pv[:, -g - 2] = -0.0                   # and of the +Y wall's
+    us2 = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
+    us2.reshape(ext + [2])[0, :, :] = -0.0
+    pd, ud = torch.from_numpy(p).to(dev), torch.from_numpy(us2).to(dev)
+    dx = 0.05
+    wc = central_weights(dx)
+    dt = np.float32(2e-3)
+    bc = LegacyGridBoundary(pd.clone(), dx, g)
+    bc.vonNeumann_BC("ALL", 0.0)
+    pf = bc(pd)
+    gp = torch.zeros(shape + (1, 2), device=dev)
+    want = torch.empty_like(ud)
+    st = N.stream_of(pd)
+    N.call("pde_grad_f32", pf.data_ptr(), gp.data_ptr(), *shape, 1, g, *wc, np.float32(1.0), st)
+    N.call("pde_forward_euler_f32", ud.data_ptr(), gp.data_ptr(), want.data_ptr(), want.numel(),
+           np.float32(-dt), st)
+    got = torch.full_like(ud, 9.0)
+    N.call("pde_cavity_correct_f32", pd.data_ptr(), ud.data_ptr(), got.data_ptr(), *shape, g, *wc,
+           dt, st)
+    assert same_bits(got.cpu().numpy(), want.cpu().numpy())
