@@ -547,6 +547,17 @@ int pde_cavity_correct_f32(const float *p, const float *us2, float *u_out, int64
                            int64_t nz, int32_t ghost, float grad_wm, float grad_wc,
                            float grad_wp, float dt, void *stream);
 
+/* Transpose of the predictor's interior part (the map u2 -> us of
+ * pde_cavity_predict_f32) in one launch: gu2 = gus + nu lap^T(dt gus) +
+ * (grad u2)^T (-dt gus) + grad^T(outer(-dt gus, u2)), with the values of
+ * pde_scale_f32, pde_laplacian_adjoint_f32, pde_grad_f32, pde_outer_f32,
+ * pde_matvec_t_f32, pde_grad_adjoint_f32 and pde_lincomb_f32 run in that
+ * chain; every padded node written.  No reference counterpart. */
+int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2, float *gu2, int64_t nx,
+                                   int64_t ny, int64_t nz, int32_t ghost, float lap_wm,
+                                   float lap_wc, float lap_wp, float grad_wm, float grad_wc,
+                                   float grad_wp, float nu, float dt, void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -628,6 +628,92 @@ __global__ __launch_bounds__(256) void cavity_correct_kernel(
     }
 }
 
+// Transpose of the predictor's interior part in one pass (thread per padded
+// node of a 2D vec2 field): from gus (the gradient of us) and u2,
+//   gF = dt gus, gconv = -1 gF, glap = nu * lap^T(gF), gdu = outer(gconv, u2),
+//   gmv = (grad u2)^T gconv, ggr = 1 * grad^T(gdu),
+//   gu2 = 1 gus + 1 glap + 1 gmv + 1 ggr,
+// each with the operation order of its own kernel (scale_kernel,
+// lap_adj_kernel / grad_adj_kernel's gathers with interior masks,
+// outer_kernel, matvec_t_kernel, lincomb_kernel); grad u2 is recomputed with
+// grad_generic's order and is 0 off the interior.
+__global__ __launch_bounds__(256) void cavity_adj_predict_kernel(
+    const float *__restrict__ gus, const float *__restrict__ u2, float *__restrict__ gu2,
+    int64_t P1, int64_t P2, int64_t g, float lm, float lc, float lp, float gm, float gc,
+    float gp, float nu, float dt)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    const int64_t R = 2 * P2;    // floats per row
+    auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
+    auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
+    for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
+        const int64_t n = x * P2 + k;
+        const bool ix = inx(x), iy = ink(k);
+        // gF and gconv at a node (floats e, e + 1)
+        auto gF = [&](int64_t e, int c) { return dt * gus[e + c]; };
+        auto gconv = [&](int64_t e, int c) { return -1.0f * gF(e, c); };
+        // gdu(node, c, t) = gconv(node, c) * u2(node, t)
+        auto gdu = [&](int64_t e, int c, int t) { return gconv(e, c) * u2[e + t]; };
+        const int64_t e = 2 * n;
+        // forward grad u2 at this node (du[c][t])
+        float du[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+        if (ix && iy) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                float a = 0.0f;
+                a += gm * u2[e - R + c];
+                a += gc * u2[e + c];
+                a += gp * u2[e + R + c];
+                du[c][0] = 1.0f * a;
+                a = 0.0f;
+                a += gm * u2[e - 2 + c];
+                a += gc * u2[e + c];
+                a += gp * u2[e + 2 + c];
+                du[c][1] = 1.0f * a;
+            }
+        }
+        const float gc0 = gconv(e, 0), gc1 = gconv(e, 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            // lap^T(gF)
+            float al = 0.0f;
+            if (iy) {
+                if (inx(x + 1)) al += lm * gF(e + R, c);
+                if (ix) al += lc * gF(e, c);
+                if (inx(x - 1)) al += lp * gF(e - R, c);
+            }
+            if (ix) {
+                if (ink(k + 1)) al += lm * gF(e + 2, c);
+                if (iy) al += lc * gF(e, c);
+                if (ink(k - 1)) al += lp * gF(e - 2, c);
+            }
+            const float glap = nu * al;
+            // (grad u2)^T gconv
+            float gmv = du[0][c] * gc0;
+            gmv = gmv + du[1][c] * gc1;
+            // grad^T(gdu)
+            float ag = 0.0f;
+            if (iy) {
+                if (inx(x + 1)) ag += gm * gdu(e + R, c, 0);
+                if (ix) ag += gc * gdu(e, c, 0);
+                if (inx(x - 1)) ag += gp * gdu(e - R, c, 0);
+            }
+            if (ix) {
+                if (ink(k + 1)) ag += gm * gdu(e + 2, c, 1);
+                if (iy) ag += gc * gdu(e, c, 1);
+                if (ink(k - 1)) ag += gp * gdu(e - 2, c, 1);
+            }
+            const float ggr = 1.0f * ag;
+            float acc = 1.0f * gus[e + c];
+            acc = acc + 1.0f * glap;
+            acc = acc + 1.0f * gmv;
+            acc = acc + 1.0f * ggr;
+            gu2[e + c] = acc;
+        }
+    }
+}
+
 // out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending (transpose of matvec).
 __global__ __launch_bounds__(256) void matvec_t_kernel(const float *__restrict__ m,
                                                        const float *__restrict__ gv,
@@ -852,6 +938,24 @@ extern "C" int pde_cavity_correct_f32(const float *p, const float *us2, float *u
     const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
     hipLaunchKernelGGL(cavity_correct_kernel, grid, dim3(256), 0, as_stream(stream), p, us2,
                        u_out, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
+    return launch_status();
+}
+
+extern "C" int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2, float *gu2,
+                                              int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                              float lap_wm, float lap_wc, float lap_wp,
+                                              float grad_wm, float grad_wc, float grad_wp,
+                                              float nu, float dt, void *stream)
+{
+    if (!gus || !u2 || !gu2 || gu2 == gus || gu2 == u2) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 2) return PDE_ESHAPE;
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
+    hipLaunchKernelGGL(cavity_adj_predict_kernel, grid, dim3(256), 0, as_stream(stream), gus, u2,
+                       gu2, G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm,
+                       grad_wc, grad_wp, nu, dt);
     return launch_status();
 }
 

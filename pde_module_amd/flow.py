@@ -348,11 +348,8 @@ class ProjectionCavity:
         gradient grhs and gu_out, u's gradient gu (u is the step's input)."""
         b, a, g, s = self.b, self.ga, self.g, N.stream_of(u)
         grid = self.shape
-        nodes = int(np.prod(grid))
-        # the two forward values this transpose reads: u2 = B_u(u), grad(u2)
+        # the forward value this transpose reads: u2 = B_u(u)
         self.u_bc.table.apply(u, b["u2"], None)
-        N.call("pde_grad_f32", b["u2"].data_ptr(), b["du"].data_ptr(), *grid, 2, g, *self.wc,
-               np.float32(1.0), s)
         # rhs = div(us2) / dt; u_out = us2 - ...; us2 = B_u(us)
         N.call("pde_divergence_adjoint_f32", grhs.data_ptr(), a["gdiv"].data_ptr(), *grid, 1, g,
                *self.wc, np.float32(1.0 / self.dt), s)
@@ -360,22 +357,12 @@ class ProjectionCavity:
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
                a["gus2"].data_ptr(), a["gus2"].numel(), s)
         self.u_bc.table.adjoint(a["gus2"], a["gus"])
-        # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2)
-        N.call("pde_scale_f32", a["gus"].data_ptr(), a["gF"].data_ptr(), a["gF"].numel(),
+        # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2): the transposes of
+        # the Euler update, F, the diffusion, the convection product and grad
+        # in one launch (the values of the separate transposed kernels)
+        N.call("pde_cavity_adjoint_predict_f32", a["gus"].data_ptr(), b["u2"].data_ptr(),
+               a["gu2"].data_ptr(), *grid, g, *self.wl, *self.wc, np.float32(self.nu),
                np.float32(self.dt), s)
-        N.call("pde_scale_f32", a["gF"].data_ptr(), a["gconv"].data_ptr(), a["gF"].numel(),
-               np.float32(-1.0), s)
-        N.call("pde_laplacian_adjoint_f32", a["gF"].data_ptr(), a["glap"].data_ptr(), *grid, 2, g,
-               *self.wl, np.float32(self.nu), s)
-        N.call("pde_outer_f32", a["gconv"].data_ptr(), b["u2"].data_ptr(), a["gdu"].data_ptr(),
-               nodes, 2, 2, s)
-        N.call("pde_matvec_t_f32", b["du"].data_ptr(), a["gconv"].data_ptr(), a["gmv"].data_ptr(),
-               nodes, 2, 2, s)
-        N.call("pde_grad_adjoint_f32", a["gdu"].data_ptr(), a["ggr"].data_ptr(), *grid, 2, g,
-               *self.wc, np.float32(1.0), s)
-        N.call("pde_lincomb_f32", a["gus"].data_ptr(), a["glap"].data_ptr(), a["gmv"].data_ptr(),
-               a["ggr"].data_ptr(), np.float32(1.0), np.float32(1.0), np.float32(1.0),
-               np.float32(1.0), 4, a["gu2"].data_ptr(), a["gu2"].numel(), s)
         # u2 = B_u(u)
         self.u_bc.table.adjoint(a["gu2"], gu)
 

@@ -471,3 +471,37 @@ def test_cavity_correction_fused_equals_boundary_grad_euler(dev, shape, g):
     N.call("pde_cavity_correct_f32", pd.data_ptr(), ud.data_ptr(), got.data_ptr(), *shape, g, *wc,
            dt, st)
     assert same_bits(got.cpu().numpy(), want.cpu().numpy())
+
+
+@pytest.mark.parametrize("shape,g", [((20, 33, 1), 1), ((1, 70, 300), 1), ((15, 12, 1), 2)])
+def test_cavity_adjoint_predictor_fused_equals_separate_kernels(dev, shape, g):
+    """pde_cavity_adjoint_predict_f32 against the chain of separate transposed
+    kernels (scale, scale, Laplacian^T, grad, outer, matvec^T, grad^T,
+    lincomb) on every padded node, bit for bit."""
+    from pde_module_amd.FDM.module.grad import central_weights
+    rng = np.random.default_rng(47)
+    gus = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
+    u2 = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
+    gd, ud = torch.from_numpy(gus).to(dev), torch.from_numpy(u2).to(dev)
+    dx = 0.05
+    wl, wc = O.laplacian_weights(dx), central_weights(dx)
+    nu, dt = np.float32(0.01), np.float32(2e-3)
+    nodes = int(np.prod(shape))
+    z = lambda *e: torch.zeros(shape + e, device=dev)
+    gF, gconv, glap, du, gdu, gmv, ggr, want = z(2), z(2), z(2), z(2, 2), z(2, 2), z(2), z(2), z(2)
+    st = N.stream_of(gd)
+    N.call("pde_scale_f32", gd.data_ptr(), gF.data_ptr(), gF.numel(), dt, st)
+    N.call("pde_scale_f32", gF.data_ptr(), gconv.data_ptr(), gF.numel(), np.float32(-1.0), st)
+    N.call("pde_laplacian_adjoint_f32", gF.data_ptr(), glap.data_ptr(), *shape, 2, g, *wl, nu, st)
+    N.call("pde_grad_f32", ud.data_ptr(), du.data_ptr(), *shape, 2, g, *wc, np.float32(1.0), st)
+    N.call("pde_outer_f32", gconv.data_ptr(), ud.data_ptr(), gdu.data_ptr(), nodes, 2, 2, st)
+    N.call("pde_matvec_t_f32", du.data_ptr(), gconv.data_ptr(), gmv.data_ptr(), nodes, 2, 2, st)
+    N.call("pde_grad_adjoint_f32", gdu.data_ptr(), ggr.data_ptr(), *shape, 2, g, *wc,
+           np.float32(1.0), st)
+    one = np.float32(1.0)
+    N.call("pde_lincomb_f32", gd.data_ptr(), glap.data_ptr(), gmv.data_ptr(), ggr.data_ptr(), one,
+           one, one, one, 4, want.data_ptr(), want.numel(), st)
+    got = torch.full_like(want, 3.0)
+    N.call("pde_cavity_adjoint_predict_f32", gd.data_ptr(), ud.data_ptr(), got.data_ptr(), *shape,
+           g, *wl, *wc, nu, dt, st)
+    assert same_bits(got.cpu().numpy(), want.cpu().numpy())
