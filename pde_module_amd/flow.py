@@ -154,11 +154,9 @@ class ProjectionCavity:
         self.wc = central_weights(self.dx)
         self.h2 = np.float32(self.dx * self.dx)
         self.inv = np.float32(0.25)
-        # work buffers (ghosts of operator outputs stay 0: no kernel writes them)
-        self.b = {k: z(*e) for k, e in (("u2", (2,)), ("du", (2, 2)), ("conv", (2,)),
-                                         ("diff", (2,)), ("F", (2,)), ("us", (2,)),
-                                         ("us2", (2,)), ("rhs", (1,)), ("pf", (1,)),
-                                         ("p0", (1,)), ("gp", (2,)))}
+        # work buffers (the rhs ghosts stay 0: no kernel writes them)
+        self.b = {k: z(*e) for k, e in (("u2", (2,)), ("us", (2,)), ("us2", (2,)),
+                                         ("rhs", (1,)), ("pf", (1,)), ("p0", (1,)))}
         m = np.zeros(self.shape + (1,), np.float32)
         m[self.g:P - self.g, self.g:P - self.g] = 1.0
         self.mask = torch.from_numpy(m).to(self.device)
@@ -277,12 +275,9 @@ class ProjectionCavity:
     def _adjoint_buffers(self):
         if getattr(self, "ga", None) is None:
             z = lambda *e: torch.zeros(self.shape + e, dtype=torch.float32, device=self.device)
-            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("ghp", (1,)),
-                                             ("gpa", (1,)),
+            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("gpa", (1,)),
                                              ("gpb", (1,)), ("grhs", (1,)), ("gdiv", (2,)),
-                                             ("gus2", (2,)), ("gus", (2,)), ("gF", (2,)),
-                                             ("gconv", (2,)), ("glap", (2,)), ("gdu", (2, 2)),
-                                             ("gmv", (2,)), ("ggr", (2,)), ("gu2", (2,)))}
+                                             ("gus2", (2,)))}
         return self.ga
 
     def prepare_adjoint(self, like_u, like_p):
@@ -317,8 +312,8 @@ class ProjectionCavity:
                np.float32(-self.dt), s)
         N.call("pde_grad_adjoint_f32", a["ggp"].data_ptr(), a["gpf"].data_ptr(), *self.shape, 1,
                g, *self.wc, np.float32(1.0), s)
-        self.p_bc.table.adjoint(a["gpf"], a["ghp"])
-        N.call("pde_lincomb_f32", a["ghp"].data_ptr(), gp_out.data_ptr(), None, None,
+        self.p_bc.table.adjoint_inplace(a["gpf"])
+        N.call("pde_lincomb_f32", a["gpf"].data_ptr(), gp_out.data_ptr(), None, None,
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
                gpb.data_ptr(), gpb.numel(), s)
 
@@ -356,15 +351,15 @@ class ProjectionCavity:
         N.call("pde_lincomb_f32", gu_out.data_ptr(), a["gdiv"].data_ptr(), None, None,
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
                a["gus2"].data_ptr(), a["gus2"].numel(), s)
-        self.u_bc.table.adjoint(a["gus2"], a["gus"])
+        self.u_bc.table.adjoint_inplace(a["gus2"])               # now the gradient of us
         # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2): the transposes of
         # the Euler update, F, the diffusion, the convection product and grad
         # in one launch (the values of the separate transposed kernels)
-        N.call("pde_cavity_adjoint_predict_f32", a["gus"].data_ptr(), b["u2"].data_ptr(),
-               a["gu2"].data_ptr(), *grid, g, *self.wl, *self.wc, np.float32(self.nu),
+        N.call("pde_cavity_adjoint_predict_f32", a["gus2"].data_ptr(), b["u2"].data_ptr(),
+               gu.data_ptr(), *grid, g, *self.wl, *self.wc, np.float32(self.nu),
                np.float32(self.dt), s)
         # u2 = B_u(u)
-        self.u_bc.table.adjoint(a["gu2"], gu)
+        self.u_bc.table.adjoint_inplace(gu)
 
     def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
         """J(u_N) = 1/2 sum |u_N|^2 dx^2 (interior), differentiable in u0 and
