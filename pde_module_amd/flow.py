@@ -435,12 +435,13 @@ class CavityRollout:
         self._run("fwd", lambda: self.cav.step(self.U, self.P, self.U, self.P))
 
     def _adj(self):
-        def body():
-            self.cav.step_adjoint(self.AU, self.gU, self.gP, self.gU2, self.gP2)
-            st = N.stream_of(self.gU)
-            N.call("pde_copy_f32", self.gU2.data_ptr(), self.gU.data_ptr(), self.gU.numel(), st)
-            N.call("pde_copy_f32", self.gP2.data_ptr(), self.gP.data_ptr(), self.gP.numel(), st)
-        self._run("adj", body)
+        """One transposed step from the gradient pair of parity self._par into
+        the other pair (two graphs, one per direction, no copies)."""
+        src = (self.gU, self.gP) if self._par == 0 else (self.gU2, self.gP2)
+        dst = (self.gU2, self.gP2) if self._par == 0 else (self.gU, self.gP)
+        self._run(("adj", self._par),
+                  lambda: self.cav.step_adjoint(self.AU, src[0], src[1], dst[0], dst[1]))
+        self._par ^= 1
 
     def _run(self, key, fn):
         if not self.use_graph:
@@ -480,6 +481,7 @@ class CavityRollout:
                                      f"state is {tuple(ref.shape)}")
         self.gU.copy_(guN)
         self.gP.copy_(gpN)
+        self._par = 0
         nseg = (self.steps + self.K - 1) // self.K
         for sg in reversed(range(nseg)):
             n0 = sg * self.K
@@ -493,7 +495,8 @@ class CavityRollout:
             for i in reversed(range(L)):
                 self.AU.copy_(self.hist[n0 + i] if self.store_all else self.hist[i])
                 self._adj()
-        return self.gU.clone(), self.gP.clone()
+        gu, gp = (self.gU, self.gP) if self._par == 0 else (self.gU2, self.gP2)
+        return gu.clone(), gp.clone()
 
 
 class _CavityRolloutFn(torch.autograd.Function):
