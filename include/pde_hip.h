@@ -558,6 +558,22 @@ int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2, float *gu2
                                    float lap_wc, float lap_wp, float grad_wm, float grad_wc,
                                    float grad_wp, float nu, float dt, void *stream);
 
+/* The correction's transpose in the pressure, up to the boundary's
+ * transpose, in one launch: gpf = grad^T(-dt * gu_out) (gpf scalar, gu_out
+ * vec2, 2D), the values of pde_scale_f32(-dt) + pde_grad_adjoint_f32
+ * (alpha 1); every node of gpf written.  No reference counterpart. */
+int pde_cavity_adjoint_correct_f32(const float *gu_out, float *gpf, int64_t nx, int64_t ny,
+                                   int64_t nz, int32_t ghost, float grad_wm, float grad_wc,
+                                   float grad_wp, float dt, void *stream);
+
+/* gus2 = gu_out + div^T(grhs) (alpha = the divergence's scale, 1/dt in the
+ * cavity) on a 2D grid (grhs scalar, gu_out and gus2 vec2) in one launch:
+ * the values of pde_divergence_adjoint_f32 + pde_lincomb_f32 (1, 1); gus2
+ * may alias gu_out.  No reference counterpart. */
+int pde_cavity_adjoint_div_f32(const float *grhs, const float *gu_out, float *gus2, int64_t nx,
+                               int64_t ny, int64_t nz, int32_t ghost, float grad_wm,
+                               float grad_wc, float grad_wp, float alpha, void *stream);
+
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */
 int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes, int32_t rows,

@@ -79,6 +79,8 @@ SIGNATURES = {
     "pde_cavity_predict_f32": [_P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F, _F, _F, _F,
                                _P],
     "pde_cavity_correct_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _P],
+    "pde_cavity_adjoint_correct_f32": [_P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _P],
+    "pde_cavity_adjoint_div_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _P],
     "pde_cavity_adjoint_predict_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
                                        _F, _F, _F, _P],
     "pde_jacobi_neumann0_adjoint_x2_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],

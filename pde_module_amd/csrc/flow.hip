@@ -644,73 +644,142 @@ __global__ __launch_bounds__(256) void cavity_adj_predict_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
-    const int64_t R = 2 * P2;    // floats per row
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 *G2 = reinterpret_cast<const f2 *>(gus);
+    const f2 *U2 = reinterpret_cast<const f2 *>(u2);
     auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
     auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
+    const int64_t km = k > 0 ? k - 1 : k, kp = k + 1 < P2 ? k + 1 : k;
     for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
-        const int64_t n = x * P2 + k;
         const bool ix = inx(x), iy = ink(k);
-        // gF and gconv at a node (floats e, e + 1)
-        auto gF = [&](int64_t e, int c) { return dt * gus[e + c]; };
-        auto gconv = [&](int64_t e, int c) { return -1.0f * gF(e, c); };
-        // gdu(node, c, t) = gconv(node, c) * u2(node, t)
-        auto gdu = [&](int64_t e, int c, int t) { return gconv(e, c) * u2[e + t]; };
-        const int64_t e = 2 * n;
-        // forward grad u2 at this node (du[c][t])
+        const bool xp_in = inx(x + 1), xm_in = inx(x - 1), kp_in = ink(k + 1), km_in = ink(k - 1);
+        // the node and its four neighbours, each loaded once (clamped to the
+        // array; a value is used only where the node it stands for is interior)
+        const int64_t xm = x > 0 ? x - 1 : x, xq = x + 1 < P1 ? x + 1 : x;
+        const int64_t n = x * P2 + k;
+        const f2 g0 = G2[n], gxm = G2[xm * P2 + k], gxp = G2[xq * P2 + k];
+        const f2 gym = G2[x * P2 + km], gyp = G2[x * P2 + kp];
+        const f2 u0 = U2[n], uxm = U2[xm * P2 + k], uxp = U2[xq * P2 + k];
+        const f2 uym = U2[x * P2 + km], uyp = U2[x * P2 + kp];
+        auto cmp = [](const f2 &v, int c) { return c == 0 ? v.x : v.y; };
+        // forward grad u2 here (du[c][t]), 0 off the interior
         float du[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
         if (ix && iy) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 float a = 0.0f;
-                a += gm * u2[e - R + c];
-                a += gc * u2[e + c];
-                a += gp * u2[e + R + c];
+                a += gm * cmp(uxm, c);
+                a += gc * cmp(u0, c);
+                a += gp * cmp(uxp, c);
                 du[c][0] = 1.0f * a;
                 a = 0.0f;
-                a += gm * u2[e - 2 + c];
-                a += gc * u2[e + c];
-                a += gp * u2[e + 2 + c];
+                a += gm * cmp(uym, c);
+                a += gc * cmp(u0, c);
+                a += gp * cmp(uyp, c);
                 du[c][1] = 1.0f * a;
             }
         }
-        const float gc0 = gconv(e, 0), gc1 = gconv(e, 1);
+        // gF = dt gus, gconv = -1 gF, gdu(c, t) = gconv(c) u2(t), at any node
+        auto gF = [&](const f2 &gv, int c) { return dt * cmp(gv, c); };
+        auto gcv = [&](const f2 &gv, int c) { return -1.0f * gF(gv, c); };
+        const float gc0 = gcv(g0, 0), gc1 = gcv(g0, 1);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            // lap^T(gF)
             float al = 0.0f;
             if (iy) {
-                if (inx(x + 1)) al += lm * gF(e + R, c);
-                if (ix) al += lc * gF(e, c);
-                if (inx(x - 1)) al += lp * gF(e - R, c);
+                if (xp_in) al += lm * gF(gxp, c);
+                if (ix) al += lc * gF(g0, c);
+                if (xm_in) al += lp * gF(gxm, c);
             }
             if (ix) {
-                if (ink(k + 1)) al += lm * gF(e + 2, c);
-                if (iy) al += lc * gF(e, c);
-                if (ink(k - 1)) al += lp * gF(e - 2, c);
+                if (kp_in) al += lm * gF(gyp, c);
+                if (iy) al += lc * gF(g0, c);
+                if (km_in) al += lp * gF(gym, c);
             }
             const float glap = nu * al;
-            // (grad u2)^T gconv
             float gmv = du[0][c] * gc0;
             gmv = gmv + du[1][c] * gc1;
-            // grad^T(gdu)
             float ag = 0.0f;
             if (iy) {
-                if (inx(x + 1)) ag += gm * gdu(e + R, c, 0);
-                if (ix) ag += gc * gdu(e, c, 0);
-                if (inx(x - 1)) ag += gp * gdu(e - R, c, 0);
+                if (xp_in) ag += gm * (gcv(gxp, c) * uxp.x);
+                if (ix) ag += gc * (gcv(g0, c) * u0.x);
+                if (xm_in) ag += gp * (gcv(gxm, c) * uxm.x);
             }
             if (ix) {
-                if (ink(k + 1)) ag += gm * gdu(e + 2, c, 1);
-                if (iy) ag += gc * gdu(e, c, 1);
-                if (ink(k - 1)) ag += gp * gdu(e - 2, c, 1);
+                if (kp_in) ag += gm * (gcv(gyp, c) * uyp.y);
+                if (iy) ag += gc * (gcv(g0, c) * u0.y);
+                if (km_in) ag += gp * (gcv(gym, c) * uym.y);
             }
             const float ggr = 1.0f * ag;
-            float acc = 1.0f * gus[e + c];
+            float acc = 1.0f * cmp(g0, c);
             acc = acc + 1.0f * glap;
             acc = acc + 1.0f * gmv;
             acc = acc + 1.0f * ggr;
-            gu2[e + c] = acc;
+            gu2[2 * n + c] = acc;
         }
+    }
+}
+
+// Transpose of the correction's pressure part, up to the boundary's
+// transpose: gpf = 1 * grad^T((-dt) gu_out) on a 2D scalar pressure field,
+// with scale_kernel's product and grad_adj_kernel's gather (N = 1) order.
+__global__ __launch_bounds__(256) void cavity_adj_correct_kernel(
+    const float *__restrict__ gu_out, float *__restrict__ gpf, int64_t P1, int64_t P2, int64_t g,
+    float gm, float gc, float gp, float ndt)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
+    auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
+    auto gg = [&](int64_t node, int t) { return ndt * gu_out[2 * node + t]; };
+    for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
+        const int64_t n = x * P2 + k;
+        const bool ix = inx(x), iy = ink(k);
+        float acc = 0.0f;
+        if (iy) {
+            if (inx(x + 1)) acc += gm * gg(n + P2, 0);
+            if (ix) acc += gc * gg(n, 0);
+            if (inx(x - 1)) acc += gp * gg(n - P2, 0);
+        }
+        if (ix) {
+            if (ink(k + 1)) acc += gm * gg(n + 1, 1);
+            if (iy) acc += gc * gg(n, 1);
+            if (ink(k - 1)) acc += gp * gg(n - 1, 1);
+        }
+        gpf[n] = 1.0f * acc;
+    }
+}
+
+// gus2 = 1 gu_out + 1 div^T(grhs) (alpha = 1/dt) on a 2D vec2 field, with
+// div_adj_kernel's order (rows = 1) and lincomb_kernel's sum.
+__global__ __launch_bounds__(256) void cavity_adj_div_kernel(
+    const float *__restrict__ grhs, const float *__restrict__ gu_out, float *__restrict__ gus2,
+    int64_t P1, int64_t P2, int64_t g, float gm, float gc, float gp, float alpha)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= P2) return;
+    auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
+    auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
+    for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
+        const int64_t n = x * P2 + k;
+        const bool ix = inx(x), iy = ink(k);
+        float a0 = 0.0f, a1 = 0.0f;
+        if (iy) {
+            if (inx(x + 1)) a0 += gm * grhs[n + P2];
+            if (ix) a0 += gc * grhs[n];
+            if (inx(x - 1)) a0 += gp * grhs[n - P2];
+        }
+        if (ix) {
+            if (ink(k + 1)) a1 += gm * grhs[n + 1];
+            if (iy) a1 += gc * grhs[n];
+            if (ink(k - 1)) a1 += gp * grhs[n - 1];
+        }
+        float v0 = 1.0f * gu_out[2 * n];
+        v0 = v0 + 1.0f * (alpha * a0);
+        float v1 = 1.0f * gu_out[2 * n + 1];
+        v1 = v1 + 1.0f * (alpha * a1);
+        gus2[2 * n] = v0;
+        gus2[2 * n + 1] = v1;
     }
 }
 
@@ -956,6 +1025,38 @@ extern "C" int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2,
     hipLaunchKernelGGL(cavity_adj_predict_kernel, grid, dim3(256), 0, as_stream(stream), gus, u2,
                        gu2, G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm,
                        grad_wc, grad_wp, nu, dt);
+    return launch_status();
+}
+
+extern "C" int pde_cavity_adjoint_correct_f32(const float *gu_out, float *gpf, int64_t nx,
+                                              int64_t ny, int64_t nz, int32_t ghost,
+                                              float grad_wm, float grad_wc, float grad_wp,
+                                              float dt, void *stream)
+{
+    if (!gu_out || !gpf) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 2) return PDE_ESHAPE;
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
+    hipLaunchKernelGGL(cavity_adj_correct_kernel, grid, dim3(256), 0, as_stream(stream), gu_out,
+                       gpf, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
+    return launch_status();
+}
+
+extern "C" int pde_cavity_adjoint_div_f32(const float *grhs, const float *gu_out, float *gus2,
+                                          int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                          float grad_wm, float grad_wc, float grad_wp,
+                                          float alpha, void *stream)
+{
+    if (!grhs || !gu_out || !gus2 || gus2 == grhs) return PDE_EINVAL;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim != 2) return PDE_ESHAPE;
+    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
+    hipLaunchKernelGGL(cavity_adj_div_kernel, grid, dim3(256), 0, as_stream(stream), grhs, gu_out,
+                       gus2, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, alpha);
     return launch_status();
 }
 

@@ -275,9 +275,8 @@ class ProjectionCavity:
     def _adjoint_buffers(self):
         if getattr(self, "ga", None) is None:
             z = lambda *e: torch.zeros(self.shape + e, dtype=torch.float32, device=self.device)
-            self.ga = {k: z(*e) for k, e in (("ggp", (2,)), ("gpf", (1,)), ("gpa", (1,)),
-                                             ("gpb", (1,)), ("grhs", (1,)), ("gdiv", (2,)),
-                                             ("gus2", (2,)))}
+            self.ga = {k: z(*e) for k, e in (("gpf", (1,)), ("gpa", (1,)), ("gpb", (1,)),
+                                             ("grhs", (1,)), ("gus2", (2,)))}
         return self.ga
 
     def prepare_adjoint(self, like_u, like_p):
@@ -297,21 +296,27 @@ class ProjectionCavity:
         self.check_fields((u, gu_out, gu), (gp_out, gp))
         a = self._adjoint_buffers()
         self.prepare_adjoint(u, gp)
-        self._adj_correct(gu_out, gp_out, a["gpb"])
+        first, spare = self._sweep_buffers(gp, a["gpb"], a["gpa"])
+        self._adj_correct(gu_out, gp_out, first)
         a["grhs"].zero_()
-        cur = self._adj_sweeps(a["gpb"], a["gpa"], a["grhs"], 1)
-        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), N.stream_of(gp))
+        self._adj_sweeps(first, spare, a["grhs"], 1)          # ends in gp
         self._adj_predict(u, gu_out, a["grhs"], gu)
         return gu, gp
+
+    def _sweep_launches(self):
+        return self.sweeps // 2 + self.sweeps % 2
+
+    def _sweep_buffers(self, gp, b0, b1):
+        """(first, spare) for the transposed sweeps so that the last launch
+        writes gp: the launches alternate first -> spare -> first ..."""
+        return (gp, b1) if self._sweep_launches() % 2 == 0 else (b0, gp)
 
     def _adj_correct(self, gu_out, gp_out, gpb):
         """Transpose of the correction u_out = us2 - dt * grad(B_p(p_out)) in
         p_out: gpb = B_p^T(grad^T(-dt * gu_out)) + gp_out."""
         a, g, s = self.ga, self.g, N.stream_of(gu_out)
-        N.call("pde_scale_f32", gu_out.data_ptr(), a["ggp"].data_ptr(), gu_out.numel(),
-               np.float32(-self.dt), s)
-        N.call("pde_grad_adjoint_f32", a["ggp"].data_ptr(), a["gpf"].data_ptr(), *self.shape, 1,
-               g, *self.wc, np.float32(1.0), s)
+        N.call("pde_cavity_adjoint_correct_f32", gu_out.data_ptr(), a["gpf"].data_ptr(),
+               *self.shape, g, *self.wc, np.float32(self.dt), s)
         self.p_bc.table.adjoint_inplace(a["gpf"])
         N.call("pde_lincomb_f32", a["gpf"].data_ptr(), gp_out.data_ptr(), None, None,
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
@@ -346,11 +351,8 @@ class ProjectionCavity:
         # the forward value this transpose reads: u2 = B_u(u)
         self.u_bc.table.apply(u, b["u2"], None)
         # rhs = div(us2) / dt; u_out = us2 - ...; us2 = B_u(us)
-        N.call("pde_divergence_adjoint_f32", grhs.data_ptr(), a["gdiv"].data_ptr(), *grid, 1, g,
-               *self.wc, np.float32(1.0 / self.dt), s)
-        N.call("pde_lincomb_f32", gu_out.data_ptr(), a["gdiv"].data_ptr(), None, None,
-               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
-               a["gus2"].data_ptr(), a["gus2"].numel(), s)
+        N.call("pde_cavity_adjoint_div_f32", grhs.data_ptr(), gu_out.data_ptr(),
+               a["gus2"].data_ptr(), *grid, g, *self.wc, np.float32(1.0 / self.dt), s)
         self.u_bc.table.adjoint_inplace(a["gus2"])               # now the gradient of us
         # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2): the transposes of
         # the Euler update, F, the diffusion, the convection product and grad
@@ -558,11 +560,11 @@ class ProjectionCavityBatch:
         cav = self.cav
         cav.check_fields((u, gu_out, gu), (gp_out, gp), (self.batch,))
         self.prepare_adjoint(u, gp)
+        first, spare = cav._sweep_buffers(gp, self.gpb, self.gpa)
         for i in range(self.batch):
-            cav._adj_correct(gu_out[i], gp_out[i], self.gpb[i])
+            cav._adj_correct(gu_out[i], gp_out[i], first[i])
         self.grhs.zero_()
-        cur = cav._adj_sweeps(self.gpb, self.gpa, self.grhs, self.batch)
-        N.call("pde_copy_f32", cur.data_ptr(), gp.data_ptr(), gp.numel(), N.stream_of(gp))
+        cav._adj_sweeps(first, spare, self.grhs, self.batch)     # ends in gp
         for i in range(self.batch):
             cav._adj_predict(u[i], gu_out[i], self.grhs[i], gu[i])
         return gu, gp

@@ -505,3 +505,35 @@ def test_cavity_adjoint_predictor_fused_equals_separate_kernels(dev, shape, g):
     N.call("pde_cavity_adjoint_predict_f32", gd.data_ptr(), ud.data_ptr(), got.data_ptr(), *shape,
            g, *wl, *wc, nu, dt, st)
     assert same_bits(got.cpu().numpy(), want.cpu().numpy())
+
+
+@pytest.mark.parametrize("shape,g", [((20, 33, 1), 1), ((1, 70, 300), 1), ((15, 12, 1), 2)])
+def test_cavity_adjoint_correct_and_div_fused_equal_chains(dev, shape, g):
+    """pde_cavity_adjoint_correct_f32 == scale(-dt) + grad transpose, and
+    pde_cavity_adjoint_div_f32 == divergence transpose + lincomb, on every
+    padded node, bit for bit."""
+    from pde_module_amd.FDM.module.grad import central_weights
+    rng = np.random.default_rng(53)
+    wc = central_weights(0.05)
+    dt = np.float32(2e-3)
+    alpha = np.float32(1.0 / dt)
+    gu = torch.from_numpy(rng.uniform(-1, 1, shape + (2,)).astype(np.float32)).to(dev)
+    gr = torch.from_numpy(rng.uniform(-1, 1, shape + (1,)).astype(np.float32)).to(dev)
+    st = N.stream_of(gu)
+    ggp, want_p = torch.empty_like(gu), torch.empty_like(gr)
+    N.call("pde_scale_f32", gu.data_ptr(), ggp.data_ptr(), gu.numel(), np.float32(-dt), st)
+    N.call("pde_grad_adjoint_f32", ggp.data_ptr(), want_p.data_ptr(), *shape, 1, g, *wc,
+           np.float32(1.0), st)
+    got_p = torch.full_like(gr, 4.0)
+    N.call("pde_cavity_adjoint_correct_f32", gu.data_ptr(), got_p.data_ptr(), *shape, g, *wc, dt,
+           st)
+    assert same_bits(got_p.cpu().numpy(), want_p.cpu().numpy())
+    gdiv, want_u = torch.empty_like(gu), torch.empty_like(gu)
+    N.call("pde_divergence_adjoint_f32", gr.data_ptr(), gdiv.data_ptr(), *shape, 1, g, *wc, alpha,
+           st)
+    N.call("pde_lincomb_f32", gu.data_ptr(), gdiv.data_ptr(), None, None, np.float32(1.0),
+           np.float32(1.0), np.float32(0), np.float32(0), 2, want_u.data_ptr(), want_u.numel(), st)
+    got_u = torch.full_like(gu, 4.0)
+    N.call("pde_cavity_adjoint_div_f32", gr.data_ptr(), gu.data_ptr(), got_u.data_ptr(), *shape, g,
+           *wc, alpha, st)
+    assert same_bits(got_u.cpu().numpy(), want_u.cpu().numpy())
