@@ -155,8 +155,8 @@ class ProjectionCavity:
         self.h2 = np.float32(self.dx * self.dx)
         self.inv = np.float32(0.25)
         # work buffers (the rhs ghosts stay 0: no kernel writes them)
-        self.b = {k: z(*e) for k, e in (("u2", (2,)), ("us", (2,)), ("us2", (2,)),
-                                         ("rhs", (1,)), ("pf", (1,)), ("p0", (1,)))}
+        self.b = {k: z(*e) for k, e in (("u2", (2,)), ("us2", (2,)), ("rhs", (1,)),
+                                         ("pf", (1,)), ("p0", (1,)))}
         m = np.zeros(self.shape + (1,), np.float32)
         m[self.g:P - self.g, self.g:P - self.g] = 1.0
         self.mask = torch.from_numpy(m).to(self.device)
@@ -177,22 +177,28 @@ class ProjectionCavity:
         be u / p.  All work on the current stream, no allocation."""
         self.check_fields((u, u_out), (p, p_out))
         b = self.b
-        self._predict(u, b["us2"], b["rhs"])
+        self._predict(u, b["us2"], b["rhs"], u_scratch=u is u_out)
         self._sweeps(p, b["rhs"], p_out, b["pf"], b["p0"], 1)
         self._correct(p_out, b["us2"], u_out)
         return u_out, p_out
 
-    def _predict(self, u, us2, rhs):
-        """u -> the boundary-fixed predictor us2 and the pressure source rhs."""
+    def _predict(self, u, us2, rhs, u_scratch=False):
+        """u -> the boundary-fixed predictor us2 and the pressure source rhs.
+        With u_scratch (u is the step's output, overwritten at its end) the
+        velocity boundary is applied to u in place instead of into a copy."""
         b, g, s = self.b, self.g, N.stream_of(u)
         grid = self.shape
         self.u_bc._ensure_table(u)
-        self.u_bc.table.apply(u, b["u2"], None)
+        if u_scratch:
+            u2 = self.u_bc.table.apply_inplace(u, None)
+        else:
+            u2 = self.u_bc.table.apply(u, b["u2"], None)
         # grad, convection product, diffusion, F and the Euler update in one
-        # launch (the values of the five separate kernels)
-        N.call("pde_cavity_predict_f32", b["u2"].data_ptr(), b["us"].data_ptr(), *grid, g,
+        # launch (the values of the five separate kernels), then the boundary
+        # on that result in place
+        N.call("pde_cavity_predict_f32", u2.data_ptr(), us2.data_ptr(), *grid, g,
                *self.wl, *self.wc, np.float32(self.nu), np.float32(self.dt), s)
-        self.u_bc.table.apply(b["us"], us2, None)
+        self.u_bc.table.apply_inplace(us2, None)
         N.call("pde_divergence_f32", us2.data_ptr(), rhs.data_ptr(), *grid, 1, g, *self.wc,
                np.float32(1.0 / self.dt), s)
 
@@ -540,7 +546,7 @@ class ProjectionCavityBatch:
         cav = self.cav
         cav.check_fields((u, u_out), (p, p_out), (self.batch,))
         for i in range(self.batch):
-            cav._predict(u[i], self.us2[i], self.rhs[i])
+            cav._predict(u[i], self.us2[i], self.rhs[i], u_scratch=u is u_out)
         cav._sweeps(p, self.rhs, p_out, self.pf, self.p0, self.batch)
         for i in range(self.batch):
             cav._correct(p_out[i], self.us2[i], u_out[i])
