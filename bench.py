@@ -126,10 +126,11 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     u0 = torch.zeros_like(c2)
     v0 = torch.zeros_like(c2)
     bc = LegacyGridBoundary(u0, cfg.dx, cfg.g)
-    mk = lambda k: WaveRollout(bc, cfg.dx, cfg.g, float(cfg.dt), steps, checkpoint_every=k,
-                               src_idx=T(cfg.src_idx), src_wavelet=T(cfg.wavelet),
-                               rec_idx=T(cfg.rec_idx), use_graph=True)
-    plain, ro, ro_all = mk(steps), mk(K), mk(None)
+    mk = lambda k, rev=False: WaveRollout(bc, cfg.dx, cfg.g, float(cfg.dt), steps,
+                                          checkpoint_every=k, src_idx=T(cfg.src_idx),
+                                          src_wavelet=T(cfg.wavelet), rec_idx=T(cfg.rec_idx),
+                                          use_graph=True, reverse=rev)
+    plain, ro, ro_all, ro_rev = mk(steps), mk(K), mk(None), mk(K, True)
     _, _, d = plain.forward_states(torch.ones_like(c2), u0, v0)
     d = d.clone()
 
@@ -152,7 +153,7 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
         _, _, r = ro(c, u0, v0)
         J = ((r - d) ** 2).sum()
         J.backward()
-        out["J"], out["g"] = float(J), float(c.grad.abs().max())
+        out["J"], out["g"] = float(J.detach()), float(c.grad.abs().max())
 
     t_fa = timed(fwd_adj)
 
@@ -162,10 +163,24 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
         ((r - d) ** 2).sum().backward()
 
     t_all = timed(fwd_adj_all, reps=2)
+
+    def fwd_adj_rev():
+        c = c2.clone().requires_grad_(True)
+        _, _, r = ro_rev(c, u0, v0)
+        J = ((r - d) ** 2).sum()
+        J.backward()
+        out["g_rev"] = float(c.grad.abs().max())
+
+    t_rev = timed(fwd_adj_rev)
     return {"grid": [cfg.n, cfg.n], "steps": steps, "checkpoint_every": K,
             "fwd_ms_per_step": round(t_fwd / steps * 1e3, 5),
             "fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
             "ratio": round(t_fa / t_fwd, 3), "J": out["J"], "max_abs_dJ_dc2": out["g"],
+            "reversed": {"fwd_adj_ms_per_step": round(t_rev / steps * 1e3, 5),
+                         "ratio": round(t_rev / t_fwd, 3),
+                         "note": "checkpoints every K steps; the backward walks each segment "
+                                 "back from its end (two transposed steps per launch, states "
+                                 "rebuilt, boundary scalars restored) instead of recomputing it"},
             "store_all": {"fwd_adj_ms_per_step": round(t_all / steps * 1e3, 5),
                           "ratio": round(t_all / t_fwd, 3),
                           "history_GB": round((steps + 1) * c2.numel() * 4 / 1e9, 1)},

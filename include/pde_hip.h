@@ -321,6 +321,59 @@ int pde_wave_step_adjoint_fused_f32(const float *gun, const float *gvn, const fl
  * alternating counters of the fused steps).  No reference counterpart. */
 int pde_counter_copy(const int64_t *src, int64_t *dst, int64_t delta, void *stream);
 
+/* Boundary ring of a 2D field for pde_wave_reverse_adjoint_pair_f32 (host
+ * built, device arrays).  The ring is every node less than `depth` from the
+ * field's edge; it is cut into pieces (output rectangles rows [r0, r1) x
+ * columns [c0, c1), at most 32 along the edge).  Entries refer to flat node
+ * indices; for a piece they cover its rectangle grown by one node:
+ *   w[w_beg, w_end):  boundary-written nodes (node, a = save slot)
+ *   t[t_beg, t_end):  boundary-transpose sources (node, a..b = range of
+ *                     cdst/csign, flags bit 0: the source is itself written)
+ *   z[z_beg, z_end):  written nodes that are no source (node)
+ * flags bit 1 marks entries inside the piece's own rectangle. */
+typedef struct pde_ring_piece {
+    int32_t r0, r1, c0, c1;
+    int32_t w_beg, w_end, t_beg, t_end, z_beg, z_end;
+    int32_t reserved[2];
+} pde_ring_piece; /* 48 bytes */
+
+typedef struct pde_ring_entry {
+    int32_t node, a, b, flags;
+} pde_ring_entry; /* 16 bytes */
+
+typedef struct pde_ring_plan {
+    int32_t depth, npieces;
+    const pde_ring_piece *pieces;
+    const pde_ring_entry *w, *t, *z;
+    const int32_t *cdst;
+    const float *csign;
+} pde_ring_plan;
+
+/* Two transposed wave steps, n then n - 1 (n = *step, *step_next = n - 2),
+ * walking the forward states backwards instead of reading them: from the
+ * step-(n+1) state (u, v: pre-boundary u, the v the step wrote) it rebuilds
+ * u2 = u - dt*v and v_n = v - dt*c2*alpha*lap(u2) - source on interior
+ * nodes, restores the boundary-written scalars from save (rows n and n-1, as
+ * recorded by pde_bc_apply_chunked_inplace_save_f32), and runs the transposes
+ * of pde_wave_step_adjoint_fused_f32 and of the boundary in between, so
+ * (uo, vo) is the step-(n-1) state and (guo, gvo) the gradient with respect
+ * to it; gc2 (may be NULL) accumulates both steps' c2 terms in place.  Nodes
+ * at least ring->depth from the edge run as register row tiles, the ring as
+ * LDS pieces, all in one launch.  The gradients g_u, g_v are those of two
+ * pde_wave_step_adjoint_fused_f32 + boundary-transpose steps bit for bit; gc2
+ * differs from the recomputed-state result only through the rounding of the
+ * rebuilt states.  Never the first step of the loop (n >= 1).  No reference
+ * counterpart. */
+int pde_wave_reverse_adjoint_pair_f32(const float *u, const float *v, const float *gu,
+                                      const float *gv, const float *c2, float *uo, float *vo,
+                                      float *guo, float *gvo, float *gc2, int64_t nx,
+                                      int64_t ny, int64_t nz, int32_t ghost, float w_minus,
+                                      float w_center, float w_plus, float alpha, float dt,
+                                      const int64_t *step, int64_t *step_next,
+                                      const pde_point_set *sources,
+                                      const pde_point_set *receivers, const float *save,
+                                      int64_t nops, const pde_ring_plan *ring, void *stream);
+
 /* pde_bc_apply_inplace_f32 in one full-chip launch.  The host orders the ops
  * in conflict-closed chunks: chunk c is ops [chunk_start[c], chunk_start[c+1])
  * (at most 256), and no op reads a scalar that an op of another chunk writes;
@@ -328,6 +381,15 @@ int pde_counter_copy(const int64_t *src, int64_t *dst, int64_t delta, void *stre
  * own.  No reference counterpart (the in-place form of the boundary apply). */
 int pde_bc_apply_chunked_inplace_f32(float *u, const pde_bc_op *ops, const int64_t *chunk_start,
                                      int64_t nchunks, const float *t, void *stream);
+
+/* pde_bc_apply_chunked_inplace_f32 that also records, for op m (chunk order),
+ * the value its destination held before the apply in save[n * nops + m],
+ * n = *step.  The wave loop's time-reversed gradient rebuilds each step's
+ * pre-boundary field from these.  No reference counterpart. */
+int pde_bc_apply_chunked_inplace_save_f32(float *u, const pde_bc_op *ops,
+                                          const int64_t *chunk_start, int64_t nchunks,
+                                          int64_t nops, const float *t, float *save,
+                                          const int64_t *step, void *stream);
 
 /* pde_bc_adjoint_inplace_f32 in one full-chip launch over conflict-closed
  * chunks: chunk c stages sources [item_start[c], item_start[c+1]) (at most

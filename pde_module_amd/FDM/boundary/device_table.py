@@ -72,6 +72,28 @@ class DeviceBoundaryTable:
                N.stream_of(u))
         return u
 
+    def apply_inplace_save(self, u: torch.Tensor, t: torch.Tensor | None, save: torch.Tensor,
+                           step_ptr: int) -> torch.Tensor:
+        """``apply_inplace`` that also records the value every written scalar
+        held before the apply, op by op in table order, in row ``*step_ptr`` (a
+        device int64) of ``save`` (rows of ``nops`` floats).  Chunked tables
+        only; a table without ops writes nothing."""
+        N.check_field(u, "field")
+        self._check_bounds(u.numel())
+        if not self.nops:
+            return u
+        if self.chunk_start is None:
+            raise ValueError("recording the overwritten scalars needs a chunked boundary table")
+        if save.dtype != torch.float32 or save.device != u.device or save.dim() != 2 \
+                or save.shape[1] != self.nops or not save.is_contiguous():
+            raise AssertionError(f"save must be a contiguous float32 (rows, {self.nops}) array "
+                                 "on the field's device")
+        N.call("pde_bc_apply_chunked_inplace_save_f32", N.dev_ptr(u), self.ops.data_ptr(),
+               self.chunk_start.data_ptr(), self.nchunks, self.nops,
+               t.data_ptr() if t is not None else None, save.data_ptr(), step_ptr,
+               N.stream_of(u))
+        return u
+
     def prepare_adjoint(self) -> None:
         """Uploads the transpose arrays (once); call before capturing a graph
         that applies the transpose."""

@@ -52,6 +52,10 @@ SIGNATURES = {
     "pde_affine_f32": [_P, _P, _I64, _I32, _I32, _F, _F, _P],
     "pde_bc_adjoint_inplace_f32": [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P],
     "pde_bc_apply_chunked_inplace_f32": [_P, _P, _P, _I64, _P, _P],
+    "pde_bc_apply_chunked_inplace_save_f32": [_P, _P, _P, _I64, _I64, _P, _P, _P, _P],
+    "pde_wave_reverse_adjoint_pair_f32": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64,
+                                          _I64, _I32, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P,
+                                          _I64, _P, _P],
     "pde_bc_adjoint_chunked_inplace_f32": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "pde_gather_f32": [_P, _P, _P, _I64, _P],
     "pde_wave_step_ctr_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
@@ -103,6 +107,19 @@ BATCH_GRID_DTYPE = np.dtype([("offset", "<i8"), ("op_begin", "<i8"), ("op_end", 
                              ("reserved0", "<i4"), ("wm", "<f4"), ("wc", "<f4"), ("wp", "<f4"),
                              ("alpha", "<f4"), ("dt", "<f4"), ("reserved1", "<f4")])
 assert BATCH_GRID_DTYPE.itemsize == 64
+
+# struct pde_ring_piece / pde_ring_entry / pde_ring_plan
+RING_PIECE_DTYPE = np.dtype([("r0", "<i4"), ("r1", "<i4"), ("c0", "<i4"), ("c1", "<i4"),
+                             ("w_beg", "<i4"), ("w_end", "<i4"), ("t_beg", "<i4"),
+                             ("t_end", "<i4"), ("z_beg", "<i4"), ("z_end", "<i4"),
+                             ("reserved", "<i4", (2,))])
+assert RING_PIECE_DTYPE.itemsize == 48
+RING_ENTRY_DTYPE = np.dtype([("node", "<i4"), ("a", "<i4"), ("b", "<i4"), ("flags", "<i4")])
+assert RING_ENTRY_DTYPE.itemsize == 16
+RING_PLAN_DTYPE = np.dtype([("depth", "<i4"), ("npieces", "<i4"), ("pieces", "<u8"),
+                            ("w", "<u8"), ("t", "<u8"), ("z", "<u8"), ("cdst", "<u8"),
+                            ("csign", "<u8")])
+assert RING_PLAN_DTYPE.itemsize == 56
 
 # struct pde_point_set (pointers as integers)
 POINT_SET_DTYPE = np.dtype([("row_start", "<u8"), ("col", "<u8"), ("slot", "<u8"),

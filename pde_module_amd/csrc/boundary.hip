@@ -56,9 +56,14 @@ __global__ __launch_bounds__(1024) void bc_inplace_small_kernel(float *__restric
 // another chunk writes, so each workgroup applies its chunk in place on its
 // own (all reads, workgroup barrier, all writes) and one full-chip launch
 // applies the whole table.
+// With SAVE, op m also records the value its destination held before the
+// apply in save[n * nops + m], n = *step: the pre-boundary field on the
+// written scalars, which a time-reversed walk needs to rebuild each state.
+template <bool SAVE>
 __global__ __launch_bounds__(BC_CHUNK) void bc_chunk_inplace_kernel(
     float *__restrict__ u, const pde_bc_op *__restrict__ ops,
-    const int64_t *__restrict__ chunk_start, const float *t)
+    const int64_t *__restrict__ chunk_start, const float *t, float *__restrict__ save,
+    int64_t nops, const int64_t *__restrict__ step)
 {
     const int64_t b = chunk_start[blockIdx.x], e = chunk_start[blockIdx.x + 1];
     const int64_t m = b + threadIdx.x;
@@ -69,6 +74,7 @@ __global__ __launch_bounds__(BC_CHUNK) void bc_chunk_inplace_kernel(
         const pde_bc_op op = ops[m];
         v = op_eval(op, u, t);
         dst = op.dst;
+        if (SAVE) save[*step * nops + m] = u[dst];
     }
     __syncthreads();
     if (on) u[dst] = v;
@@ -118,7 +124,21 @@ extern "C" int pde_bc_apply_chunked_inplace_f32(float *u, const pde_bc_op *ops,
     if (!u || nchunks < 0) return PDE_EINVAL;
     if (nchunks == 0) return PDE_OK;
     if (!ops || !chunk_start || nchunks > 0x7fffffff) return PDE_EINVAL;
-    hipLaunchKernelGGL(bc_chunk_inplace_kernel, dim3((unsigned)nchunks), dim3(BC_CHUNK), 0,
-                       as_stream(stream), u, ops, chunk_start, t);
+    hipLaunchKernelGGL(bc_chunk_inplace_kernel<false>, dim3((unsigned)nchunks), dim3(BC_CHUNK),
+                       0, as_stream(stream), u, ops, chunk_start, t, nullptr, 0,
+                       (const int64_t *)nullptr);
+    return launch_status();
+}
+
+extern "C" int pde_bc_apply_chunked_inplace_save_f32(float *u, const pde_bc_op *ops,
+                                                     const int64_t *chunk_start, int64_t nchunks,
+                                                     int64_t nops, const float *t, float *save,
+                                                     const int64_t *step, void *stream)
+{
+    if (!u || nchunks < 0 || nops < 0 || !save || !step) return PDE_EINVAL;
+    if (nchunks == 0) return PDE_OK;
+    if (!ops || !chunk_start || nchunks > 0x7fffffff) return PDE_EINVAL;
+    hipLaunchKernelGGL(bc_chunk_inplace_kernel<true>, dim3((unsigned)nchunks), dim3(BC_CHUNK), 0,
+                       as_stream(stream), u, ops, chunk_start, t, save, nops, step);
     return launch_status();
 }

@@ -41,67 +41,6 @@ struct Args2 {
     int64_t step_delta;
 };
 
-// Point lists are walked by the whole wave: the row index is the same on
-// every lane, the wave loads up to 64 of the row's points at once (one per
-// lane), then each lane walks them in list order through v_readlane and
-// applies those in its own columns.  A row's cost is one round of loads
-// however many points it holds, and sums at a node keep the list order.
-
-// [a, b): the list range of padded row x (empty outside the field).
-__device__ __forceinline__ void pts_range(const pde_point_set &P, int64_t x, int64_t P1, int &a,
-                                          int &b)
-{
-    a = b = 0;
-    if (x < 0 || x >= P1) return;
-    a = __builtin_amdgcn_readfirstlane(P.row_start[x]);
-    b = __builtin_amdgcn_readfirstlane(P.row_start[x + 1]);
-}
-
-// Adds scale * data[n][slot] of the points [a, b) to the elements of v in
-// columns k0 .. k0+3, and to ex when the point's column is kex (kex < 0:
-// none).  Every lane of the wave calls it with the same range.
-__device__ __forceinline__ void pts_add(const pde_point_set &P, int a, int b, int64_t k0,
-                                        int64_t n, int lane, v4 &v, int64_t kex, float &ex)
-{
-    for (int base = a; base < b; base += 64) {
-        const int p = base + lane;
-        int col = -1;
-        float val = 0.0f;
-        if (p < b) {
-            col = P.col[p];
-            val = P.scale * P.data[n * P.stride + P.slot[p]];
-        }
-        const int cnt = b - base < 64 ? b - base : 64;
-        for (int i = 0; i < cnt; ++i) {
-            const int64_t c = __builtin_amdgcn_readlane(col, i);
-            const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(val), i));
-            const int64_t dd = c - k0;
-            if (dd >= 0 && dd < 4) set_el(v, (int)dd, el(v, (int)dd) + d);
-            if (c == kex) ex = ex + d;
-        }
-    }
-}
-
-// data[n][slot] = v at the point's column for the points [a, b) whose column
-// lies in the wave's strip [kw0, kw0 + 256) (lane l holds columns
-// kw0 + 4l .. kw0 + 4l + 3).  Every lane of the wave calls it.
-__device__ __forceinline__ void pts_store(const pde_point_set &P, int a, int b, int64_t kw0,
-                                          int64_t n, int lane, const v4 &v)
-{
-    for (int base = a; base < b; base += 64) {
-        const int p = base + lane;
-        const int64_t d = p < b ? (int64_t)P.col[p] - kw0 : -1;
-        const bool mine = d >= 0 && d < 256;
-        const int from = mine ? (int)(d >> 2) : lane;
-        const float e0 = __shfl(v.x, from, 64), e1 = __shfl(v.y, from, 64);
-        const float e2 = __shfl(v.z, from, 64), e3 = __shfl(v.w, from, 64);
-        if (mine) {
-            const int e = (int)(d & 3);
-            P.data[n * P.stride + P.slot[p]] = e == 0 ? e0 : (e == 1 ? e1 : (e == 2 ? e2 : e3));
-        }
-    }
-}
-
 template <int OP>
 __global__ __launch_bounds__(64 * BY2) void step2d_kernel(Args2 A, Tile2 t)
 {

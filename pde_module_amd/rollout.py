@@ -130,10 +130,21 @@ class WaveRollout:
     row, the first-step flag) is read through a device-resident step counter,
     so with ``use_graph=True`` one HIP graph per loop segment (forward,
     recompute, adjoint) is captured once and replayed for every segment; the
-    eager path runs the same kernel sequence."""
+    eager path runs the same kernel sequence.
+
+    ``reverse=True`` (2D scalar fields, checkpointed): the backward pass does
+    not recompute segments.  The forward also records the scalars the
+    boundary overwrites at each step (one float per boundary op and step), and
+    every segment but the first is walked backwards from the checkpoint at its
+    end, two transposed steps per launch, rebuilding each state from the next
+    one (``pde_wave_reverse_adjoint_pair_f32``).  The gradients in u0 and v0
+    are those of the recompute path bit for bit; the c2 gradient differs only
+    by the rounding of the rebuilt states (checkpoints bound how far that
+    rounding can travel).  Segments of odd length and the first segment (the
+    plain-Euler first step is not reversible) use the recompute path."""
 
     def __init__(self, boundary, dx, ghost_cells, dt, steps, checkpoint_every=50, alpha=1.0,
-                 src_idx=None, src_wavelet=None, rec_idx=None, use_graph=False):
+                 src_idx=None, src_wavelet=None, rec_idx=None, use_graph=False, reverse=False):
         self.bc = boundary
         self.w = laplacian_weights(dx)
         self.g = int(ghost_cells)
@@ -152,6 +163,10 @@ class WaveRollout:
                 raise ValueError("src_wavelet must be (steps, n_sources)")
             self.src_wavelet = src_wavelet.contiguous()
         self.use_graph = bool(use_graph)
+        self.reverse = bool(reverse)
+        if self.reverse and self.store_all:
+            raise ValueError("reverse=True walks back from checkpoints: give checkpoint_every")
+        self._saving = False
         self._shape = None
         self._c2 = None
         self._graphs = {}
@@ -178,7 +193,8 @@ class WaveRollout:
         if self.store_all:
             self._cu = self._cv = None
         else:
-            self._cu, self._cv = E(nseg), E(nseg)
+            # reverse: one more checkpoint, the final state
+            self._cu, self._cv = E(nseg + self.reverse), E(nseg + self.reverse)
         nrec = self.rec_idx.numel() if self.rec_idx is not None else 0
         self._nrec = nrec
         self._rec = torch.empty((self.steps, max(nrec, 1)), dtype=like.dtype, device=dev)
@@ -199,7 +215,23 @@ class WaveRollout:
                                      "values")
         if self._fused:
             self._build_point_sets(shape)
+        if self.reverse:
+            self._prepare_reverse(shape, E)
         self._graphs = {}
+
+    def _prepare_reverse(self, shape, E):
+        from .ring_plan import DeviceRingPlan, build_ring_plan
+        table = self.bc.table
+        if not self._fused:
+            raise ValueError("reverse=True needs a 2D scalar field")
+        if table.nops and table.chunk_start is None:
+            raise ValueError("reverse=True needs a boundary table that chunks")
+        p1, p2 = [int(s_) for s_ in shape[:3] if s_ > 1]
+        self._ring = DeviceRingPlan(build_ring_plan(table.host_ops, p1, p2, self.g), self._dev)
+        self._nops = int(table.nops)
+        self._save = torch.empty((self.steps, max(self._nops, 1)), dtype=torch.float32,
+                                 device=self._dev)
+        self._ru, self._rv = [E(), E()], [E(), E()]
 
     def _build_point_sets(self, shape):
         """Row-indexed source / receiver lists of the fused 2D steps (struct
@@ -262,7 +294,10 @@ class WaveRollout:
     def _step_fused(self, u, v, c2, out, record, slot):
         """One launch per step after the boundary (2D scalar fields): source,
         u/v update, receivers and the counter move slot -> 1 - slot."""
-        self.bc.apply_inplace(u)
+        if self._saving and self._nops:
+            self.bc.table.apply_inplace_save(u, self.bc._time_tensor(u), self._save, self._c(slot))
+        else:
+            self.bc.apply_inplace(u)
         N.call("pde_wave_step_fused_f32", u.data_ptr(), v.data_ptr(),
                c2.data_ptr() if c2 is not None else None, out.data_ptr(), v.data_ptr(),
                *_grid(u), self.g, *self.w, self.alpha, self.dt, self._c(slot),
@@ -344,6 +379,30 @@ class WaveRollout:
             _copy(lu, self._lu)
             _copy(lv, self._lv)
 
+    def _rev_segment(self, L, c2, need_c2):
+        """L / 2 pair launches walking the states back from (ru[0], rv[0]),
+        the state at the segment's end, with the counter (slot 0) at the
+        segment's last step.  The gradients live in lu / lv on entry and on
+        exit."""
+        st = N.stream_of(self._lu)
+        lu, lv, gu2, gvn = self._lu, self._lv, self._gu2, self._gvn
+        ua, va, ub, vb = self._ru[0], self._rv[0], self._ru[1], self._rv[1]
+        slot = 0
+        for _ in range(L // 2):
+            N.call("pde_wave_reverse_adjoint_pair_f32", ua.data_ptr(), va.data_ptr(),
+                   lu.data_ptr(), lv.data_ptr(), c2.data_ptr() if c2 is not None else None,
+                   ub.data_ptr(), vb.data_ptr(), gu2.data_ptr(), gvn.data_ptr(),
+                   self._gc2.data_ptr() if need_c2 else None, *_grid(lu), self.g, *self.w,
+                   self.alpha, self.dt, self._c(slot), self._c(1 - slot),
+                   self._ps(self._ps_src), self._ps(self._ps_grec), self._save.data_ptr(),
+                   self._nops, self._ring.ptr, st)
+            slot = 1 - slot
+            ua, ub, va, vb = ub, ua, vb, va
+            lu, gu2, lv, gvn = gu2, lu, gvn, lv
+        if lu is not self._lu:
+            _copy(lu, self._lu)
+            _copy(lv, self._lv)
+
     def _run(self, key, fn, *args):
         """Run fn eagerly, or through a captured graph replayed per call."""
         if not self.use_graph:
@@ -377,11 +436,18 @@ class WaveRollout:
         else:
             self._a.copy_(u0)
             self._v.copy_(v0)
-            for s in range(nseg):
-                L = min(self.K, self.steps - s * self.K)
-                self._cu[s].copy_(self._a)
-                self._cv[s].copy_(self._v)
-                self._run(("fwd", L, c2 is None), self._fwd_segment, L, c2)
+            self._saving = self.reverse
+            try:
+                for s in range(nseg):
+                    L = min(self.K, self.steps - s * self.K)
+                    self._cu[s].copy_(self._a)
+                    self._cv[s].copy_(self._v)
+                    self._run(("fwd", L, c2 is None, self._saving), self._fwd_segment, L, c2)
+            finally:
+                self._saving = False
+            if self.reverse:
+                self._cu[nseg].copy_(self._a)
+                self._cv[nseg].copy_(self._v)
             uN, vN = self._a, self._v
         rec = self._rec[:, :self._nrec] if self._nrec else self._rec[:, :0]
         return uN.clone(), vN.clone(), rec.clone()
@@ -412,6 +478,13 @@ class WaveRollout:
             for s in reversed(range(nseg)):
                 n0 = s * self.K
                 L = min(self.K, self.steps - n0)
+                if self.reverse and s > 0 and L % 2 == 0:
+                    self._ru[0].copy_(self._cu[s + 1])
+                    self._rv[0].copy_(self._cv[s + 1])
+                    self._set_counter(n0 + L - 1)
+                    self._run(("rev", L, c2 is None, need_c2), self._rev_segment, L, c2,
+                              need_c2)
+                    continue
                 self._hist[0].copy_(self._cu[s])
                 self._vt.copy_(self._cv[s])
                 self._set_counter(n0)
