@@ -35,6 +35,9 @@ LAP_BYTES = 8          # read s, write lap
 GRAD_BYTES = 16        # read s, write vec3
 DIV_BYTES = 16         # read vec3, write s
 FUSED_BYTES = 36       # read s + vec3, write lap + vec3 + div
+# two reversed transposed wave steps: read u, v, g_u, g_v, c2, gc2; write
+# u, v, g_u, g_v, gc2 (per padded node)
+WAVE_PAIR_BYTES = 44
 
 
 def parse():
@@ -113,9 +116,11 @@ def cpu_baseline(seconds):
 def wave_adjoint_timing(dev, steps=2000, K=50):
     """2D acoustic workload (2048^2, smoothed random c, Ricker source at the
     centre, 64 receivers, Dirichlet-0 walls, 2000 steps): time of the plain
-    forward loop vs forward + adjoint with checkpoints every K steps, both as
-    HIP-graph-replayed segments.  J = sum of squared receiver misfit against
-    data from the unperturbed model (c = 1)."""
+    forward loop vs forward + adjoint with checkpoints every K steps, all as
+    HIP-graph-replayed segments.  The checkpointed gradient walks each segment
+    back from its end (WaveRollout(reverse=True)); the recompute form and the
+    store-every-state form are timed beside it.  J = sum of squared receiver
+    misfit against data from the unperturbed model (c = 1)."""
     import torch
     from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
     from pde_module_amd.rollout import WaveRollout
@@ -148,44 +153,54 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     t_fwd = timed(lambda: plain.forward_states(c2, u0, v0))
     out = {}
 
-    def fwd_adj():
-        c = c2.clone().requires_grad_(True)
-        _, _, r = ro(c, u0, v0)
-        J = ((r - d) ** 2).sum()
-        J.backward()
-        out["J"], out["g"] = float(J.detach()), float(c.grad.abs().max())
+    def grad_of(rollout, key):
+        def run():
+            c = c2.clone().requires_grad_(True)
+            _, _, r = rollout(c, u0, v0)
+            J = ((r - d) ** 2).sum()
+            J.backward()
+            out[key] = (float(J.detach()), float(c.grad.abs().max()))
+        return run
 
-    t_fa = timed(fwd_adj)
+    t_rev = timed(grad_of(ro_rev, "rev"))
+    t_fa = timed(grad_of(ro, "rec"))
+    t_all = timed(grad_of(ro_all, "all"), reps=2)
 
-    def fwd_adj_all():
-        c = c2.clone().requires_grad_(True)
-        _, _, r = ro_all(c, u0, v0)
-        ((r - d) ** 2).sum().backward()
-
-    t_all = timed(fwd_adj_all, reps=2)
-
-    def fwd_adj_rev():
-        c = c2.clone().requires_grad_(True)
-        _, _, r = ro_rev(c, u0, v0)
-        J = ((r - d) ** 2).sum()
-        J.backward()
-        out["g_rev"] = float(c.grad.abs().max())
-
-    t_rev = timed(fwd_adj_rev)
+    # the pair kernel alone: one reversed segment's K/2 launches, HIP events
+    # on the launch stream (eager launches; state from the last backward)
+    ro_rev._lu.zero_()
+    ro_rev._lv.zero_()
+    ro_rev._set_counter(2 * K - 1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ro_rev._rev_segment(K, ro_rev._c2, True)
+    torch.cuda.synchronize()
+    ro_rev._set_counter(2 * K - 1)
+    e0.record()
+    ro_rev._rev_segment(K, ro_rev._c2, True)
+    e1.record()
+    torch.cuda.synchronize()
+    pair_us = e0.elapsed_time(e1) * 1e3 / (K // 2)
+    pair_bytes = WAVE_PAIR_BYTES * c2.numel()
     return {"grid": [cfg.n, cfg.n], "steps": steps, "checkpoint_every": K,
             "fwd_ms_per_step": round(t_fwd / steps * 1e3, 5),
-            "fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
-            "ratio": round(t_fa / t_fwd, 3), "J": out["J"], "max_abs_dJ_dc2": out["g"],
-            "reversed": {"fwd_adj_ms_per_step": round(t_rev / steps * 1e3, 5),
-                         "ratio": round(t_rev / t_fwd, 3),
-                         "note": "checkpoints every K steps; the backward walks each segment "
-                                 "back from its end (two transposed steps per launch, states "
-                                 "rebuilt, boundary scalars restored) instead of recomputing it"},
+            "fwd_adj_ms_per_step": round(t_rev / steps * 1e3, 5),
+            "ratio": round(t_rev / t_fwd, 3),
+            "J": out["rev"][0], "max_abs_dJ_dc2": out["rev"][1],
+            "mode": "checkpoint every K steps; the backward walks each segment back from the "
+                    "checkpoint at its end, two transposed steps per launch, rebuilding the "
+                    "states (boundary-overwritten scalars recorded by the forward) instead of "
+                    "recomputing the segment; the first segment is recomputed",
+            "pair_kernel": {"kernel": "wave_pair_kernel", "avg_launch_us": round(pair_us, 2),
+                            "algorithmic_bytes_per_launch": pair_bytes,
+                            "GBs": round(pair_bytes / pair_us / 1e3, 1),
+                            "frac": round(pair_bytes / pair_us / 1e3 / HBM_PEAK_GBS, 4)},
+            "recompute": {"fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
+                          "ratio": round(t_fa / t_fwd, 3)},
             "store_all": {"fwd_adj_ms_per_step": round(t_all / steps * 1e3, 5),
                           "ratio": round(t_all / t_fwd, 3),
                           "history_GB": round((steps + 1) * c2.numel() * 4 / 1e9, 1)},
-            "note": "graph-replayed 50-step segments; recompute from checkpoints in the backward "
-                    "(store_all: every state kept, no recompute)"}
+            "note": "graph-replayed 50-step segments; recompute: each segment recomputed from "
+                    "its checkpoint in the backward; store_all: every state kept"}
 
 
 def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
@@ -554,7 +569,7 @@ def main():
             "projection_cavity_1024": cavity,
             "heat_256": heat,
         },
-        "fwd_adjoint_vs_fwd": wave["ratio"] if wave else None,
+        "fwd_adjoint_vs_fwd": wave["ratio"] if wave else None,   # checkpointed, reversed
     }
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
