@@ -432,10 +432,14 @@ __device__ __forceinline__ void fast_out(const PairArgs &A, const FN &up, const 
 }
 
 // Rows x0 .. x0+RW-1 (RW a multiple of 3: the row slots rotate with period 3,
-// so the loop body is unrolled three times and nothing moves between slots);
-// rows below own_x and columns outside [lo, hi) are computed, not stored.
-// Each row's loads (and its gc2 row) are issued one row ahead of its use.
-template <bool C2, int RW>
+// so the loop body is unrolled three times and nothing moves between slots),
+// walked downwards (DIR = 1) or upwards (DIR = -1); rows below own_x and
+// columns outside [lo, hi) are computed, not stored.  Each row's loads (and
+// its gc2 row) are issued one row ahead of its use.  Vertically adjacent
+// waves walk in opposite directions, so the halo rows two waves share are
+// loaded by both at the same moment (start or end of their walks) and the
+// second load hits L2.
+template <bool C2, int RW, int DIR>
 __device__ __forceinline__ void pair_fast(const PairArgs &A, int64_t P1, int64_t P2, int64_t x0,
                                           int64_t own_x, int64_t k0, int lane, bool lane_out,
                                           int64_t lo, int64_t hi, int64_t n, bool has_src,
@@ -449,42 +453,50 @@ __device__ __forceinline__ void pair_fast(const PairArgs &A, int64_t P1, int64_t
         return (A.gc2 && lane_out && x >= own_x) ? ldv(A.gc2, off(x))
                                                  : v4{0.0f, 0.0f, 0.0f, 0.0f};
     };
+    // step n at row y from rows y - DIR (prev), y, y + DIR (next)
+    auto LV = [&](const F1 &prev, const F1 &mid, const F1 &next, int64_t y, FN &r) {
+        if (DIR > 0) fast_leveln<C2>(A, prev, mid, next, y, k0, lane, n, P1, has_src, has_rec, r);
+        else fast_leveln<C2>(A, next, mid, prev, y, k0, lane, n, P1, has_src, has_rec, r);
+    };
+    auto OUT = [&](const FN &prev, const FN &mid, const FN &next, int64_t y, const v4 &g) {
+        const bool st = lane_out && y >= own_x;
+        if (DIR > 0) fast_out<C2>(A, prev, mid, next, off(y), y, k0, lane, n, P1, has_src, st, lo, hi, g);
+        else fast_out<C2>(A, next, mid, prev, off(y), y, k0, lane, n, P1, has_src, st, lo, hi, g);
+    };
+    const int64_t xs = DIR > 0 ? x0 : x0 + RW - 1;   // first output row of the walk
     F1 a, b, c;
     FN p, q, s;
     Raw w;
-    fast_raw<C2>(A, off(x0 - 2), w);
-    fast_finish1<C2>(A, w, x0 - 2, k0, lane, n, P1, has_rec, a);
-    fast_raw<C2>(A, off(x0 - 1), w);
-    fast_finish1<C2>(A, w, x0 - 1, k0, lane, n, P1, has_rec, b);
-    fast_raw<C2>(A, off(x0), w);
-    fast_finish1<C2>(A, w, x0, k0, lane, n, P1, has_rec, c);
-    fast_raw<C2>(A, off(x0 + 1), w);
-    fast_leveln<C2>(A, a, b, c, x0 - 1, k0, lane, n, P1, has_src, has_rec, p);
-    fast_finish1<C2>(A, w, x0 + 1, k0, lane, n, P1, has_rec, a);
-    fast_raw<C2>(A, off(x0 + 2), w);
-    v4 g0 = gcrow(x0);
-    fast_leveln<C2>(A, b, c, a, x0, k0, lane, n, P1, has_src, has_rec, q);
+    fast_raw<C2>(A, off(xs - 2 * DIR), w);
+    fast_finish1<C2>(A, w, xs - 2 * DIR, k0, lane, n, P1, has_rec, a);
+    fast_raw<C2>(A, off(xs - DIR), w);
+    fast_finish1<C2>(A, w, xs - DIR, k0, lane, n, P1, has_rec, b);
+    fast_raw<C2>(A, off(xs), w);
+    fast_finish1<C2>(A, w, xs, k0, lane, n, P1, has_rec, c);
+    fast_raw<C2>(A, off(xs + DIR), w);
+    LV(a, b, c, xs - DIR, p);
+    fast_finish1<C2>(A, w, xs + DIR, k0, lane, n, P1, has_rec, a);
+    fast_raw<C2>(A, off(xs + 2 * DIR), w);
+    v4 g0 = gcrow(xs);
+    LV(b, c, a, xs, q);
 #pragma unroll 1
     for (int r = 0; r < RW; r += 3) {
-        const int64_t x = x0 + r;
-        fast_finish1<C2>(A, w, x + 2, k0, lane, n, P1, has_rec, b);
-        fast_raw<C2>(A, off(x + 3), w);
-        v4 g1 = gcrow(x + 1);
-        fast_leveln<C2>(A, c, a, b, x + 1, k0, lane, n, P1, has_src, has_rec, s);
-        fast_out<C2>(A, p, q, s, off(x), x, k0, lane, n, P1, has_src, lane_out && x >= own_x,
-                     lo, hi, g0);
-        fast_finish1<C2>(A, w, x + 3, k0, lane, n, P1, has_rec, c);
-        fast_raw<C2>(A, off(x + 4), w);
-        g0 = gcrow(x + 2);
-        fast_leveln<C2>(A, a, b, c, x + 2, k0, lane, n, P1, has_src, has_rec, p);
-        fast_out<C2>(A, q, s, p, off(x + 1), x + 1, k0, lane, n, P1, has_src,
-                     lane_out && x + 1 >= own_x, lo, hi, g1);
-        fast_finish1<C2>(A, w, x + 4, k0, lane, n, P1, has_rec, a);
-        if (r + 3 < RW) fast_raw<C2>(A, off(x + 5), w);
-        g1 = gcrow(x + 3);
-        fast_leveln<C2>(A, b, c, a, x + 3, k0, lane, n, P1, has_src, has_rec, q);
-        fast_out<C2>(A, s, p, q, off(x + 2), x + 2, k0, lane, n, P1, has_src,
-                     lane_out && x + 2 >= own_x, lo, hi, g0);
+        const int64_t x = xs + r * DIR;
+        fast_finish1<C2>(A, w, x + 2 * DIR, k0, lane, n, P1, has_rec, b);
+        fast_raw<C2>(A, off(x + 3 * DIR), w);
+        v4 g1 = gcrow(x + DIR);
+        LV(c, a, b, x + DIR, s);
+        OUT(p, q, s, x, g0);
+        fast_finish1<C2>(A, w, x + 3 * DIR, k0, lane, n, P1, has_rec, c);
+        fast_raw<C2>(A, off(x + 4 * DIR), w);
+        g0 = gcrow(x + 2 * DIR);
+        LV(a, b, c, x + 2 * DIR, p);
+        OUT(q, s, p, x + DIR, g1);
+        fast_finish1<C2>(A, w, x + 4 * DIR, k0, lane, n, P1, has_rec, a);
+        if (r + 3 < RW) fast_raw<C2>(A, off(x + 5 * DIR), w);
+        g1 = gcrow(x + 3 * DIR);
+        LV(b, c, a, x + 3 * DIR, q);
+        OUT(s, p, q, x + 2 * DIR, g0);
         g0 = g1;
     }
 }
@@ -536,12 +548,20 @@ __device__ __forceinline__ void pair_interior(const PairArgs &A, const PairGeom 
             const bool has_rec = pts_any(A.rec, xs - 2, xs + RW + 2, pg.P1);
             const int64_t k0 = ks + 4 * (lane - 1);
             const bool lane_out = lane >= 1 && lane <= 62;
-            if (A.c2)
-                pair_fast<true, RW>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi, n,
-                                    has_src, has_rec);
+            // waves alternate their walk direction (odd waves upwards)
+            const bool up = threadIdx.y & 1;
+            if (A.c2 && !up)
+                pair_fast<true, RW, 1>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi, n,
+                                       has_src, has_rec);
+            else if (A.c2)
+                pair_fast<true, RW, -1>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi,
+                                        n, has_src, has_rec);
+            else if (!up)
+                pair_fast<false, RW, 1>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi,
+                                        n, has_src, has_rec);
             else
-                pair_fast<false, RW>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi, n,
-                                     has_src, has_rec);
+                pair_fast<false, RW, -1>(A, pg.P1, pg.P2, xs, x0, k0, lane, lane_out, own_k, hi,
+                                         n, has_src, has_rec);
             return;
         }
     }
@@ -824,11 +844,14 @@ __global__ __launch_bounds__(64 * PR_BY) void wave_pair_kernel(PairArgs A, PairG
     __shared__ float lds[RING_ARR * RING_CAP];
     const int64_t n = *A.step;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *A.step_next = n - 2;
-    if (blockIdx.x < pg.nring) {
-        if ((int)blockIdx.x < A.ring.npieces) pair_ring(A, pg, (int)blockIdx.x, n, lds);
+    // ring pieces after the interior tiles: they fill the slots the tiles'
+    // last round leaves free
+    if (blockIdx.x >= pg.nint) {
+        const int pid = (int)(blockIdx.x - pg.nint);
+        if (pid < A.ring.npieces) pair_ring(A, pg, pid, n, lds);
         return;
     }
-    pair_interior<PR_ROWS>(A, pg, blockIdx.x - pg.nring, n);
+    pair_interior<PR_ROWS>(A, pg, blockIdx.x, n);
 }
 
 }  // namespace pde
