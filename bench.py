@@ -49,6 +49,10 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--quick", action="store_true",
                    help="operators only: skip the wave and ensemble workloads (profiling runs)")
+    p.add_argument("--only-wave", action="store_true",
+                   help="profiling runs: only the 2048^2 wave workload, its JSON on stdout")
+    p.add_argument("--wave-steps", type=int, default=2000,
+                   help="steps of the wave workload (profiling runs shorten it)")
     return p.parse_args()
 
 
@@ -166,8 +170,11 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     t_fa = timed(grad_of(ro, "rec"))
     t_all = timed(grad_of(ro_all, "all"), reps=2)
 
-    # the pair kernel alone: one reversed segment's K/2 launches, HIP events
-    # on the launch stream (eager launches; state from the last backward)
+    # the pair kernel alone: one reversed segment's K/2 launches (steps
+    # 2K-1 .. K, which need at least 2K recorded steps), HIP events on the
+    # launch stream (eager launches; state from the last backward)
+    if steps < 2 * K:
+        raise ValueError(f"the wave workload needs at least {2 * K} steps")
     ro_rev._lu.zero_()
     ro_rev._lv.zero_()
     ro_rev._set_counter(2 * K - 1)
@@ -451,6 +458,10 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only_wave:
+        print(json.dumps({"wave2048_fwd_vs_fwd_adjoint":
+                          wave_adjoint_timing(dev, steps=args.wave_steps)}))
+        return
     gen = torch.Generator(device=dev).manual_seed(0)
     s = torch.rand((P, P, P, 1), device=dev, generator=gen) * 2 - 1
     v = torch.rand((P, P, P, 3), device=dev, generator=gen) * 2 - 1
@@ -514,7 +525,7 @@ def main():
              f"{t_lap * 1e6:.1f} us")
     wave = ensemble = cavity = heat = None
     if not args.quick:
-        wave = wave_adjoint_timing(dev)
+        wave = wave_adjoint_timing(dev, steps=args.wave_steps)
         progress("wave forward / forward+adjoint timed")
         ensemble = heat_ensemble_timing(dev)
         progress("heat ensemble timed")
