@@ -299,8 +299,8 @@ struct F1 {              // a step-(n+1) row, every node interior
     v4 u2, h, V, GU, gvt, C;
 };
 
-struct FN {              // a step-n row: u2 / h of step n-1, v_n, g_u, gvt of step n, c2, T1
-    v4 u2, h, vn, gun, gvt, C, T1;
+struct FN {              // a step-n row: u2 / h of step n-1, v_n, g_u, gvt of step n, and
+    v4 u2, h, vn, gun, gvt, gcp;   // gc2 with step n's term added
 };
 
 // Row x of step n+1 as loaded (the lane's 4 columns at byte offset o)...
@@ -348,14 +348,13 @@ template <bool C2>
 __device__ __forceinline__ void fast_leveln(const PairArgs &A, const F1 &up, const F1 &mid,
                                             const F1 &dn, int64_t x, int64_t k0, int lane,
                                             int64_t n, int64_t P1, bool has_src, bool has_rec,
-                                            FN &r)
+                                            const v4 &gc0, FN &r)
 {
     // lanes 0 and 63 take 0 beyond the strip: only their inner columns,
     // which never read it, reach a stored value
     const float ul = from_left(mid.u2.w), ur = from_right(mid.u2.x);
     const float hl = from_left(mid.h.w), hr = from_right(mid.h.x);
     const float dt = A.dt, alpha = A.alpha;
-    r.C = mid.C;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float Lu = acc5(up.u2, mid.u2, dn.u2, ul, ur, i, A.wm, A.wc, A.wp);
@@ -363,7 +362,7 @@ __device__ __forceinline__ void fast_leveln(const PairArgs &A, const F1 &up, con
         set_el(r.vn, i, el(mid.V, i) - dt * rhs);
         const float at = acct5_in(up.h, mid.h, dn.h, hl, hr, i, A.wm, A.wc, A.wp);
         set_el(r.gun, i, el(mid.GU, i) + dt * (alpha * at));
-        set_el(r.T1, i, dt * (el(mid.gvt, i) * (alpha * Lu)));
+        set_el(r.gcp, i, el(gc0, i) + dt * (el(mid.gvt, i) * (alpha * Lu)));
     }
     if (has_src) {
         int a, b;
@@ -403,8 +402,9 @@ template <bool C2>
 __device__ __forceinline__ void fast_out(const PairArgs &A, const FN &up, const FN &mid,
                                          const FN &dn, uint32_t o, int64_t x, int64_t k0,
                                          int lane, int64_t n, int64_t P1, bool has_src,
-                                         bool store, int64_t lo, int64_t hi, const v4 &gc0)
+                                         bool store, int64_t lo, int64_t hi)
 {
+    const v4 C = C2 ? ldv(A.c2, o) : v4{1.0f, 1.0f, 1.0f, 1.0f};   // reloaded: L2
     const float ul = from_left(mid.u2.w), ur = from_right(mid.u2.x);
     const float hl = from_left(mid.h.w), hr = from_right(mid.h.x);
     const float dt = A.dt, alpha = A.alpha;
@@ -412,11 +412,11 @@ __device__ __forceinline__ void fast_out(const PairArgs &A, const FN &up, const 
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float Lu = acc5(up.u2, mid.u2, dn.u2, ul, ur, i, A.wm, A.wc, A.wp);
-        const float rhs = C2 ? el(mid.C, i) * (alpha * Lu) : alpha * Lu;
+        const float rhs = C2 ? el(C, i) * (alpha * Lu) : alpha * Lu;
         set_el(vo, i, el(mid.vn, i) - dt * rhs);
         const float at = acct5_in(up.h, mid.h, dn.h, hl, hr, i, A.wm, A.wc, A.wp);
         set_el(guo, i, el(mid.gun, i) + dt * (alpha * at));
-        set_el(gc, i, (el(gc0, i) + el(mid.T1, i)) + dt * (el(mid.gvt, i) * (alpha * Lu)));
+        set_el(gc, i, el(mid.gcp, i) + dt * (el(mid.gvt, i) * (alpha * Lu)));
     }
     if (has_src) {
         int a, b;
@@ -434,8 +434,7 @@ __device__ __forceinline__ void fast_out(const PairArgs &A, const FN &up, const 
 // Rows x0 .. x0+RW-1 (RW a multiple of 3: the row slots rotate with period 3,
 // so the loop body is unrolled three times and nothing moves between slots),
 // walked downwards (DIR = 1) or upwards (DIR = -1); rows below own_x and
-// columns outside [lo, hi) are computed, not stored.  Each row's loads (and
-// its gc2 row) are issued one row ahead of its use.  Vertically adjacent
+// columns outside [lo, hi) are computed, not stored.  Vertically adjacent
 // waves walk in opposite directions, so the halo rows two waves share are
 // loaded by both at the same moment (start or end of their walks) and the
 // second load hits L2.
@@ -453,51 +452,48 @@ __device__ __forceinline__ void pair_fast(const PairArgs &A, int64_t P1, int64_t
         return (A.gc2 && lane_out && x >= own_x) ? ldv(A.gc2, off(x))
                                                  : v4{0.0f, 0.0f, 0.0f, 0.0f};
     };
-    // step n at row y from rows y - DIR (prev), y, y + DIR (next)
-    auto LV = [&](const F1 &prev, const F1 &mid, const F1 &next, int64_t y, FN &r) {
-        if (DIR > 0) fast_leveln<C2>(A, prev, mid, next, y, k0, lane, n, P1, has_src, has_rec, r);
-        else fast_leveln<C2>(A, next, mid, prev, y, k0, lane, n, P1, has_src, has_rec, r);
+    // step n at row y from rows y - DIR (prev), y, y + DIR (next); g = gc2 row y
+    auto LV = [&](const F1 &prev, const F1 &mid, const F1 &next, int64_t y, const v4 &g,
+                  FN &r) {
+        if (DIR > 0)
+            fast_leveln<C2>(A, prev, mid, next, y, k0, lane, n, P1, has_src, has_rec, g, r);
+        else
+            fast_leveln<C2>(A, next, mid, prev, y, k0, lane, n, P1, has_src, has_rec, g, r);
     };
-    auto OUT = [&](const FN &prev, const FN &mid, const FN &next, int64_t y, const v4 &g) {
+    auto OUT = [&](const FN &prev, const FN &mid, const FN &next, int64_t y) {
         const bool st = lane_out && y >= own_x;
-        if (DIR > 0) fast_out<C2>(A, prev, mid, next, off(y), y, k0, lane, n, P1, has_src, st, lo, hi, g);
-        else fast_out<C2>(A, next, mid, prev, off(y), y, k0, lane, n, P1, has_src, st, lo, hi, g);
+        if (DIR > 0)
+            fast_out<C2>(A, prev, mid, next, off(y), y, k0, lane, n, P1, has_src, st, lo, hi);
+        else
+            fast_out<C2>(A, next, mid, prev, off(y), y, k0, lane, n, P1, has_src, st, lo, hi);
+    };
+    auto LD = [&](int64_t y, F1 &r) {
+        Raw w;
+        fast_raw<C2>(A, off(y), w);
+        fast_finish1<C2>(A, w, y, k0, lane, n, P1, has_rec, r);
     };
     const int64_t xs = DIR > 0 ? x0 : x0 + RW - 1;   // first output row of the walk
+    const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     F1 a, b, c;
     FN p, q, s;
-    Raw w;
-    fast_raw<C2>(A, off(xs - 2 * DIR), w);
-    fast_finish1<C2>(A, w, xs - 2 * DIR, k0, lane, n, P1, has_rec, a);
-    fast_raw<C2>(A, off(xs - DIR), w);
-    fast_finish1<C2>(A, w, xs - DIR, k0, lane, n, P1, has_rec, b);
-    fast_raw<C2>(A, off(xs), w);
-    fast_finish1<C2>(A, w, xs, k0, lane, n, P1, has_rec, c);
-    fast_raw<C2>(A, off(xs + DIR), w);
-    LV(a, b, c, xs - DIR, p);
-    fast_finish1<C2>(A, w, xs + DIR, k0, lane, n, P1, has_rec, a);
-    fast_raw<C2>(A, off(xs + 2 * DIR), w);
-    v4 g0 = gcrow(xs);
-    LV(b, c, a, xs, q);
+    LD(xs - 2 * DIR, a);
+    LD(xs - DIR, b);
+    LD(xs, c);
+    LV(a, b, c, xs - DIR, zero, p);
+    LD(xs + DIR, a);
+    LV(b, c, a, xs, gcrow(xs), q);
 #pragma unroll 1
     for (int r = 0; r < RW; r += 3) {
         const int64_t x = xs + r * DIR;
-        fast_finish1<C2>(A, w, x + 2 * DIR, k0, lane, n, P1, has_rec, b);
-        fast_raw<C2>(A, off(x + 3 * DIR), w);
-        v4 g1 = gcrow(x + DIR);
-        LV(c, a, b, x + DIR, s);
-        OUT(p, q, s, x, g0);
-        fast_finish1<C2>(A, w, x + 3 * DIR, k0, lane, n, P1, has_rec, c);
-        fast_raw<C2>(A, off(x + 4 * DIR), w);
-        g0 = gcrow(x + 2 * DIR);
-        LV(a, b, c, x + 2 * DIR, p);
-        OUT(q, s, p, x + DIR, g1);
-        fast_finish1<C2>(A, w, x + 4 * DIR, k0, lane, n, P1, has_rec, a);
-        if (r + 3 < RW) fast_raw<C2>(A, off(x + 5 * DIR), w);
-        g1 = gcrow(x + 3 * DIR);
-        LV(b, c, a, x + 3 * DIR, q);
-        OUT(s, p, q, x + 2 * DIR, g0);
-        g0 = g1;
+        LD(x + 2 * DIR, b);
+        LV(c, a, b, x + DIR, gcrow(x + DIR), s);
+        OUT(p, q, s, x);
+        LD(x + 3 * DIR, c);
+        LV(a, b, c, x + 2 * DIR, gcrow(x + 2 * DIR), p);
+        OUT(q, s, p, x + DIR);
+        LD(x + 4 * DIR, a);
+        LV(b, c, a, x + 3 * DIR, gcrow(x + 3 * DIR), q);
+        OUT(s, p, q, x + 2 * DIR);
     }
 }
 
@@ -839,7 +835,10 @@ __device__ __forceinline__ void pair_ring(const PairArgs &A, const PairGeom &pg,
     }
 }
 
-__global__ __launch_bounds__(64 * PR_BY) void wave_pair_kernel(PairArgs A, PairGeom pg)
+// 3 workgroups per CU (168 VGPRs, 3 waves per SIMD; a few registers spill):
+// the tiles are latency-bound, and the third wave measures faster than the
+// spill-free 173-register build at 2 waves per SIMD.
+__global__ __launch_bounds__(64 * PR_BY, 3) void wave_pair_kernel(PairArgs A, PairGeom pg)
 {
     __shared__ float lds[RING_ARR * RING_CAP];
     const int64_t n = *A.step;
