@@ -156,3 +156,28 @@ def test_reverse_rejects_store_all(dev):
     bc = make_bc(torch.from_numpy(u0).to(dev))
     with pytest.raises(ValueError):
         WaveRollout(bc, dx, 1, float(dt), 10, checkpoint_every=None, reverse=True)
+
+
+def test_reverse_gradient_vs_central_differences(dev):
+    """Directional derivatives of the reversed GPU gradient (c2, u0 and v0
+    together) against a central difference of the FP64 restatement of the
+    same loop: within 1e-4 relative (the bound the gradient is held to)."""
+    steps = 120
+    b = _run(dev, "square", steps, 10, reverse=True)
+    rng = np.random.default_rng(21)
+    dirs = [torch.from_numpy(rng.standard_normal(b[k].shape)) for k in ("c2", "u0", "v0")]
+    base = [torch.from_numpy(b[k]).double() for k in ("c2", "u0", "v0")]
+
+    def J(t):
+        c, x, y = (a + t * d for a, d in zip(base, dirs))
+        uN, _, r = TF.wave_rollout(c, x, y, b["dx"], 1, float(b["dt"]), steps,
+                                   src_idx=torch.from_numpy(b["src"]),
+                                   src_wavelet=torch.from_numpy(b["wav"]).double(),
+                                   rec_idx=torch.from_numpy(b["rec"]))
+        return float(((r - b["d"].cpu().double()) ** 2).sum() + (uN * uN).sum() * 1e-3)
+
+    eps = 1e-5
+    fd = (J(eps) - J(-eps)) / (2 * eps)
+    gd = sum(float((g.detach().double().cpu() * d).sum()) for g, d in zip(b["grads"], dirs))
+    print(f"reverse gradient . d = {gd:.8e}, central difference {fd:.8e}", flush=True)
+    assert abs(gd - fd) <= 1e-4 * abs(fd)
