@@ -13,17 +13,21 @@
 // step n+1; writes the same five at step n-1 (44 bytes per node for two
 // steps).  Two kinds of workgroups share the launch:
 //  * interior tiles (nodes at least `depth` from the edge): a wave owns 64
-//    lanes of 4 columns and walks 4 output rows; step-n rows are computed
-//    one row and one lane beyond the output (strips overlap by one lane per
-//    side, 248 output columns each), so the second step needs no exchange.
-//    The boundary touches no node these tiles compute, so none of its work
-//    appears here.
-//  * ring pieces (the outer `depth` rows and columns, 32 nodes along the
-//    edge): the piece grown by two nodes is staged in LDS; step n is computed
-//    on the piece grown by one, the saved boundary scalars and the boundary
-//    transpose are applied there, then step n-1 on the piece itself, with its
-//    own boundary fix-ups before the stores.  The host checks that every
-//    boundary transpose a piece needs stays inside it.
+//    lanes of 4 columns and walks PR_ROWS output rows; step-n rows are
+//    computed one row and one lane beyond the output (strips overlap by one
+//    lane per side, 248 output columns each), so the second step needs no
+//    exchange.  The boundary touches no node these tiles compute, so none of
+//    its work appears here.  Waves whose nodes are all interior take a fast
+//    path (no interior tests, DPP lane shifts, 32-bit offsets; adjacent waves
+//    walk in opposite directions so shared halo rows are loaded together);
+//    the rest take a general path with the interior tests of steps2d.hip.
+//  * ring pieces (the outer `depth` rows and columns, up to 32 + depth nodes
+//    along the edge), launched after the tiles: the piece grown by two nodes
+//    is staged in LDS; step n is computed on the piece grown by one, the
+//    saved boundary scalars and the boundary transpose are applied there,
+//    then step n-1 on the piece itself, with its own boundary fix-ups before
+//    the stores.  The host checks that every boundary transpose a piece needs
+//    stays inside it.
 // Arithmetic mirrors steps2d.hip's transposed step operation by operation, so
 // g_u and g_v equal two fused transposed steps plus boundary transposes bit
 // for bit; only gc2 sees the rounding of the rebuilt states.
