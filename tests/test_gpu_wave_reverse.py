@@ -37,13 +37,19 @@ def _ricker(steps, dt, dx, amp=100.0):
     return (amp * (1 - 2 * a) * np.exp(-a)).astype(np.float32)
 
 
+GHOST = {"g2": 2}
+
+
 def _case(name):
     """(shape, dx, dt, c2, u0, v0, src, wavelet, rec, make_bc)."""
-    rng = np.random.default_rng({"square": 11, "wide": 12, "functional": 13}[name])
+    rng = np.random.default_rng({"square": 11, "wide": 12, "functional": 13, "g2": 14}[name])
+    g = GHOST.get(name, 1)
     if name == "square":
         P1, P2 = 34, 34
     elif name == "wide":
         P1, P2 = 61, 525          # 3 strips (the last one partial), 4 row tiles
+    elif name == "g2":
+        P1, P2 = 40, 300          # ghost width 2: ring depth 5, fast-path tiles
     else:
         P1, P2 = 45, 40
     shape = (P1, P2, 1)
@@ -68,8 +74,8 @@ def _case(name):
             bc.dirichlet_BC("ALL", 0.0)
             bc.vonNeumann_BC("+Y", 0.25)
             return bc
-        bc = LegacyGridBoundary(u, dx, 1)
-        if name == "wide":
+        bc = LegacyGridBoundary(u, dx, g)
+        if name in ("wide", "g2"):
             bc.vonNeumann_BC("-X", 0.5)
             bc.vonNeumann_BC("+Y", 0.0)
         return bc
@@ -85,7 +91,8 @@ def _run(dev, name, steps, K, reverse, use_graph=False, c2_on=True, seed=5):
     g = torch.Generator(device=dev).manual_seed(seed)
     d = torch.randn(steps, len(rec), device=dev, generator=g) * 1e-2
     bc = make_bc(T(u0))
-    ro = WaveRollout(bc, dx, 1, float(dt), steps, checkpoint_every=K, src_idx=T(src),
+    ro = WaveRollout(bc, dx, GHOST.get(name, 1), float(dt), steps, checkpoint_every=K,
+                     src_idx=T(src),
                      src_wavelet=T(wav), rec_idx=T(rec), use_graph=use_graph, reverse=reverse)
     c = T(c2).requires_grad_(True) if c2_on else None
     x = T(u0).requires_grad_(True)
@@ -100,7 +107,7 @@ def _run(dev, name, steps, K, reverse, use_graph=False, c2_on=True, seed=5):
 
 
 @pytest.mark.parametrize("name,steps,K", [("square", 120, 10), ("wide", 60, 12),
-                                          ("functional", 80, 20)])
+                                          ("functional", 80, 20), ("g2", 60, 12)])
 def test_reverse_matches_recompute(dev, name, steps, K):
     print(f"reverse vs recompute: {name} steps={steps} K={K}", flush=True)
     a = _run(dev, name, steps, K, reverse=False)
