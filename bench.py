@@ -43,8 +43,10 @@ WAVE_PAIR_BYTES = 44
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
-    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--steps", type=int, default=200)
+    # the first ~10 ms of back-to-back launches run slower (clock ramp): warm
+    # up past that
+    p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--quick", action="store_true",
