@@ -349,6 +349,20 @@ typedef struct pde_ring_plan {
     const float *csign;
 } pde_ring_plan;
 
+/* The heat step of pde_heat_step_f32 with the in-place boundary folded into
+ * the same launch, on a 2D scalar field: u is left boundary-applied (the ops
+ * of pde_bc_apply_inplace_f32, t as there) and un = u2 + dt*(alpha*lap(u2))
+ * on interior nodes, u2 elsewhere; the same values as the two launches.  ring
+ * is the field's boundary ring (pde_ring_plan, built by the host for the op
+ * table and checked so that each piece's ops read inside the piece and no
+ * scalar another piece writes).  No reference counterpart as one call: it
+ * replaces GridBoundary + Laplacian + ForwardEuler of
+ * examples/2D_Heat_Equation.py:58-62 with one launch. */
+int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t ny, int64_t nz,
+                           int32_t ghost, const pde_bc_op *ops, int64_t nops, const float *t,
+                           float w_minus, float w_center, float w_plus, float alpha, float dt,
+                           const pde_ring_plan *ring, void *stream);
+
 /* Two transposed wave steps, n then n - 1 (n = *step, *step_next = n - 2),
  * walking the forward states backwards instead of reading them: from the
  * step-(n+1) state (u, v: pre-boundary u, the v the step wrote) it rebuilds

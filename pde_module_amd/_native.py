@@ -53,6 +53,8 @@ SIGNATURES = {
     "pde_bc_adjoint_inplace_f32": [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P],
     "pde_bc_apply_chunked_inplace_f32": [_P, _P, _P, _I64, _P, _P],
     "pde_bc_apply_chunked_inplace_save_f32": [_P, _P, _P, _I64, _I64, _P, _P, _P, _P],
+    "pde_heat_step_ring_f32": [_P, _P, _I64, _I64, _I64, _I32, _P, _I64, _P, _F, _F, _F, _F, _F,
+                               _P, _P],
     "pde_wave_reverse_adjoint_pair_f32": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64,
                                           _I64, _I32, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P,
                                           _I64, _P, _P],

@@ -156,3 +156,30 @@ class DeviceRingPlan:
     @property
     def ptr(self):
         return self.rec.ctypes.data
+
+
+def check_step_sources(plan, ops, P1, P2):
+    """For the one-launch heat step (``pde_heat_step_ring_f32``): every
+    boundary op a piece evaluates (its written node in the piece grown by one)
+    must read a node inside the piece grown by two, and not a written node
+    that another piece stores back concurrently.  Raises ``ValueError``."""
+    P1, P2 = int(P1), int(P2)
+    pieces, W = plan["pieces"], plan["w"]
+    owner = {}
+    for pid, p in enumerate(pieces):
+        for e in W[p["w_beg"]:p["w_end"]]:
+            if e["flags"] & 2:
+                owner[int(e["node"])] = pid
+    for pid, p in enumerate(pieces):
+        r0, r1 = max(int(p["r0"]) - 2, 0), min(int(p["r1"]) + 2, P1)
+        c0, c1 = max(int(p["c0"]) - 2, 0), min(int(p["c1"]) + 2, P2)
+        for e in W[p["w_beg"]:p["w_end"]]:
+            op = ops[int(e["a"])]
+            if int(op["kind"]) == N.BC_SET:
+                continue
+            src = int(op["src"])
+            x, k = divmod(src, P2)
+            if not (r0 <= x < r1 and c0 <= k < c1):
+                raise ValueError("a boundary op reads outside its ring piece")
+            if owner.get(src, pid) != pid:
+                raise ValueError("a boundary op reads a scalar another ring piece writes")

@@ -52,12 +52,13 @@ class HeatRollout:
         self.dt = np.float32(dt)
         self.steps = int(steps)
         self.K = max(1, int(checkpoint_every))
+        from .time_step.fused import HeatStep
+        self._hs = HeatStep(boundary, dx, ghost_cells, alpha, dt)
 
     def _step(self, u, out):
-        """u -> boundary in place -> out = fused heat step(u)."""
-        self.bc.apply_inplace(u)
-        N.call("pde_heat_step_f32", u.data_ptr(), out.data_ptr(), *_grid(u), int(u.shape[3]),
-               self.g, *self.w, self.alpha, self.dt, N.stream_of(u))
+        """u -> boundary in place -> out = fused heat step(u) (one launch on
+        2D scalar fields whose boundary table allows it)."""
+        self._hs.step_into(u, out)
 
     def forward_states(self, u0):
         """Runs the loop; returns (u_N, checkpoints {n: u^n})."""
@@ -86,10 +87,8 @@ class HeatRollout:
             tmp = torch.empty_like(lam)
             hist[0].copy_(ckpt[n0])
             for i in range(L):
-                self.bc.apply_inplace(hist[i])            # hist[i] = u2 at step n0 + i
-                nxt = hist[i + 1] if i + 1 < L else tmp
-                N.call("pde_heat_step_f32", hist[i].data_ptr(), nxt.data_ptr(), *_grid(lam),
-                       int(shape[3]), self.g, *self.w, self.alpha, self.dt, N.stream_of(lam))
+                # hist[i] becomes u2 at step n0 + i
+                self._step(hist[i], hist[i + 1] if i + 1 < L else tmp)
             for i in reversed(range(L)):
                 N.call("pde_heat_step_adjoint_f32", lam.data_ptr(), gu2.data_ptr(), *_grid(lam),
                        int(shape[3]), self.g, *self.w, self.alpha, self.dt, N.stream_of(lam))

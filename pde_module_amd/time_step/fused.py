@@ -5,8 +5,11 @@ BC copy, Laplacian, ForwardEuler; ``examples/wave2d/standingWave.py:72-76``: the
 same plus a second Euler), moving about 28 bytes per lattice update for a scalar
 heat step.  Here the boundary writes go into the state in place (a surface-sized
 launch) and one kernel reads the state once and writes the next one: 8 bytes per
-update for heat, 20 for the u/v wave step.  Every non-ghost value is the same
-FP32 number the module-by-module loop gives.
+update for heat, 20 for the u/v wave step.  On 2D scalar fields the heat step
+folds the boundary into the same launch (``pde_heat_step_ring_f32``: the outer
+rows and columns apply it in LDS) whenever the boundary table allows, so a
+step is a single launch.  Every non-ghost value is the same FP32 number the
+module-by-module loop gives.
 
 The steppers own two ping-pong buffers.  The first call copies the caller's
 state into them; later calls are expected to pass back the tensor the previous
@@ -27,13 +30,39 @@ class HeatStep:
 
     ``boundary`` is a GridBoundary (functional or legacy) built for the field."""
 
-    def __init__(self, boundary, dx: float, ghost_cells: int, alpha: float, dt: float) -> None:
+    def __init__(self, boundary, dx: float, ghost_cells: int, alpha: float, dt: float,
+                 one_launch: bool = True) -> None:
         self.boundary = boundary
         self.weights = laplacian_weights(dx)
         self.ghost_cells = ghost_cells
         self.alpha = np.float32(alpha)
         self.dt = np.float32(dt)
         self._bufs = None
+        # one_launch=False keeps the boundary launch and the step launch apart
+        # (same values; used to check the one-launch form)
+        self.one_launch = bool(one_launch)
+        self._ring = None   # (key, DeviceRingPlan or None)
+
+    def _ring_plan(self, u: torch.Tensor):
+        """The ring plan of the one-launch step for u's shape and the
+        boundary's current table, or None when the step cannot take it."""
+        shape = tuple(int(s_) for s_ in u.shape[:3])
+        if not self.one_launch or u.shape[3] != 1 or sum(1 for s_ in shape if s_ > 1) != 2:
+            return None
+        self.boundary._ensure_table(u)
+        table = self.boundary.table
+        key = (shape, u.device, id(table))
+        if self._ring is None or self._ring[0] != key:
+            from ..ring_plan import DeviceRingPlan, build_ring_plan, check_step_sources
+            p1, p2 = [s_ for s_ in shape if s_ > 1]
+            try:
+                plan = build_ring_plan(table.host_ops, p1, p2, self.ghost_cells)
+                check_step_sources(plan, table.host_ops, p1, p2)
+                dev_plan = DeviceRingPlan(plan, u.device)
+            except ValueError:
+                dev_plan = None
+            self._ring = (key, dev_plan, table)
+        return self._ring[1]
 
     def _own(self, u: torch.Tensor) -> torch.Tensor:
         if self._bufs is None or self._bufs[0].shape != u.shape or self._bufs[0].device != u.device:
@@ -44,15 +73,29 @@ class HeatStep:
         a.copy_(u)
         return a
 
+    def step_into(self, u: torch.Tensor, out: torch.Tensor, t=None) -> None:
+        """One step from u into out (distinct tensors of u's shape); u is left
+        holding the boundary-applied state u2."""
+        X, Y, Z = (int(s) for s in u.shape[:3])
+        wm, wc, wp = self.weights
+        ring = self._ring_plan(u)
+        if ring is not None:
+            table = self.boundary.table
+            tt = self.boundary._time_tensor(u, t)
+            N.call("pde_heat_step_ring_f32", N.dev_ptr(u), N.dev_ptr(out), X, Y, Z,
+                   self.ghost_cells, table.ops.data_ptr() if table.nops else None, table.nops,
+                   tt.data_ptr() if tt is not None else None, wm, wc, wp, self.alpha, self.dt,
+                   ring.ptr, N.stream_of(u))
+            return
+        self.boundary.apply_inplace(u, t)
+        N.call("pde_heat_step_f32", N.dev_ptr(u), N.dev_ptr(out), X, Y, Z, int(u.shape[3]),
+               self.ghost_cells, wm, wc, wp, self.alpha, self.dt, N.stream_of(u))
+
     def __call__(self, u: torch.Tensor, t=None) -> torch.Tensor:
         N.check_field(u, "u")
         u = self._own(u)
         nxt = self._bufs[1] if u.data_ptr() == self._bufs[0].data_ptr() else self._bufs[0]
-        self.boundary.apply_inplace(u, t)
-        X, Y, Z = (int(s) for s in u.shape[:3])
-        wm, wc, wp = self.weights
-        N.call("pde_heat_step_f32", N.dev_ptr(u), N.dev_ptr(nxt), X, Y, Z, int(u.shape[3]),
-               self.ghost_cells, wm, wc, wp, self.alpha, self.dt, N.stream_of(u))
+        self.step_into(u, nxt, t)
         return nxt
 
 
