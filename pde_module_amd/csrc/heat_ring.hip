@@ -78,9 +78,24 @@ __device__ __forceinline__ void heat_interior(const HeatRArgs &A, const HeatRGeo
     }
 }
 
+// op_value with the time value already loaded (t ? t[0] : 0): the same arithmetic
+__device__ __forceinline__ float op_value_tv(const pde_bc_op &op, float tv)
+{
+    if (op.tmax > 0.0f) {
+        float ramp = tv / op.tmax;
+        ramp = ramp < 1.0f ? ramp : 1.0f;
+        return ramp * op.value;
+    }
+    return op.value;
+}
+
 __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &pg, int pid,
                                           float *lds)
 {
+    // the time value and the piece record load together; each thread's first
+    // boundary entry loads with the staging of the field and its op before the
+    // barrier, so the chain before the boundary evaluation is three loads deep
+    const float tv = A.t ? A.t[0] : 0.0f;
     const pde_ring_piece pc = A.ring.pieces[pid];
     const int64_t P1 = pg.P1, P2 = pg.P2, g = pg.g;
     const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -91,25 +106,31 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
     const int nI = wI * hI;
     if (wI <= 0 || hI <= 0 || nI > HR_CAP) return;   // the host never builds such a piece
     float *U = lds, *U2 = lds + HR_CAP;
+    // node indices fit in 32 bits (P1 * P2 <= 2^30, checked at launch)
+    const uint32_t P2u = (uint32_t)P2;
     auto lidx = [&](int64_t node) -> int {
-        const int64_t x = node / P2, k = node - x * P2;
+        const uint32_t n = (uint32_t)node;
+        const int64_t x = n / P2u, k = n - (uint32_t)x * P2u;
         if (x < xI0 || x >= xI1 || k < kI0 || k >= kI1) return -1;
         return (int)((x - xI0) * wI + (k - kI0));
     };
+    const bool has0 = tid < pc.w_end - pc.w_beg;
+    pde_ring_entry w0{};
+    if (has0) w0 = A.ring.w[pc.w_beg + tid];
     for (int i = tid; i < nI; i += NT) {
         const int64_t x = xI0 + i / wI, k = kI0 + i % wI;
         const float v = A.u[x * P2 + k];
         U[i] = v;
         U2[i] = v;
     }
+    pde_bc_op op0{};
+    if (has0) op0 = A.ops[w0.a];
     __syncthreads();
     // the boundary on the piece grown by one, every value from the untouched U
-    for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
-        const pde_ring_entry w = A.ring.w[e];
+    auto eval = [&](const pde_ring_entry &w, const pde_bc_op &op) {
         const int l = lidx(w.node);
-        if (l < 0) continue;
-        const pde_bc_op op = A.ops[w.a];
-        const float v = op_value(op, A.t);
+        if (l < 0) return;
+        const float v = op_value_tv(op, tv);
         const int ls = op.kind == PDE_BC_SET ? 0 : lidx(op.src);
         const float in = ls >= 0 ? U[ls] : 0.0f;
         float r;
@@ -120,6 +141,11 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         default: r = op.coef * v + in; break;
         }
         U2[l] = r;
+    };
+    if (has0) eval(w0, op0);
+    for (int e = pc.w_beg + tid + NT; e < pc.w_end; e += NT) {
+        const pde_ring_entry w = A.ring.w[e];
+        eval(w, A.ops[w.a]);
     }
     __syncthreads();
     const float wm = A.wm, wc = A.wc, wp = A.wp, dt = A.dt, alpha = A.alpha;
@@ -141,11 +167,12 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         A.un[x * P2 + k] = o;
     }
     // the written scalars of the piece itself back into u
-    for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
-        const pde_ring_entry w = A.ring.w[e];
+    auto back = [&](const pde_ring_entry &w) {
         const int l = lidx(w.node);
         if (l >= 0 && (w.flags & 2)) A.u[w.node] = U2[l];
-    }
+    };
+    if (has0) back(w0);
+    for (int e = pc.w_beg + tid + NT; e < pc.w_end; e += NT) back(A.ring.w[e]);
 }
 
 __global__ __launch_bounds__(64 * HR_BY) void heat_ring_kernel(HeatRArgs A, HeatRGeom pg)
