@@ -1,5 +1,5 @@
 """Host checks of the boundary ring plan behind the time-reversed wave
-gradient (``pde_module_amd.ring_plan``): the pieces tile exactly the nodes
+gradient and the one-launch heat step (``pde_module_amd.ring_plan``): the pieces tile exactly the nodes
 closer than ``depth`` to the edge, every entry the kernel applies lies where
 the piece can reach it, and tables the kernel cannot handle are refused."""
 import numpy as np
@@ -10,7 +10,7 @@ from pde_module_amd import _native as N
 from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
 from pde_module_amd.FDM import GridBoundary
 from pde_module_amd.FDM.boundary.tables import chunk_ops, transpose_table
-from pde_module_amd.ring_plan import RING_CAP, build_ring_plan, ring_pieces
+from pde_module_amd.ring_plan import RING_CAP, build_ring_plan, check_step_sources, ring_pieces
 
 
 def _ops(shape, make):
@@ -113,3 +113,31 @@ def test_pieces_keep_corner_pairs_together():
         pcs = ring_pieces(40, P2, 4)
         top = [p for p in pcs if p[0] == 0]
         assert all(c0 == 0 or (4 <= c0 <= P2 - 4) for (_, _, c0, _) in top)
+
+
+@pytest.mark.parametrize("shape", [(9, 40, 1), (34, 34, 1), (61, 525, 1), (300, 77, 1)])
+@pytest.mark.parametrize("make", [LEGACY, _mixed, _functional])
+def test_step_sources_of_standard_tables_fit_the_one_launch_heat_step(shape, make):
+    ops = _ops(shape, make)
+    plan = build_ring_plan(ops, shape[0], shape[1], 1)
+    check_step_sources(plan, ops, shape[0], shape[1])
+
+
+def test_step_sources_refuse_a_read_of_a_scalar_another_piece_writes():
+    """A ghost on a piece cut reading the next ghost along: the ring plan takes
+    it (its transpose stays in the owning piece), but the piece on the other
+    side of the cut also evaluates that op and would stage the source while its
+    owner writes the boundary value back, so the one-launch heat step refuses
+    the table (``HeatStep`` then takes the two-launch path)."""
+    P1, P2 = 40, 134
+    ops = _ops((P1, P2, 1), LEGACY)
+    plan = build_ring_plan(ops, P1, P2, 1)
+    check_step_sources(plan, ops, P1, P2)
+    cut = min(int(p["c0"]) for p in plan["pieces"] if p["r0"] == 0 and p["c0"] > 0)
+    bad = ops.copy()
+    i = np.flatnonzero((bad["kind"] != N.BC_SET) & (bad["dst"] == cut))
+    assert len(i) == 1
+    bad["src"][i[0]] = cut + 1
+    plan2 = build_ring_plan(bad, P1, P2, 1)
+    with pytest.raises(ValueError):
+        check_step_sources(plan2, bad, P1, P2)
