@@ -1,7 +1,8 @@
 """Host checks of the boundary ring plan behind the time-reversed wave
-gradient and the one-launch heat step (``pde_module_amd.ring_plan``): the pieces tile exactly the nodes
-closer than ``depth`` to the edge, every entry the kernel applies lies where
-the piece can reach it, and tables the kernel cannot handle are refused."""
+gradient and the one-launch heat step (``pde_module_amd.ring_plan``): the
+pieces tile exactly the nodes closer than ``depth`` to the edge, every entry
+the kernel applies lies where the piece can reach it, and tables the kernel
+cannot handle are refused."""
 import numpy as np
 import pytest
 import torch
