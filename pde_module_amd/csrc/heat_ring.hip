@@ -54,17 +54,21 @@ __device__ __forceinline__ void heat_interior(const HeatRArgs &A, const HeatRGeo
     v4 rows[HR_ROWS + 2];
 #pragma unroll
     for (int r = 0; r < HR_ROWS + 2; ++r) rows[r] = ld_row4(A.u, t, x0 - 1 + r, k0);
+    // every row's column edges before any store (u and un may alias as far as
+    // the compiler knows, so a load after a store would wait for its turn)
+    float left[HR_ROWS], right[HR_ROWS];
+#pragma unroll
+    for (int r = 0; r < HR_ROWS; ++r)
+        col_edges(A.u, t, x0 + r, k0, rows[r + 1], lane, 64, left[r], right[r]);
 #pragma unroll
     for (int r = 0; r < HR_ROWS; ++r) {
         const int64_t x = x0 + r;
-        float left, right;
-        col_edges(A.u, t, x, k0, rows[r + 1], lane, 64, left, right);
         if (!lane_ok || x >= xend) continue;
         v4 o;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i, A.wm, A.wc,
-                                   A.wp);
+            const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left[r], right[r], i, A.wm,
+                                   A.wc, A.wp);
             set_el(o, i, el(rows[r + 1], i) + A.dt * (A.alpha * acc));
         }
         if (k0 + 4 <= kend) {

@@ -204,19 +204,22 @@ __device__ __forceinline__ void heat_rows(const float *__restrict__ u, float *__
     v4 rows[RXT + 2];
 #pragma unroll
     for (int r = 0; r < RXT + 2; ++r) rows[r] = ld_row4(u, t, x0 - 1 + r, k0);
+    // every row's column edges before any store, so their loads issue together
+    float left[RXT], right[RXT];
+#pragma unroll
+    for (int r = 0; r < RXT; ++r)
+        col_edges(u, t, x0 + r, k0, rows[r + 1], sl, cw, left[r], right[r]);
 #pragma unroll
     for (int r = 0; r < RXT; ++r) {
         const int64_t x = x0 + r;
-        float left, right;
-        col_edges(u, t, x, k0, rows[r + 1], sl, cw, left, right);
         if (!lane_ok || x >= t.P1) continue;
         const bool xin = inside(x, t.g, t.P1);
         v4 o = rows[r + 1];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (xin && kin[i]) {
-                const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left, right, i, wm, wc,
-                                       wp);
+                const float acc = acc5(rows[r], rows[r + 1], rows[r + 2], left[r], right[r], i,
+                                       wm, wc, wp);
                 set_el(o, i, el(rows[r + 1], i) + dt * (alpha * acc));
             }
         st_row4(un, t, x, k0, o);
