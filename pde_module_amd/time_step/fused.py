@@ -41,7 +41,9 @@ class HeatStep:
         # one_launch=False keeps the boundary launch and the step launch apart
         # (same values; used to check the one-launch form)
         self.one_launch = bool(one_launch)
-        self._ring = None   # (key, DeviceRingPlan or None)
+        # (key, DeviceRingPlan or None, table): the table is held so that its id,
+        # part of the key, cannot be reused by a later table while cached
+        self._ring = None
 
     def _ring_plan(self, u: torch.Tensor):
         """The ring plan of the one-launch step for u's shape and the
