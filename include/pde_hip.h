@@ -355,7 +355,8 @@ typedef struct pde_ring_plan {
  * on interior nodes, u2 elsewhere; the same values as the two launches.  ring
  * is the field's boundary ring (pde_ring_plan, built by the host for the op
  * table and checked so that each piece's ops read inside the piece and no
- * scalar another piece writes).  No reference counterpart as one call: it
+ * scalar another piece writes; a plan entry whose op slot is not below nops
+ * is skipped).  No reference counterpart as one call: it
  * replaces GridBoundary + Laplacian + ForwardEuler of
  * examples/2D_Heat_Equation.py:58-62 with one launch. */
 int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t ny, int64_t nz,

@@ -21,6 +21,7 @@ constexpr int HR_CAP = 448;     // nodes of a ring piece grown by two
 struct HeatRArgs {
     float *u, *un;
     const pde_bc_op *ops;
+    int64_t nops;   // entries whose op slot is not below nops are skipped
     const float *t;
     float wm, wc, wp, alpha, dt;
     pde_ring_plan ring;
@@ -128,7 +129,8 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         U2[i] = v;
     }
     pde_bc_op op0{};
-    if (has0) op0 = A.ops[w0.a];
+    const bool ok0 = has0 && w0.a >= 0 && w0.a < A.nops;
+    if (ok0) op0 = A.ops[w0.a];
     __syncthreads();
     // the boundary on the piece grown by one, every value from the untouched U
     auto eval = [&](const pde_ring_entry &w, const pde_bc_op &op) {
@@ -146,10 +148,10 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         }
         U2[l] = r;
     };
-    if (has0) eval(w0, op0);
+    if (ok0) eval(w0, op0);
     for (int e = pc.w_beg + tid + NT; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
-        eval(w, A.ops[w.a]);
+        if (w.a >= 0 && w.a < A.nops) eval(w, A.ops[w.a]);
     }
     __syncthreads();
     const float wm = A.wm, wc = A.wc, wp = A.wp, dt = A.dt, alpha = A.alpha;
@@ -221,7 +223,7 @@ extern "C" int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t n
     pg.nrt = (uint32_t)nr;
     pg.nint = (uint32_t)nint;
     pg.nring = (uint32_t)nring;
-    HeatRArgs A{u, un, ops, t, wm, wc, wp, alpha, dt, *ring};
+    HeatRArgs A{u, un, ops, nops, t, wm, wc, wp, alpha, dt, *ring};
     hipLaunchKernelGGL(heat_ring_kernel, dim3((unsigned)(nint + nring)), dim3(64, HR_BY), 0,
                        as_stream(stream), A, pg);
     return launch_status();
