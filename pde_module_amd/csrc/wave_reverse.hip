@@ -723,7 +723,7 @@ __device__ __forceinline__ void pair_ring(const PairArgs &A, const PairGeom &pg,
     for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
         const int l = lidx(w.node);
-        if (l >= 0) UN[l] = A.save[n * A.nops + w.a];
+        if (l >= 0 && w.a >= 0 && w.a < A.nops) UN[l] = A.save[n * A.nops + w.a];
     }
     for (int e = pc.t_beg + tid; e < pc.t_end; e += NT) {
         const pde_ring_entry w = A.ring.t[e];
@@ -804,7 +804,8 @@ __device__ __forceinline__ void pair_ring(const PairArgs &A, const PairGeom &pg,
     for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
         const int l = lidx(w.node);
-        if (l >= 0 && (w.flags & 2)) U[l] = A.save[(n - 1) * A.nops + w.a];
+        if (l >= 0 && (w.flags & 2) && w.a >= 0 && w.a < A.nops)
+            U[l] = A.save[(n - 1) * A.nops + w.a];
     }
     for (int e = pc.t_beg + tid; e < pc.t_end; e += NT) {
         const pde_ring_entry w = A.ring.t[e];
