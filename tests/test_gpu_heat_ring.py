@@ -109,3 +109,23 @@ def test_heat_rollout_forward_matches_two_launch_loop(dev):
             assert bits_equal(ckpt[n + 1], ub), f"checkpoint {n + 1}"
     torch.cuda.synchronize()
     assert bits_equal(uN, ub)
+
+
+def test_one_launch_heat_step_reads_a_device_time_tensor(dev):
+    """The ramp's time as a 1-element FP32 device tensor (the form a captured
+    loop updates in place) gives the same states as the two launches."""
+    shape, g = (90, 130), 1
+    rng = np.random.default_rng(13)
+    u0 = torch.from_numpy(rng.uniform(-1, 1, shape + (1, 1)).astype(np.float32)).to(dev)
+    dx, alpha = 0.01, 0.05
+    dt = 0.2 * dx * dx / alpha
+    one = HeatStep(_ramp(u0, dx, g), dx, g, alpha, dt)
+    two = HeatStep(_ramp(u0, dx, g), dx, g, alpha, dt, one_launch=False)
+    t = torch.zeros(1, dtype=torch.float32, device=dev)
+    ua, ub = u0.clone(), u0.clone()
+    for n in range(5):
+        t.fill_(0.7 * n)
+        ua, ub = one(ua, t), two(ub, t)
+        torch.cuda.synchronize()
+        assert one._ring[1] is not None
+        assert bits_equal(ua, ub), f"step {n}"
