@@ -343,6 +343,9 @@ typedef struct pde_ring_entry {
 
 typedef struct pde_ring_plan {
     int32_t depth, npieces;
+    int64_t nops; /* ops in the table the plan was built for; every entry's op
+                     slot is below it, and the entry points refuse a table of
+                     any other size (PDE_EINVAL) */
     const pde_ring_piece *pieces;
     const pde_ring_entry *w, *t, *z;
     const int32_t *cdst;
@@ -355,8 +358,8 @@ typedef struct pde_ring_plan {
  * on interior nodes, u2 elsewhere; the same values as the two launches.  ring
  * is the field's boundary ring (pde_ring_plan, built by the host for the op
  * table and checked so that each piece's ops read inside the piece and no
- * scalar another piece writes; a plan entry whose op slot is not below nops
- * is skipped).  No reference counterpart as one call: it
+ * scalar another piece writes; PDE_EINVAL unless ring->nops == nops).  No
+ * reference counterpart as one call: it
  * replaces GridBoundary + Laplacian + ForwardEuler of
  * examples/2D_Heat_Equation.py:58-62 with one launch. */
 int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t ny, int64_t nz,
@@ -377,8 +380,9 @@ int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t ny, int64_t 
  * LDS pieces, all in one launch.  The gradients g_u, g_v are those of two
  * pde_wave_step_adjoint_fused_f32 + boundary-transpose steps bit for bit; gc2
  * differs from the recomputed-state result only through the rounding of the
- * rebuilt states.  Never the first step of the loop (n >= 1).  No reference
- * counterpart. */
+ * rebuilt states.  Never the first step of the loop (n >= 1).  ring must be
+ * the plan of the op table whose scalars save holds (PDE_EINVAL unless
+ * ring->nops == nops).  No reference counterpart. */
 int pde_wave_reverse_adjoint_pair_f32(const float *u, const float *v, const float *gu,
                                       const float *gv, const float *c2, float *uo, float *vo,
                                       float *guo, float *gvo, float *gc2, int64_t nx,

@@ -183,7 +183,8 @@ def immersed_boundary(u, bitmask, btype, bvalue, dx, fill_value=0.0):
     (axes ascending, side -1 then +1) of u[nb] - (s * v) * dx.  Interior solid
     nodes take fill_value.  Everything else is u.  Neighbours are flat index
     +- the axis stride (the reference indexes without a bounds check, so on the
-    outermost layer a row index -1 is the previous row's last node)."""
+    outermost layer a row index -1 is the previous row's last node); a
+    neighbour outside the whole array raises ValueError."""
     dt = u.dtype.type
     out = np.array(u, copy=True)
     shape3 = bitmask.shape
@@ -193,8 +194,15 @@ def immersed_boundary(u, bitmask, btype, bvalue, dx, fill_value=0.0):
     C = u.shape[3]
     uflat = u.reshape(-1, C)
     oflat = out.reshape(-1, C)
-    # neighbours by flat index +- axis stride, the reference's unchecked indexing
+    # neighbours by flat index +- axis stride, the reference's unchecked indexing;
+    # an index outside the whole array (which NumPy would wrap) is an error, as
+    # in the product's ImmersedBoundary.finalize
     solid = [int(f) for f in np.flatnonzero(bflat)]
+    for f in solid:
+        for a in elig:
+            for s in (-1, 1):
+                if not 0 <= f + s * strides[a] < bflat.size:
+                    raise ValueError("a solid node's neighbour falls outside the array")
     bnd, inner = [], []
     for f in solid:
         is_b = any(bflat[f + s * strides[a]] == 0 for a in elig for s in (-1, 1))

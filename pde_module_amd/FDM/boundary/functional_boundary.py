@@ -141,6 +141,13 @@ class GridBoundary(ExplicitUniformGridStencil):
         self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
 
     @before_forward
+    @staticmethod
+    def _check_time_array(t: torch.Tensor, u: torch.Tensor) -> None:
+        """A caller's time array is read on the device by the boundary kernels:
+        it must be one float32 value on the field's device."""
+        if t.numel() != 1 or t.dtype != torch.float32 or t.device != u.device:
+            raise AssertionError("t must be a 1-element float32 array on the field's device")
+
     def set_default_params(self, input_array, t, params, **kwargs) -> None:
         for key in self.func_groups:
             params.setdefault(key, 0)
@@ -148,8 +155,7 @@ class GridBoundary(ExplicitUniformGridStencil):
             self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
         if t is not None:
             if isinstance(t, torch.Tensor):
-                if t.numel() != 1 or t.dtype != torch.float32 or t.device != input_array.device:
-                    raise AssertionError("t must be a 1-element float32 array on the field's device")
+                self._check_time_array(t, input_array)
                 self.t = t
             else:
                 self.t.fill_(float(np.float32(t)))
@@ -165,6 +171,7 @@ class GridBoundary(ExplicitUniformGridStencil):
     def _time_tensor(self, u: torch.Tensor, t=None):
         """The 1-element device time array the ramp values read."""
         if isinstance(t, torch.Tensor):
+            self._check_time_array(t, u)
             return t
         if not hasattr(self, "t") or self.t.device != u.device:
             self.t = torch.zeros(1, dtype=torch.float32, device=u.device)

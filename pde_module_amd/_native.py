@@ -118,10 +118,10 @@ RING_PIECE_DTYPE = np.dtype([("r0", "<i4"), ("r1", "<i4"), ("c0", "<i4"), ("c1",
 assert RING_PIECE_DTYPE.itemsize == 48
 RING_ENTRY_DTYPE = np.dtype([("node", "<i4"), ("a", "<i4"), ("b", "<i4"), ("flags", "<i4")])
 assert RING_ENTRY_DTYPE.itemsize == 16
-RING_PLAN_DTYPE = np.dtype([("depth", "<i4"), ("npieces", "<i4"), ("pieces", "<u8"),
-                            ("w", "<u8"), ("t", "<u8"), ("z", "<u8"), ("cdst", "<u8"),
-                            ("csign", "<u8")])
-assert RING_PLAN_DTYPE.itemsize == 56
+RING_PLAN_DTYPE = np.dtype([("depth", "<i4"), ("npieces", "<i4"), ("nops", "<i8"),
+                            ("pieces", "<u8"), ("w", "<u8"), ("t", "<u8"), ("z", "<u8"),
+                            ("cdst", "<u8"), ("csign", "<u8")])
+assert RING_PLAN_DTYPE.itemsize == 64
 
 # struct pde_point_set (pointers as integers)
 POINT_SET_DTYPE = np.dtype([("row_start", "<u8"), ("col", "<u8"), ("slot", "<u8"),

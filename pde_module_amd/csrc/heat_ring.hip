@@ -129,8 +129,7 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         U2[i] = v;
     }
     pde_bc_op op0{};
-    const bool ok0 = has0 && w0.a >= 0 && w0.a < A.nops;
-    if (ok0) op0 = A.ops[w0.a];
+    if (has0) op0 = A.ops[w0.a];
     __syncthreads();
     // the boundary on the piece grown by one, every value from the untouched U
     auto eval = [&](const pde_ring_entry &w, const pde_bc_op &op) {
@@ -148,10 +147,10 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
         }
         U2[l] = r;
     };
-    if (ok0) eval(w0, op0);
+    if (has0) eval(w0, op0);
     for (int e = pc.w_beg + tid + NT; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
-        if (w.a >= 0 && w.a < A.nops) eval(w, A.ops[w.a]);
+        eval(w, A.ops[w.a]);
     }
     __syncthreads();
     const float wm = A.wm, wc = A.wc, wp = A.wp, dt = A.dt, alpha = A.alpha;
@@ -203,6 +202,9 @@ extern "C" int pde_heat_step_ring_f32(float *u, float *un, int64_t nx, int64_t n
 {
     if (!u || !un || u == un || nops < 0 || (nops > 0 && !ops)) return PDE_EINVAL;
     if (!ring || ring->npieces < 1 || !ring->pieces || !ring->w) return PDE_EINVAL;
+    // every plan entry's op slot is below the plan's nops (host-built): the
+    // table passed must be the one the plan was built for
+    if (ring->nops != nops) return PDE_EINVAL;
     Geom G;
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;

@@ -723,7 +723,7 @@ __device__ __forceinline__ void pair_ring(const PairArgs &A, const PairGeom &pg,
     for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
         const int l = lidx(w.node);
-        if (l >= 0 && w.a >= 0 && w.a < A.nops) UN[l] = A.save[n * A.nops + w.a];
+        if (l >= 0) UN[l] = A.save[n * A.nops + w.a];
     }
     for (int e = pc.t_beg + tid; e < pc.t_end; e += NT) {
         const pde_ring_entry w = A.ring.t[e];
@@ -804,8 +804,7 @@ __device__ __forceinline__ void pair_ring(const PairArgs &A, const PairGeom &pg,
     for (int e = pc.w_beg + tid; e < pc.w_end; e += NT) {
         const pde_ring_entry w = A.ring.w[e];
         const int l = lidx(w.node);
-        if (l >= 0 && (w.flags & 2) && w.a >= 0 && w.a < A.nops)
-            U[l] = A.save[(n - 1) * A.nops + w.a];
+        if (l >= 0 && (w.flags & 2)) U[l] = A.save[(n - 1) * A.nops + w.a];
     }
     for (int e = pc.t_beg + tid; e < pc.t_end; e += NT) {
         const pde_ring_entry w = A.ring.t[e];
@@ -888,6 +887,9 @@ extern "C" int pde_wave_reverse_adjoint_pair_f32(
     if (nops < 0 || (nops > 0 && !save)) return PDE_EINVAL;
     if (!ring || ring->npieces < 1 || !ring->pieces) return PDE_EINVAL;
     if ((ring->w == nullptr) || (ring->t == nullptr) || (ring->z == nullptr)) return PDE_EINVAL;
+    // save rows and plan entries are indexed by op slot: the plan must be the
+    // one built for this table (every entry's slot is below ring->nops)
+    if (ring->nops != nops) return PDE_EINVAL;
     Geom G;
     int st = make_geom(nx, ny, nz, ghost, &G);
     if (st) return st;

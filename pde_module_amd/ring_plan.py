@@ -59,7 +59,8 @@ def ring_pieces(P1, P2, D):
 def build_ring_plan(ops, P1, P2, ghost):
     """Host arrays of the ring plan for the chunk-ordered op table ``ops``
     (BC_OP_DTYPE, scalar field of padded extents P1 x P2).  Returns a dict
-    with depth, pieces (RING_PIECE_DTYPE), w/t/z (RING_ENTRY_DTYPE), cdst
+    with depth, nops (the table's op count, which the kernels' entry points
+    check against the table they are given), pieces (RING_PIECE_DTYPE), w/t/z (RING_ENTRY_DTYPE), cdst
     (int32), csign (float32)."""
     P1, P2, ghost = int(P1), int(P2), int(ghost)
     if P1 * P2 > 0x3FFFFFFF:
@@ -131,7 +132,7 @@ def build_ring_plan(ops, P1, P2, ghost):
         pieces.append(rec)
 
     ent = lambda L: np.array(L if L else [(0, 0, 0, 0)], dtype=N.RING_ENTRY_DTYPE)
-    return dict(depth=D, pieces=np.array(pieces, dtype=N.RING_PIECE_DTYPE), w=ent(W),
+    return dict(depth=D, nops=len(ops), pieces=np.array(pieces, dtype=N.RING_PIECE_DTYPE), w=ent(W),
                 t=ent(Tt), z=ent(Z),
                 cdst=np.ascontiguousarray(cdst if len(cdst) else np.zeros(1), dtype=np.int32),
                 csign=np.ascontiguousarray(T["csign"] if len(T["csign"]) else np.zeros(1),
@@ -147,10 +148,11 @@ class DeviceRingPlan:
         up = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(device)
         self.depth = plan["depth"]
         self.npieces = len(plan["pieces"])
+        self.nops = int(plan["nops"])
         self._arrs = {k: up(plan[k]) for k in ("pieces", "w", "t", "z", "cdst", "csign")}
         rec = np.zeros(1, N.RING_PLAN_DTYPE)
-        rec[0] = (self.depth, self.npieces, *(self._arrs[k].data_ptr() for k in
-                                              ("pieces", "w", "t", "z", "cdst", "csign")))
+        rec[0] = (self.depth, self.npieces, self.nops,
+                  *(self._arrs[k].data_ptr() for k in ("pieces", "w", "t", "z", "cdst", "csign")))
         self.rec = rec
 
     @property

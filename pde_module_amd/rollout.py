@@ -168,17 +168,26 @@ class WaveRollout:
         self._saving = False
         self._shape = None
         self._c2 = None
+        # the boundary table the buffers, ring plan and graphs were built for;
+        # held so that its device arrays (which captured graphs point at) live
+        # as long as they are used
+        self._table = None
         self._graphs = {}
 
     # ---------------------------------------------------------------- buffers
     def _prepare(self, like, c2):
         # device tables must exist before any graph capture (no uploads inside)
         self.bc._ensure_table(like)
-        self.bc.table.prepare_adjoint()
+        table = self.bc.table
+        table.prepare_adjoint()
         shape = tuple(like.shape)
         if self._shape == shape and self._dev == like.device and \
-                (self._c2 is None) == (c2 is None):
+                (self._c2 is None) == (c2 is None) and self._table is table:
             return
+        # a new boundary table (set_BC, shift_group, ...) invalidates the
+        # captured graphs and the reversed walk's ring plan and save buffer
+        self._graphs = {}
+        self._table = table
         dev = like.device
         self._shape, self._dev = shape, dev
         E = lambda *s: torch.empty(s + shape, dtype=like.dtype, device=dev)
@@ -452,6 +461,9 @@ class WaveRollout:
         return uN.clone(), vN.clone(), rec.clone()
 
     def backward_states(self, c2, gu, gv, grec, need_c2):
+        if self.bc.table is not self._table:
+            raise RuntimeError("the boundary conditions changed between the rollout's forward "
+                               "and backward passes")
         self.bc.table.prepare_adjoint()
         if c2 is not None:
             self._c2.copy_(c2)

@@ -77,3 +77,37 @@ def test_argument_errors_need_no_gpu():
     assert lib.pde_bc_apply_f32(fake, fake, 10, None, 5, None, None) == -1
     assert lib.pde_wave_step_f32(fake, fake, None, fake, fake, 8, 8, 1, 1, 1.0, -2.0, 1.0,
                                  1.0, 0.1, 0, None) == -1
+
+
+def test_ring_plan_record_layout_and_op_count_check():
+    """The plan record matches the header's struct, and both ring entry points
+    refuse a plan built for another op table (its nops differs from the table
+    passed) on the host, before any launch."""
+    text = open(os.path.join(ROOT, "include", "pde_hip.h")).read()
+    body = re.search(r"typedef struct pde_ring_plan \{(.*?)\} pde_ring_plan;", text,
+                     flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"(\w+)\s*[,;]", body)
+    assert names == list(N.RING_PLAN_DTYPE.names)
+    import torch
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    from pde_module_amd.FDM.boundary.tables import chunk_ops
+    from pde_module_amd.ring_plan import build_ring_plan
+    ops = chunk_ops(LegacyGridBoundary(torch.zeros((34, 34, 1, 1)), 0.01, 1).resolved_ops())[0]
+    plan = build_ring_plan(ops, 34, 34, 1)
+    assert plan["nops"] == len(ops)
+    lib = N.load()
+    pieces = np.zeros(4, np.uint8)
+    rec = np.zeros(1, N.RING_PLAN_DTYPE)
+    fake = pieces.ctypes.data          # never dereferenced: the op count check fails first
+    rec[0] = (plan["depth"], len(plan["pieces"]), len(ops) + 1, fake, fake, fake, fake, fake,
+              fake)
+    u, un, ops_p, t, save = (ctypes.c_void_p(16 * k) for k in (1, 2, 3, 4, 5))
+    assert lib.pde_heat_step_ring_f32(u, un, 34, 34, 1, 1, ops_p, len(ops), None, 1.0, -2.0,
+                                      1.0, 0.01, 0.1, rec.ctypes.data, None) == -1
+    step = np.zeros(2, np.int64)
+    assert lib.pde_wave_reverse_adjoint_pair_f32(
+        u, un, ctypes.c_void_p(48), ctypes.c_void_p(64), None, ctypes.c_void_p(80),
+        ctypes.c_void_p(96), ctypes.c_void_p(112), ctypes.c_void_p(128), None, 34, 34, 1, 1,
+        1.0, -2.0, 1.0, 1.0, 0.1, step.ctypes.data, step[1:].ctypes.data, None, None, save,
+        len(ops), rec.ctypes.data, None) == -1

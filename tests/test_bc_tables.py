@@ -207,3 +207,18 @@ def test_chunking_refuses_oversized_components():
     ops["kind"] = BC_COPY
     assert chunk_ops(ops) is None
     assert chunk_ops(ops[:200]) is not None
+
+
+def test_caller_time_array_is_checked_before_any_launch():
+    """The one-launch steps read a caller's time array on the device: a
+    wrong dtype, size or device is refused on the host (the same check as the
+    module call's), never handed to a kernel."""
+    u = field((6, 6, 1), 1)
+    bc = GridBoundary(u, 0.1, 1)
+    bc.dirichlet_BC("ALL", get_ramp_func(1.0, 2.0))
+    for bad in (torch.zeros(1, dtype=torch.float64), torch.zeros(2),
+                torch.zeros(1, device="meta")):
+        with pytest.raises(AssertionError):
+            bc._time_tensor(u, bad)
+    good = torch.zeros(1)
+    assert bc._time_tensor(u, good) is good

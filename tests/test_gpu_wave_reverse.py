@@ -188,3 +188,47 @@ def test_reverse_gradient_vs_central_differences(dev):
     gd = sum(float((g.detach().double().cpu() * d).sum()) for g, d in zip(b["grads"], dirs))
     print(f"reverse gradient . d = {gd:.8e}, central difference {fd:.8e}", flush=True)
     assert abs(gd - fd) <= 1e-4 * abs(fd)
+
+
+def test_boundary_change_between_graph_replayed_rollouts(dev):
+    """A WaveRollout reused after its boundary's table was rebuilt (a new
+    vonNeumann_BC) drops its captured graphs, ring plan and save buffer: the
+    second rollout equals, bit for bit, a fresh rollout with the new
+    boundary; changing the boundary between a forward and its backward
+    raises instead of walking back with the old records."""
+    print("wave reverse: boundary change between graph-replayed rollouts", flush=True)
+    shape, dx, dt, c2, u0, v0, src, nsrc, rec, make_bc = _case("wide")
+    steps, K = 48, 12
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    wav = np.stack([_ricker(steps, dt, dx, a) for a in (100.0, 30.0, -20.0, 10.0)[:nsrc]], 1)
+
+    def grads(ro):
+        c = T(c2).requires_grad_(True)
+        x = T(u0).requires_grad_(True)
+        y = T(v0).requires_grad_(True)
+        uN, vN, r = ro(c, x, y)
+        J = (r ** 2).sum() + (uN * uN).sum() * 1e-3
+        out = (r.detach(), uN.detach()) + torch.autograd.grad(J, (c, x, y))
+        torch.cuda.synchronize()
+        return out
+
+    def rollout(bc):
+        return WaveRollout(bc, dx, 1, float(dt), steps, checkpoint_every=K, src_idx=T(src),
+                           src_wavelet=T(wav), rec_idx=T(rec), use_graph=True, reverse=True)
+
+    bc = make_bc(T(u0))
+    ro = rollout(bc)
+    first = grads(ro)
+    bc.vonNeumann_BC("+X", 0.75)       # rebuilds the table (the old one is released)
+    second = grads(ro)
+    bc2 = make_bc(T(u0))
+    bc2.vonNeumann_BC("+X", 0.75)
+    fresh = grads(rollout(bc2))
+    for a, b in zip(second, fresh):
+        assert bits_equal(a, b)
+    assert not bits_equal(first[0], second[0])
+    c = T(c2).requires_grad_(True)
+    uN, vN, r = ro(c, T(u0), T(v0))
+    bc.vonNeumann_BC("+X", 0.5)
+    with pytest.raises(RuntimeError):
+        torch.autograd.grad((r ** 2).sum(), c)

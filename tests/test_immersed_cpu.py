@@ -81,3 +81,34 @@ def test_outer_layer_neighbours_use_flat_index_arithmetic():
     ib2.bitmask[0, 0, 0] = 1
     with pytest.raises(ValueError):
         ib2.finalize()
+
+
+def test_oracle_rejects_what_the_module_rejects():
+    """A solid node on the first row of the first plane has its -X and -Y
+    neighbours before the array's start: the module's finalize and the oracle
+    both raise instead of wrapping the index; a solid in ghost row 0 further in
+    is accepted by both and treated alike."""
+    u = np.zeros((6, 5, 1, 1))
+    bm = np.zeros((6, 5, 1), np.int8)
+    bm[0, 0, 0] = 1
+    ib = ImmersedBoundary(torch.zeros((6, 5, 1, 1)), 0.1, 1)
+    ib.bitmask[...] = bm
+    with pytest.raises(ValueError):
+        ib.finalize()
+    with pytest.raises(ValueError):
+        O.immersed_boundary(u, bm, np.ones((1, 1), np.int8), np.zeros((1, 1)), 0.1)
+    bm2 = np.zeros((6, 5, 1), np.int8)
+    bm2[-1, 4, 0] = 1                 # +X neighbour after the array's end
+    ib3 = ImmersedBoundary(torch.zeros((6, 5, 1, 1)), 0.1, 1)
+    ib3.bitmask[...] = bm2
+    with pytest.raises(ValueError):
+        ib3.finalize()
+    with pytest.raises(ValueError):
+        O.immersed_boundary(u, bm2, np.ones((1, 1), np.int8), np.zeros((1, 1)), 0.1)
+    bm4 = np.zeros((6, 5, 1), np.int8)
+    bm4[2, 0, 0] = 1
+    ib4 = ImmersedBoundary(torch.zeros((6, 5, 1, 1)), 0.1, 1)
+    ib4.bitmask[...] = bm4
+    ib4.finalize()
+    out = O.immersed_boundary(u + 1.0, bm4, np.ones((1, 1), np.int8), np.full((1, 1), 2.0), 0.1)
+    assert out[2, 0, 0, 0] == 2.0 and len(ib4.solid_boundary) == 1
