@@ -40,6 +40,9 @@ FUSED_BYTES = 36       # read s + vec3, write lap + vec3 + div
 WAVE_PAIR_BYTES = 44
 
 
+PREWARM_S = 0.3     # untimed seconds of the step before the warm-up steps
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -119,19 +122,24 @@ def cpu_baseline(seconds):
     return res
 
 
-def wave_adjoint_timing(dev, steps=2000, K=50):
+def wave_adjoint_timing(dev, steps=2000, K=50, rec_distance=None, variants=True):
     """2D acoustic workload (2048^2, smoothed random c, Ricker source at the
     centre, 64 receivers, Dirichlet-0 walls, 2000 steps): time of the plain
     forward loop vs forward + adjoint with checkpoints every K steps, all as
     HIP-graph-replayed segments.  The checkpointed gradient walks each segment
     back from its end (WaveRollout(reverse=True)); the recompute form and the
-    store-every-state form are timed beside it.  J = sum of squared receiver
-    misfit against data from the unperturbed model (c = 1)."""
+    store-every-state form are timed beside it (variants=True).  J = sum of
+    squared receiver misfit against data from the unperturbed model (c = 1).
+
+    rec_distance=None puts the receivers on the wall row, which the front does
+    not reach in 2000 steps (J = 0); with a distance the wave arrives, and the
+    reversed gradient is checked against the recompute path's (records and
+    final state bit-identical, dJ/dc2 within 2e-5 relative)."""
     import torch
     from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
     from pde_module_amd.rollout import WaveRollout
     from pde_module_amd.workloads import wave_2d_config
-    cfg = wave_2d_config(seed=0, steps=steps)
+    cfg = wave_2d_config(seed=0, steps=steps, rec_distance=rec_distance)
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     c2 = T(cfg.c2)
     u0 = torch.zeros_like(c2)
@@ -141,7 +149,7 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
                                           checkpoint_every=k, src_idx=T(cfg.src_idx),
                                           src_wavelet=T(cfg.wavelet), rec_idx=T(cfg.rec_idx),
                                           use_graph=True, reverse=rev)
-    plain, ro, ro_all, ro_rev = mk(steps), mk(K), mk(None), mk(K, True)
+    plain, ro, ro_rev = mk(steps), mk(K), mk(K, True)
     _, _, d = plain.forward_states(torch.ones_like(c2), u0, v0)
     d = d.clone()
 
@@ -162,15 +170,50 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     def grad_of(rollout, key):
         def run():
             c = c2.clone().requires_grad_(True)
-            _, _, r = rollout(c, u0, v0)
+            uN, _, r = rollout(c, u0, v0)
             J = ((r - d) ** 2).sum()
             J.backward()
-            out[key] = (float(J.detach()), float(c.grad.abs().max()))
+            out[key] = (J.detach(), c.grad, r.detach(), uN.detach())
         return run
 
     t_rev = timed(grad_of(ro_rev, "rev"))
-    t_fa = timed(grad_of(ro, "rec"))
-    t_all = timed(grad_of(ro_all, "all"), reps=2)
+    res = {"grid": [cfg.n, cfg.n], "steps": steps, "checkpoint_every": K,
+           "receiver_row_from_source": int(cfg.rec_idx[0] % c2.shape[1]) - (cfg.g + cfg.n // 2),
+           "fwd_ms_per_step": round(t_fwd / steps * 1e3, 5),
+           "fwd_adj_ms_per_step": round(t_rev / steps * 1e3, 5),
+           "ratio": round(t_rev / t_fwd, 3),
+           "J": float(out["rev"][0]), "max_abs_dJ_dc2": float(out["rev"][1].abs().max()),
+           "mode": "checkpoint every K steps; the backward walks each segment back from the "
+                   "checkpoint at its end, two transposed steps per launch, rebuilding the "
+                   "states (boundary-overwritten scalars recorded by the forward) instead of "
+                   "recomputing the segment; the first segment is recomputed"}
+    if variants:
+        t_fa = timed(grad_of(ro, "rec"))
+        ro_all = mk(None)
+        t_all = timed(grad_of(ro_all, "all"), reps=2)
+        del ro_all
+        res["recompute"] = {"fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
+                            "ratio": round(t_fa / t_fwd, 3)}
+        res["store_all"] = {"fwd_adj_ms_per_step": round(t_all / steps * 1e3, 5),
+                            "ratio": round(t_all / t_fwd, 3),
+                            "history_GB": round((steps + 1) * c2.numel() * 4 / 1e9, 1)}
+        res["note"] = ("graph-replayed 50-step segments; recompute: each segment recomputed "
+                       "from its checkpoint in the backward; store_all: every state kept")
+    else:
+        grad_of(ro, "rec")()
+    if rec_distance is not None:
+        bits = lambda a, b: torch.equal(a.contiguous().view(torch.int32),
+                                        b.contiguous().view(torch.int32))
+        J, g, r, uN = out["rev"]
+        Jr, gr, rr, uNr = out["rec"]
+        e = float((g.double() - gr.double()).norm() / gr.double().norm())
+        same = bits(r, rr) and bits(uN, uNr)
+        res["check"] = {"J_gt_0": float(J) > 0, "records_and_uN_bitwise_equal": same,
+                        "dJ_dc2_rel_vs_recompute": e}
+        if not (float(J) > 0 and same and e < 2e-5):
+            raise AssertionError(f"wave gradient check failed: {res['check']}")
+        res["data_note"] = (f"receiver row {rec_distance} cells from the source (the wave "
+                            "arrives); everything else as the workload")
 
     # the pair kernel alone: one reversed segment's K/2 launches (steps
     # 2K-1 .. K, which need at least 2K recorded steps), HIP events on the
@@ -190,29 +233,14 @@ def wave_adjoint_timing(dev, steps=2000, K=50):
     torch.cuda.synchronize()
     pair_us = e0.elapsed_time(e1) * 1e3 / (K // 2)
     pair_bytes = WAVE_PAIR_BYTES * c2.numel()
-    return {"grid": [cfg.n, cfg.n], "steps": steps, "checkpoint_every": K,
-            "fwd_ms_per_step": round(t_fwd / steps * 1e3, 5),
-            "fwd_adj_ms_per_step": round(t_rev / steps * 1e3, 5),
-            "ratio": round(t_rev / t_fwd, 3),
-            "J": out["rev"][0], "max_abs_dJ_dc2": out["rev"][1],
-            "mode": "checkpoint every K steps; the backward walks each segment back from the "
-                    "checkpoint at its end, two transposed steps per launch, rebuilding the "
-                    "states (boundary-overwritten scalars recorded by the forward) instead of "
-                    "recomputing the segment; the first segment is recomputed",
-            "pair_kernel": {"kernel": "wave_pair_kernel", "avg_launch_us": round(pair_us, 2),
-                            "algorithmic_bytes_per_launch": pair_bytes,
-                            "GBs": round(pair_bytes / pair_us / 1e3, 1),
-                            "frac": round(pair_bytes / pair_us / 1e3 / HBM_PEAK_GBS, 4),
-                            "note": "algorithmic bytes / time against 8 TB/s; the launch's "
-                                    "fields (~170 MB) partly stay in the 256 MB Infinity "
-                                    "Cache"},
-            "recompute": {"fwd_adj_ms_per_step": round(t_fa / steps * 1e3, 5),
-                          "ratio": round(t_fa / t_fwd, 3)},
-            "store_all": {"fwd_adj_ms_per_step": round(t_all / steps * 1e3, 5),
-                          "ratio": round(t_all / t_fwd, 3),
-                          "history_GB": round((steps + 1) * c2.numel() * 4 / 1e9, 1)},
-            "note": "graph-replayed 50-step segments; recompute: each segment recomputed from "
-                    "its checkpoint in the backward; store_all: every state kept"}
+    res["pair_kernel"] = {"kernel": "wave_pair_kernel", "avg_launch_us": round(pair_us, 2),
+                          "algorithmic_bytes_per_launch": pair_bytes,
+                          "GBs": round(pair_bytes / pair_us / 1e3, 1),
+                          "frac": round(pair_bytes / pair_us / 1e3 / HBM_PEAK_GBS, 4),
+                          "note": "algorithmic bytes / time against 8 TB/s; the launch's "
+                                  "fields (~170 MB) partly stay in the 256 MB Infinity "
+                                  "Cache"}
+    return res
 
 
 def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
@@ -489,6 +517,22 @@ def main():
         periodic(s)
         return lap(s, 1.0)
 
+    # correctness of the timed step on this grid: the one-launch periodic
+    # Laplacian equals ghost fill + the drop-in Laplacian on every interior point
+    want = dropin_step().clone()
+    got = step()
+    inner = (slice(G, -G),) * 3
+    if not torch.equal(got[inner].view(torch.int32), want[inner].view(torch.int32)):
+        raise AssertionError("periodic Laplacian differs from fill + Laplacian")
+    del want
+    # the GPU's clocks ramp up over its first ~10 ms of work after idle: run the
+    # step for PREWARM_S seconds (untimed) before the W warm-up steps, so the
+    # K timed steps measure the steady state a long solver loop runs at
+    t_end = time.perf_counter() + PREWARM_S
+    while time.perf_counter() < t_end:
+        for _ in range(10):
+            step()
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -536,6 +580,9 @@ def main():
     if not args.quick:
         wave = wave_adjoint_timing(dev, steps=args.wave_steps)
         progress("wave forward / forward+adjoint timed")
+        wave["receivers_reached"] = wave_adjoint_timing(dev, steps=args.wave_steps,
+                                                        rec_distance=400, variants=False)
+        progress("wave gradient with receivers the front reaches: checked against recompute")
         ensemble = heat_ensemble_timing(dev)
         progress("heat ensemble timed")
         cavity = projection_cavity_timing(dev)
@@ -556,6 +603,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: s ~ U[-1,1] (torch.Generator seed 0), periodic ghosts",
+        "prewarm_s": PREWARM_S,
         "config": {"workload": "3D 7-point Laplacian, 512^3 periodic grid (514^3 padded, g=1), "
                                "step = one periodic Laplacian launch (wrapped neighbour "
                                "indices; equal bit for bit to periodic ghost fill + Laplacian)",

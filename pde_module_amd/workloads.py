@@ -91,13 +91,19 @@ class Wave2D:
 
 
 def wave_2d_config(seed=0, n=2048, steps=2000, nrec=64, corr=32.0, ghost_cells=1,
-                   perturbation=0.3):
+                   perturbation=0.3, rec_distance=None):
     """2D acoustic workload: c = 1 + perturbation * g (g a smoothed Gaussian
     random field, correlation length ``corr`` cells), dt = 0.5 dx / c_max (CFL
     0.5), Ricker source at the centre with f0 = c_min / (20 dx) (20 cells per
     wavelength) and t0 = 1.5 / f0, ``nrec`` receivers evenly spaced on the row
     next to the +Y wall, Dirichlet-0 walls.  ``perturbation=0`` gives the
-    unperturbed model (c = 1)."""
+    unperturbed model (c = 1).
+
+    At CFL 0.5 the front moves at most half a cell per step, so after 2000
+    steps on 2048^2 it is at most 1000 cells from the source while the wall
+    row is 1022 away: those receivers record exactly 0.  ``rec_distance``
+    (cells) puts the receiver row that far from the source along +Y instead,
+    where the wave arrives; everything else is unchanged."""
     rng = np.random.default_rng(seed)
     g = ghost_cells
     P = n + 2 * g
@@ -114,6 +120,10 @@ def wave_2d_config(seed=0, n=2048, steps=2000, nrec=64, corr=32.0, ghost_cells=1
     src = np.array([(g + n // 2) * P + (g + n // 2)], dtype=np.int64)
     cols = np.linspace(g + 1, g + n - 2, nrec).round().astype(np.int64)
     row = P - 2 - g                                # next to the +Y wall
+    if rec_distance is not None:
+        row = g + n // 2 + int(rec_distance)
+        if not g + n // 2 < row < P - g:
+            raise ValueError("receiver row outside the interior")
     rec = cols * P + row
     return Wave2D(n, g, dx, dt, steps, c2, src, wav, rec, f0)
 

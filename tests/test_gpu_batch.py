@@ -132,3 +132,46 @@ def test_ensemble_workload_subset_matches_per_instance(dev):
     ref.run(5)
     for a, b in zip(ens.fields(), ref.fields()):
         assert same_bits(a.cpu().numpy(), b.cpu().numpy())
+
+
+def test_full_ensemble_workload_matches_per_instance_and_oracle(dev, corc):
+    """All 512 instances of the ensemble workload (sides 64..1024 log-uniform,
+    seed 3, Dirichlet-0 walls), 20 steps through the graph-replayed batch:
+    every instance bit-identical to stepping it alone, and six instances
+    spanning the size range bit-identical to the C oracle's loop
+    (examples/2D_Heat_Equation.py:58-62: boundary, Laplacian, ForwardEuler)."""
+    from oracle import np_oracle as O
+    steps = 20
+    cfg = heat_ensemble_config(seed=3)
+    print(f"full ensemble: {len(cfg.n)} instances, {steps} steps", flush=True)
+    u0s = [cfg.u0(i, 3) for i in range(len(cfg.n))]
+    fields = [torch.from_numpy(u).to(dev) for u in u0s]
+    mk = lambda: [LegacyGridBoundary(f, float(cfg.dx[i]), 1) for i, f in enumerate(fields)]
+    dxs = [float(d) for d in cfg.dx]
+    alphas = [float(a) for a in cfg.alpha]
+    dts = [float(d) for d in cfg.dt]
+    ens = HeatEnsemble(mk(), dxs, 1, alphas, dts, device=dev, graph_chunk=10)
+    ens.load(fields)
+    ens.run(steps, use_graph=True)
+    ref = PerInstanceHeat(mk(), dxs, 1, alphas, dts)
+    ref.load(fields)
+    ref.run(steps)
+    torch.cuda.synchronize()
+    for i, (a, b) in enumerate(zip(ens.fields(), ref.fields())):
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), f"instance {i} n={cfg.n[i]}"
+    order = np.argsort(cfg.n)
+    picks = order[np.linspace(0, len(order) - 1, 6).round().astype(int)]
+    got = ens.fields()
+    for i in picks:
+        bc = LegacyGridBoundary(fields[i], dxs[i], 1)
+        u = u0s[i].copy()
+        w = O.laplacian_weights(dxs[i])
+        for _ in range(steps):
+            u2 = corc.bc_legacy(u, bc.boundary_ijk_indices, bc.boundary_interior,
+                                bc.boundary_type, bc.boundary_value, dxs[i])
+            L = corc.laplacian(u2, w, np.float32(alphas[i]), 1)
+            u = corc.forward_euler(u2, L, np.float32(dts[i]))
+        M = np.zeros(u.shape, bool)
+        M[1:-1, 1:-1] = True
+        print(f"  oracle anchor: instance {i} n={cfg.n[i]}", flush=True)
+        assert same_bits(got[i].cpu().numpy(), u, M), f"instance {i} n={cfg.n[i]}"
