@@ -140,7 +140,6 @@ class GridBoundary(ExplicitUniformGridStencil):
         self.t = torch.zeros(1, dtype=torch.float32, device=input_array.device)
         self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
 
-    @before_forward
     @staticmethod
     def _check_time_array(t: torch.Tensor, u: torch.Tensor) -> None:
         """A caller's time array is read on the device by the boundary kernels:
@@ -148,6 +147,7 @@ class GridBoundary(ExplicitUniformGridStencil):
         if t.numel() != 1 or t.dtype != torch.float32 or t.device != u.device:
             raise AssertionError("t must be a 1-element float32 array on the field's device")
 
+    @before_forward
     def set_default_params(self, input_array, t, params, **kwargs) -> None:
         for key in self.func_groups:
             params.setdefault(key, 0)

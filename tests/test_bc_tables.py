@@ -222,3 +222,13 @@ def test_caller_time_array_is_checked_before_any_launch():
             bc._time_tensor(u, bad)
     good = torch.zeros(1)
     assert bc._time_tensor(u, good) is good
+
+
+def test_boundary_hooks_are_registered():
+    """The time and params set-up of both boundary classes runs as a
+    before-forward hook (a decorator left on another method would make every
+    ramp read a stale time)."""
+    for cls in (GridBoundary, LegacyGridBoundary):
+        fn = getattr(cls, "set_default_params")
+        assert hasattr(fn, "_before_forward_order"), cls
+    assert not hasattr(GridBoundary.__dict__["_check_time_array"], "_before_forward_order")
