@@ -172,25 +172,47 @@ int pde_scatter_add_f32(float *field, const int64_t *idx, const float *vals, int
 
 /* ------------------------------------------------------------------ boundary */
 
-/* One resolved boundary write.  v = value, or min(t/tmax, 1) * value when
- * tmax > 0 (t read from device memory at launch).  kinds:
+/* One resolved boundary write.  The value v is read at launch from the
+ * time array t (device memory: t[0] is the time, t[1 + k] the k-th value a
+ * host-evaluated boundary function produced for this call), by value kind:
+ *   PDE_BCV_CONST:  v = value, or min(t[0]/tmax, 1) * value when tmax > 0
+ *                   (the reference's get_constant_func / get_ramp_func,
+ *                   pm/FDM/boundary/boundary_functions.py:9-92)
+ *   PDE_BCV_SINE:   v = value * sin(tmax * t[0])   (amplitude, angular
+ *                   frequency: the examples' sin_wave,
+ *                   examples/wave2d/Interference.py:53-62)
+ *   PDE_BCV_ARRAY:  v = t[1 + slot]   (an arbitrary boundary function,
+ *                   evaluated by the host layer on the device before the
+ *                   launch; pm/FDM/boundary/functional_boundary.py:376 and
+ *                   pm/deprecated/func_boundary.py:118)
+ * and written by kind:
  *   PDE_BC_SET:     out[dst] = v
  *   PDE_BC_COPY:    out[dst] = in[src]
  *   PDE_BC_GHOST_D: out[dst] = 2*v - in[src]            (Dirichlet ghost)
  *   PDE_BC_GHOST_N: out[dst] = coef*v + in[src]         (Neumann ghost, coef =
- *                                                       -sign(s)*2*dx) */
+ *                                                       -sign(s)*2*dx)
+ * A NULL t reads as time 0 and every array value as 0; the host passes an
+ * array of 1 + (largest slot + 1) floats whenever a table has array values. */
 #define PDE_BC_SET 0
 #define PDE_BC_COPY 1
 #define PDE_BC_GHOST_D 2
 #define PDE_BC_GHOST_N 3
 
+#define PDE_BCV_CONST 0
+#define PDE_BCV_SINE 1
+#define PDE_BCV_ARRAY 2
+
 typedef struct pde_bc_op {
     int64_t dst;
     int64_t src;
-    int32_t kind;
-    float value;
+    int16_t kind;       /* PDE_BC_* */
+    int16_t vkind;      /* PDE_BCV_* */
+    union {
+        float value;    /* PDE_BCV_CONST, PDE_BCV_SINE */
+        int32_t slot;   /* PDE_BCV_ARRAY */
+    };
     float coef;
-    float tmax;
+    float tmax;         /* PDE_BCV_CONST: ramp end time (0: none); PDE_BCV_SINE: omega */
 } pde_bc_op; /* 32 bytes */
 
 /* out = in (n floats; in and out must not alias), then every op applied with

@@ -327,10 +327,36 @@ def functional_groups(shape3, g):
 
 
 # --------------------------------------------------------------------------- boundary apply
-def bc_legacy(u, ijk, inter, btype, bvalue, dx):
+def _record_value(grp, t, dt):
+    """A group's record value at time t in the field's precision: constant,
+    ramp min(t / tmax, 1) * const (reference ``boundary_functions.py:85-89``)
+    or sine amp * sin(omega * t) (the examples' ``sin_wave``,
+    ``examples/wave2d/Interference.py:53-62``; the sine of the rounded product
+    evaluated in FP64 and rounded once)."""
+    if grp.get("sine") is not None:
+        amp, omega = grp["sine"]
+        arg = dt(omega) * dt(t)
+        return dt(amp) * dt(np.sin(np.float64(arg)))
+    v = dt(grp["const"])
+    if grp.get("tmax") is not None:
+        ramp = dt(t) / dt(grp["tmax"])
+        ramp = min(ramp, dt(1.0))
+        v = dt(ramp) * dt(grp["const"])
+    return v
+
+
+def bc_legacy(u, ijk, inter, btype, bvalue, dx, func_groups=(), t=0.0, coords=None,
+              params=None):
     """Legacy table apply (reference ``pm/deprecated/gridBoundary.py:267-302``).
     btype/bvalue are (nb, C).  Writes of one launch never collide, so applying
-    them vectorised per axis is the same as the reference's threads."""
+    them vectorised per axis is the same as the reference's threads.
+
+    Then each function group in order, one thread per entry (reference
+    ``pm/deprecated/func_boundary.py:96-134``, launched after the table at
+    ``gridBoundary.py:244-245``): dicts with ids (entry ids), type, var, name
+    and either record keys (const / tmax / sine) or ``func`` called per entry
+    as func(u, nodeID (3,), var, coords, t, dx, params[name]) -> scalar, with
+    every read from the unmodified u."""
     dt = u.dtype.type
     X, Y, Z, C = u.shape
     sn = np.array([Y * Z, Z, 1], dtype=np.int64)
@@ -349,30 +375,59 @@ def bc_legacy(u, ijk, inter, btype, bvalue, dx):
             fo[ghost[d], c] = dt(2.0) * v[d] - fi[adj[d], c]
             n = ty == 2
             fo[ghost[n], c] = k[n] * v[n] + fi[adj[n], c]
+    params = params or {}
+    for grp in func_groups:
+        var = int(grp["var"])
+        for e in np.asarray(grp["ids"], dtype=np.int64):
+            nid = ijk[e].astype(np.int64)
+            if grp.get("func") is not None:
+                val = dt(grp["func"](u, nid, var, coords, dt(t), dt(dx),
+                                     params.get(grp.get("name"), 0)))
+            else:
+                val = _record_value(grp, t, dt)
+            for a in range(3):
+                s = int(inter[e, a])
+                if s == 0:
+                    continue
+                f = int(nid @ sn)
+                ghost, adj = f - s * sn[a], f + s * sn[a]
+                if grp["type"] == 1:
+                    fo[f, var] = val
+                    fo[ghost, var] = dt(2.0) * val - fi[adj, var]
+                else:
+                    kk = (dt(-2.0) if s > 0 else dt(2.0)) * dt(dx)
+                    fo[ghost, var] = kk * val + fi[adj, var]
     return out
 
 
-def bc_functional(u, g, groups, dx, t=0.0):
+def bc_functional(u, g, groups, dx, t=0.0, coords=None, params=None):
     """Functional apply (reference ``pm/FDM/boundary/functional_boundary.py:260-400``).
 
     ``groups`` is a list of dicts with keys ids (flat node ids), type (1 Dirichlet,
-    2 Neumann), comps (selected components), const, tmax (None for constant)."""
+    2 Neumann), comps (selected components), and the value: const, tmax (None
+    for constant) or sine (amp, omega), the node vector being the node's own
+    with the selected components replaced (``boundary_functions.py:42-45``);
+    or ``func``, called per node as func(u, nodeID (3,), coords, t, dx,
+    params[name]) -> vector (C,), with every read from the unmodified u
+    (``functional_boundary.py:376``)."""
     dt = u.dtype.type
     X, Y, Z, C = u.shape
     shape3 = (X, Y, Z)
     sn = np.array([Y * Z, Z, 1], dtype=np.int64)
     out = u.copy()
     fo, fi = out.reshape(-1, C), u.reshape(-1, C)
+    params = params or {}
     for grp in groups:
         ids = np.asarray(grp["ids"], dtype=np.int64)
         comps = list(grp["comps"])
-        v = dt(grp["const"])
-        if grp.get("tmax") is not None:
-            ramp = dt(t) / dt(grp["tmax"])
-            ramp = min(ramp, dt(1.0))
-            v = dt(ramp) * dt(grp["const"])
-        val = fi[ids].copy()
-        val[:, comps] = v
+        if grp.get("func") is not None:
+            idx3 = np.stack([ids // (Y * Z), (ids // Z) % Y, ids % Z], axis=1)
+            val = np.array([np.broadcast_to(np.asarray(
+                grp["func"](u, nid, coords, dt(t), dt(dx), params.get(grp.get("name"), 0)),
+                dtype=u.dtype), (C,)) for nid in idx3], dtype=u.dtype).reshape(len(ids), C)
+        else:
+            val = fi[ids].copy()
+            val[:, comps] = _record_value(grp, t, dt)
         if grp["type"] == 1:
             fo[ids] = val
         idx = np.stack([ids // (Y * Z), (ids // Z) % Y, ids % Z], axis=1)

@@ -13,7 +13,20 @@ import torch
 
 from ...utils import get_unique_key
 from ..module.explicitUniformGridStencil import ExplicitUniformGridStencil
+from .boundary_functions import ConstantValue, as_boundary_value
 from .flags import Boundary_Types
+
+
+class FunctionBC:
+    """One function-valued group of the legacy boundary (reference
+    ``pm/deprecated/func_boundary.py:47-81``): entry ids into the boundary
+    node list, the type, the component it writes and the value."""
+
+    def __init__(self, face_ids, boundary_type, varID, func):
+        self.face_ids = np.atleast_1d(np.asarray(face_ids, dtype=np.int64)).ravel()
+        self.boundary_type = int(boundary_type)
+        self.varID = int(varID)
+        self.func = func
 
 
 class Boundary(ExplicitUniformGridStencil):
@@ -76,18 +89,54 @@ class Boundary(ExplicitUniformGridStencil):
         self.table = None
 
     def set_func_BC(self, face_ids, func, boundary_type, outputs_id) -> None:
-        raise NotImplementedError(
-            "per-node boundary functions are not supported by the GPU boundary tables; "
-            "use constant values (legacy) or get_ramp_func with the functional GridBoundary")
+        """A function-valued group (reference ``boundary.py:115-143``): applied
+        after the table, groups in the order set, one component each.  The
+        group is keyed (for ``params``) by its name, or a fresh "BC_Group"
+        name for explicit entry ids; setting a function on the same group name
+        again replaces it.  ``func`` is a Python callable
+        ``func(current_values, nodeID, varID, coordinates, t, dx, params)``
+        evaluated on the device for the whole group, or a value record
+        (get_ramp_func, get_sine_func)."""
+        if type(outputs_id) is not int:
+            raise AssertionError("a boundary function writes one component: give an int id")
+        if not 0 <= outputs_id < self.inputs[0]:
+            raise AssertionError("output id out of range")
+        if boundary_type not in (Boundary_Types.DIRICHLET, Boundary_Types.VON_NEUMANN):
+            raise ValueError(f"unknown boundary type {boundary_type}")
+        value = as_boundary_value(func)
+        if isinstance(face_ids, str):
+            if face_ids not in self.groups:
+                raise AssertionError(f"unknown group {face_ids!r}")
+            name = face_ids
+            face_ids = self.groups[face_ids]
+        else:
+            if not isinstance(face_ids, (np.ndarray, list, tuple, int)):
+                raise AssertionError("face ids must be a group name or entry ids")
+            name = get_unique_key(self.groups, "BC_Group")
+            face_ids = np.atleast_1d(np.array(face_ids, dtype=np.int64))
+            if face_ids.size and (face_ids.min() < 0 or face_ids.max() >= len(self.boundary_ids)):
+                raise IndexError("entry ids must index the boundary node list")
+        self.func_groups[name] = FunctionBC(face_ids, boundary_type, outputs_id, value)
+        self.table = None
 
     def dirichlet_BC(self, group, value, outputs_ids=None) -> None:
         if isinstance(value, (float, int)) and not isinstance(value, bool):
             self.set_BC(group, float(value), Boundary_Types.DIRICHLET, outputs_ids)
+        elif isinstance(value, ConstantValue):
+            self.set_BC(group, float(value.constant), Boundary_Types.DIRICHLET, outputs_ids)
         else:
             self.set_func_BC(group, value, Boundary_Types.DIRICHLET, outputs_ids)
 
     def vonNeumann_BC(self, group, value, outputs_ids=None) -> None:
-        self.set_BC(group, value, Boundary_Types.VON_NEUMANN, outputs_ids)
+        """Numbers go to the table as in the reference; a function or value
+        record (which the reference's legacy vonNeumann_BC refuses) becomes a
+        Neumann function group."""
+        if isinstance(value, (float, int)) and not isinstance(value, bool):
+            self.set_BC(group, float(value), Boundary_Types.VON_NEUMANN, outputs_ids)
+        elif isinstance(value, ConstantValue):
+            self.set_BC(group, float(value.constant), Boundary_Types.VON_NEUMANN, outputs_ids)
+        else:
+            self.set_func_BC(group, value, Boundary_Types.VON_NEUMANN, outputs_ids)
 
     def no_slip(self, group) -> None:
         if self.inputs[0] != self.dimension:
@@ -109,4 +158,4 @@ class Boundary(ExplicitUniformGridStencil):
         return not np.any(self.boundary_type == Boundary_Types.NO_BC)
 
 
-__all__ = ["Boundary", "get_unique_key"]
+__all__ = ["Boundary", "FunctionBC", "get_unique_key"]

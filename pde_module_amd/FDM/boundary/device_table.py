@@ -23,7 +23,11 @@ class DeviceBoundaryTable:
             ops = ch[0]
             self.chunk_start = torch.from_numpy(ch[1]).to(device)
             self.nchunks = len(ch[1]) - 1
-        self.needs_t = bool(self.nops and np.any(ops["tmax"] > 0))
+        self.needs_t = bool(self.nops and np.any((ops["tmax"] > 0)
+                                                 | (ops["vkind"] != N.BCV_CONST)))
+        # array-valued ops read t[1 + slot]: the time array passed must hold them
+        arr = ops[ops["vkind"] == N.BCV_ARRAY] if self.nops else ops[:0]
+        self.nslots = int(N.bc_slots(arr).max()) + 1 if len(arr) else 0
         raw = np.ascontiguousarray(ops).view(np.uint8) if self.nops else np.zeros(32, np.uint8)
         self.ops = torch.from_numpy(raw.copy()).to(device)
         self.staging = torch.empty(max(self.nops, 1), dtype=torch.float32, device=device)
@@ -46,9 +50,20 @@ class DeviceBoundaryTable:
                 f"boundary table touches scalar index {self.min_index}..{self.max_index} "
                 f"outside a field of {numel} values (node ids or shifted groups leave the grid)")
 
+    def _check_t(self, t: torch.Tensor | None, u: torch.Tensor) -> None:
+        if t is None:
+            if self.nslots:
+                raise AssertionError("a boundary with function values needs its value array")
+            return
+        if t.dtype != torch.float32 or t.device != u.device or not t.is_contiguous() \
+                or t.numel() < 1 + self.nslots:
+            raise AssertionError(f"the time array must be a contiguous float32 array of at "
+                                 f"least {1 + self.nslots} values on the field's device")
+
     def apply(self, x: torch.Tensor, out: torch.Tensor, t: torch.Tensor | None) -> torch.Tensor:
         """out = copy(x) with the boundary writes (reads all from x)."""
         N.check_field(x, "input_array")
+        self._check_t(t, x)
         N.check_field(out, "output_array")
         if out.shape != x.shape:
             raise AssertionError("boundary output must match the input's shape")
@@ -61,6 +76,7 @@ class DeviceBoundaryTable:
         """The same writes applied to u itself (values gathered from the
         unmodified u first)."""
         N.check_field(u, "field")
+        self._check_t(t, u)
         self._check_bounds(u.numel())
         if self.chunk_start is not None:
             N.call("pde_bc_apply_chunked_inplace_f32", N.dev_ptr(u), self.ops.data_ptr(),
@@ -79,6 +95,7 @@ class DeviceBoundaryTable:
         device int64) of ``save`` (rows of ``nops`` floats).  Chunked tables
         only; a table without ops writes nothing."""
         N.check_field(u, "field")
+        self._check_t(t, u)
         self._check_bounds(u.numel())
         if not self.nops:
             return u
@@ -99,6 +116,11 @@ class DeviceBoundaryTable:
         that applies the transpose."""
         if self._adj is not None:
             return
+        if self.nslots:
+            raise NotImplementedError(
+                "a boundary value given by a Python function may read the field, so the "
+                "boundary's transpose is not known: use constant, ramp or sine values in "
+                "differentiated rollouts")
         if not self.nops:
             self._adj = (None, 0, None, None, None, None, None, 0)
             return

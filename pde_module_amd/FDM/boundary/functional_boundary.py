@@ -22,9 +22,10 @@ import torch
 from ...stencil import before_forward, setup
 from ...utils import get_unique_key
 from ..module.explicitUniformGridStencil import ExplicitUniformGridStencil
-from .boundary_functions import as_boundary_value
+from .boundary_functions import FunctionValue, as_boundary_value
 from .device_table import DeviceBoundaryTable
 from .flags import BC_SRING, DIRICHLET, VON_NEUMANN
+from .function_values import FunctionGroup, FunctionSlots
 from .tables import functional_ops
 
 
@@ -120,12 +121,40 @@ class GridBoundary(ExplicitUniformGridStencil):
         self.vonNeumann_BC(group, 0.0)
         self.dirichlet_BC(group, 0.0, axis)
 
+    def _slot_bases(self):
+        """First value slot of each function group (FunctionValue), in order."""
+        bases, n = {}, 0
+        for key, fb in self.func_groups.items():
+            if isinstance(fb.func, FunctionValue):
+                bases[key] = n
+                n += len(fb.face_ids) * self.inputs[0]
+        return bases
+
     def resolved_ops(self) -> np.ndarray:
         """The host-resolved op table (struct pde_bc_op records)."""
-        groups = [(fb.face_ids, fb.boundary_type, fb.output_ids, fb.func)
-                  for fb in self.func_groups.values()]
+        bases = self._slot_bases()
+        groups = [(fb.face_ids, fb.boundary_type, fb.output_ids, fb.func, bases.get(key))
+                  for key, fb in self.func_groups.items()]
         return functional_ops(self.grid_shape, self.inputs[0], self.ghost_cells, groups,
                               np.float32(self.dx))
+
+    def function_groups(self) -> list[FunctionGroup]:
+        """The groups whose values a Python function gives, with their node
+        indices and slots."""
+        Y, Z = self.grid_shape[1], self.grid_shape[2]
+        out = []
+        for key, base in self._slot_bases().items():
+            fb = self.func_groups[key]
+            ids = fb.face_ids
+            ijk = np.stack([ids // (Y * Z), (ids // Z) % Y, ids % Z], axis=1)
+            out.append(FunctionGroup(key, fb.func.func, ijk, base, self.inputs[0], None))
+        return out
+
+    def _build_table(self, device) -> None:
+        self.table = DeviceBoundaryTable(self.resolved_ops(), device)
+        fg = self.function_groups()
+        self._fslots = FunctionSlots(fg, device, np.float32(self.dx), self.grid_coordinates) \
+            if fg else None
 
     def __call__(self, input_array: torch.Tensor, t=None, params: dict[str, Any] | None = None):
         if params is None:
@@ -137,8 +166,9 @@ class GridBoundary(ExplicitUniformGridStencil):
         self._check_input(input_array)
         if tuple(input_array.shape[:3]) != self.grid_shape:
             raise AssertionError("field shape differs from the one the boundary was built for")
-        self.t = torch.zeros(1, dtype=torch.float32, device=input_array.device)
-        self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
+        self._t_own = torch.zeros(1, dtype=torch.float32, device=input_array.device)
+        self.t = self._t_own
+        self._build_table(input_array.device)
 
     @staticmethod
     def _check_time_array(t: torch.Tensor, u: torch.Tensor) -> None:
@@ -151,39 +181,56 @@ class GridBoundary(ExplicitUniformGridStencil):
     def set_default_params(self, input_array, t, params, **kwargs) -> None:
         for key in self.func_groups:
             params.setdefault(key, 0)
+        # the params of the last call also feed the fused steps' evaluations
+        self._params = params
         if self.table is None:  # groups changed after setup
-            self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
-        if t is not None:
-            if isinstance(t, torch.Tensor):
-                self._check_time_array(t, input_array)
-                self.t = t
-            else:
-                self.t.fill_(float(np.float32(t)))
+            self._build_table(input_array.device)
+        self._set_time(input_array, t)
+
+    def _set_time(self, u: torch.Tensor, t) -> None:
+        """t: None keeps the current time array; a tensor (the caller's
+        1-element float32 device array) becomes it; a number is written into
+        the boundary's own array (never into a caller's)."""
+        own = getattr(self, "_t_own", None)
+        if own is None or own.device != u.device:
+            own = self._t_own = torch.zeros(1, dtype=torch.float32, device=u.device)
+            if not hasattr(self, "t") or self.t.device != u.device:
+                self.t = own
+        if isinstance(t, torch.Tensor):
+            self._check_time_array(t, u)
+            self.t = t
+        elif t is not None:
+            own.fill_(float(np.float32(t)))
+            self.t = own
+
+    def _values(self, u: torch.Tensor) -> torch.Tensor:
+        """The array the kernels read: the time, then (when some group's value
+        is a Python function) every function group's values on u."""
+        fs = getattr(self, "_fslots", None)
+        if fs is None:
+            return self.t
+        return fs.evaluate(u, self.t, getattr(self, "_params", None))
 
     def forward(self, input_array, t=None, params=None, **kwargs):
-        return self.table.apply(input_array, self.output_array, self.t)
+        return self.table.apply(input_array, self.output_array, self._values(input_array))
 
     def _ensure_table(self, u: torch.Tensor) -> None:
         self._check_input(u)
         if self.table is None or self.table.device != u.device:
-            self.table = DeviceBoundaryTable(self.resolved_ops(), u.device)
+            self._build_table(u.device)
 
     def _time_tensor(self, u: torch.Tensor, t=None):
-        """The 1-element device time array the ramp values read."""
-        if isinstance(t, torch.Tensor):
-            self._check_time_array(t, u)
-            return t
-        if not hasattr(self, "t") or self.t.device != u.device:
-            self.t = torch.zeros(1, dtype=torch.float32, device=u.device)
-        if t is not None:
-            self.t.fill_(float(np.float32(t)))
-        return self.t
+        """The device array the boundary values read (time, then function
+        values evaluated on u, the field before the boundary)."""
+        self._set_time(u, t)
+        self._ensure_table(u)
+        return self._values(u)
 
     def apply_inplace(self, u: torch.Tensor, t=None) -> torch.Tensor:
         """Apply the boundary to ``u`` itself (no copy): the form the fused time
         steps use.  Same values as ``__call__`` on every written scalar."""
         self._check_input(u)
-        if not hasattr(self, "t"):
-            self.t = torch.zeros(1, dtype=torch.float32, device=u.device)
-        self.set_default_params(u, t, {})
-        return self.table.apply_inplace(u, self.t)
+        self.set_default_params(u, t, getattr(self, "_params", None) or {})
+        if self.table.device != u.device:
+            self._build_table(u.device)
+        return self.table.apply_inplace(u, self._values(u))

@@ -15,7 +15,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from ..._native import BC_COPY, BC_GHOST_D, BC_GHOST_N, BC_OP_DTYPE, BC_SET
+from ..._native import (BC_COPY, BC_GHOST_D, BC_GHOST_N, BC_OP_DTYPE, BC_SET, BCV_ARRAY,
+                         BCV_CONST, bc_slot_bits)
 from .flags import DIRICHLET, VON_NEUMANN
 
 
@@ -44,7 +45,8 @@ class _Writes:
     def next_launch(self):
         self.launch += 1
 
-    def add(self, dst, src, kind, value, coef, tmax, key):
+    def add(self, dst, src, kind, value, coef, tmax, key, vkind=BCV_CONST):
+        """value: FP32 values, or for vkind == BCV_ARRAY the slots (ints)."""
         n = len(dst)
         if n == 0:
             return
@@ -52,7 +54,11 @@ class _Writes:
         rec["dst"] = dst
         rec["src"] = src
         rec["kind"] = kind
-        rec["value"] = value
+        rec["vkind"] = vkind
+        if vkind == BCV_ARRAY:
+            rec["value"] = bc_slot_bits(np.broadcast_to(np.asarray(value, dtype=np.int64), (n,)))
+        else:
+            rec["value"] = value
         rec["coef"] = coef
         rec["tmax"] = tmax
         self.parts.append(rec)
@@ -75,30 +81,38 @@ class _Writes:
 
 
 def functional_ops(grid_shape, ncomp, ghost_cells, groups, dx32):
-    """``groups``: ordered iterable of (ids, boundary_type, output_ids, value)
-    with value a ConstantValue/RampValue."""
+    """``groups``: ordered iterable of (ids, boundary_type, output_ids, value,
+    slot_base) with value a ConstantValue/RampValue/SineValue, or a
+    FunctionValue whose vector for the group's node at position p is in slots
+    slot_base + p*ncomp + c (``function_values.FunctionSlots``)."""
     g = ghost_cells
     sn = _strides(grid_shape)
     Y, Z = grid_shape[1], grid_shape[2]
     elig = [a for a in range(3) if grid_shape[a] > 1]
     W = _Writes()
-    for ids, btype, output_ids, value in groups:
+    for ids, btype, output_ids, value, slot_base in groups:
         W.next_launch()
         ids = np.asarray(ids, dtype=np.int64).ravel()
         pos = np.arange(len(ids), dtype=np.int64)
         comps = [int(c) for c in (output_ids if isinstance(output_ids, (list, tuple)) else [output_ids])]
-        v32, t32 = value.value32, value.tmax32
+        vk = value.vkind
+        t32 = value.tmax32
+        # the value of component c for the nodes at positions p
+        val = (lambda p, c: slot_base + p * ncomp + c) if vk == BCV_ARRAY else \
+            (lambda p, c: value.value32)
         if btype not in (DIRICHLET, VON_NEUMANN):
             raise ValueError(f"unknown boundary type {btype}")
         # per node: the node write (phase c), then for each axis a and each
         # selected component m the ghost write (phase ncomp + a*len(comps) + m)
         nph = ncomp + 3 * max(len(comps), 1)
         if btype == DIRICHLET:
-            # out[node] = val: the whole vector, selected components replaced.
+            # out[node] = val: the whole vector.  A record's vector is the
+            # node's own with the selected components replaced
+            # (boundary_functions.py:42-45); a function's is what it returns.
             for c in range(ncomp):
                 dst = ids * ncomp + c
-                if c in comps:
-                    W.add(dst, -1, BC_SET, v32, 0.0, t32, pos * nph + c)
+                if c in comps or vk == BCV_ARRAY:
+                    W.add(dst, -1, BC_SET, val(pos, c), 0.0, t32, pos * nph + c, vk)
                 else:
                     W.add(dst, dst, BC_COPY, 0.0, 0.0, 0.0, pos * nph + c)
         idx = np.stack([ids // (Y * Z), (ids // Z) % Y, ids % Z])
@@ -110,15 +124,22 @@ def functional_ops(grid_shape, ncomp, ghost_cells, groups, dx32):
             for m, c in enumerate(comps):
                 key = pq * nph + ncomp + a * len(comps) + m
                 if btype == DIRICHLET:
-                    W.add(ghost * ncomp + c, adj * ncomp + c, BC_GHOST_D, v32, 0.0, t32, key)
+                    W.add(ghost * ncomp + c, adj * ncomp + c, BC_GHOST_D, val(pq, c), 0.0, t32,
+                          key, vk)
                 else:
-                    W.add(ghost * ncomp + c, adj * ncomp + c, BC_GHOST_N, v32,
-                          _neumann_coef(sq, dx32), t32, key)
+                    W.add(ghost * ncomp + c, adj * ncomp + c, BC_GHOST_N, val(pq, c),
+                          _neumann_coef(sq, dx32), t32, key, vk)
     return W.resolve()
 
 
-def legacy_ops(grid_shape, ncomp, ijk, interior, btype, bvalue, dx32):
-    """Legacy table: ijk/interior (nb, 3), btype/bvalue (nb, C)."""
+def legacy_ops(grid_shape, ncomp, ijk, interior, btype, bvalue, dx32, func_groups=()):
+    """Legacy table: ijk/interior (nb, 3), btype/bvalue (nb, C); then the
+    function groups, each one launch after the table's in order
+    (reference ``pm/deprecated/gridBoundary.py:244-245``, kernel
+    ``pm/deprecated/func_boundary.py:96-134``): ``func_groups`` holds
+    (entry ids, boundary_type, varID, value, slot_base) with value a
+    RampValue/SineValue/ConstantValue record or a FunctionValue whose value for
+    the entry at position p is in slot slot_base + p."""
     sn = _strides(grid_shape)
     node = np.asarray(ijk, dtype=np.int64) @ sn
     interior = np.asarray(interior)
@@ -139,6 +160,30 @@ def legacy_ops(grid_shape, ncomp, ijk, interior, btype, bvalue, dx32):
             n = ty == VON_NEUMANN
             W.add(ghost[n] * ncomp + c, adj[n] * ncomp + c, BC_GHOST_N, v[n], coef[n], 0.0,
                   key[n] + 1)
+    for ids, fbtype, var, value, slot_base in func_groups:
+        # one thread per entry: for each axis with an interior direction,
+        # Dirichlet writes the node and its ghost, Neumann the ghost
+        W.next_launch()
+        ids = np.asarray(ids, dtype=np.int64).ravel()
+        pos = np.arange(len(ids), dtype=np.int64)
+        var = int(var)
+        vk, t32 = value.vkind, value.tmax32
+        for a in range(3):
+            sel = interior[ids, a] != 0
+            b, p = ids[sel], pos[sel]
+            s = interior[b, a].astype(np.int64)
+            ghost, adj = node[b] - s * sn[a], node[b] + s * sn[a]
+            vals = slot_base + p if vk == BCV_ARRAY else value.value32
+            key = (p * 3 + a) * 2
+            if fbtype == DIRICHLET:
+                W.add(node[b] * ncomp + var, -1, BC_SET, vals, 0.0, t32, key, vk)
+                W.add(ghost * ncomp + var, adj * ncomp + var, BC_GHOST_D, vals, 0.0, t32,
+                      key + 1, vk)
+            elif fbtype == VON_NEUMANN:
+                W.add(ghost * ncomp + var, adj * ncomp + var, BC_GHOST_N, vals,
+                      _neumann_coef(s, dx32), t32, key + 1, vk)
+            else:
+                raise ValueError(f"unknown boundary type {fbtype}")
     return W.resolve()
 
 

@@ -97,11 +97,24 @@ SIGNATURES = {
     "pde_status_string": [C.c_int],
 }
 
-# Must match struct pde_bc_op in include/pde_hip.h (32 bytes).
-BC_OP_DTYPE = np.dtype([("dst", "<i8"), ("src", "<i8"), ("kind", "<i4"),
+# Must match struct pde_bc_op in include/pde_hip.h (32 bytes).  "value"
+# holds the int32 bits of the slot for array-valued ops (the header's union);
+# set_bc_slots / bc_slots convert.
+BC_OP_DTYPE = np.dtype([("dst", "<i8"), ("src", "<i8"), ("kind", "<i2"), ("vkind", "<i2"),
                         ("value", "<f4"), ("coef", "<f4"), ("tmax", "<f4")])
 BC_SET, BC_COPY, BC_GHOST_D, BC_GHOST_N = 0, 1, 2, 3
+BCV_CONST, BCV_SINE, BCV_ARRAY = 0, 1, 2
 assert BC_OP_DTYPE.itemsize == 32
+
+
+def bc_slot_bits(slots) -> np.ndarray:
+    """Slots of array-valued ops as the float32 bit patterns stored in "value"."""
+    return np.ascontiguousarray(slots, dtype=np.int32).view(np.float32)
+
+
+def bc_slots(ops) -> np.ndarray:
+    """The slots of array-valued ops (value field read as int32 bits)."""
+    return np.ascontiguousarray(ops["value"]).view(np.int32)
 
 # struct pde_batch_grid
 BATCH_GRID_DTYPE = np.dtype([("offset", "<i8"), ("op_begin", "<i8"), ("op_end", "<i8"),

@@ -162,6 +162,10 @@ class HeatEnsemble:
         grid = np.zeros(self.n, N.BATCH_GRID_DTYPE)
         for i, bc in enumerate(boundaries):
             ops = np.array(bc.resolved_ops(), dtype=N.BC_OP_DTYPE, copy=True)
+            if np.any(ops["vkind"] == N.BCV_ARRAY):
+                raise ValueError(f"instance {i}: boundary values given by Python functions are "
+                                 "not supported in a batched ensemble (use constant, ramp or "
+                                 "sine values)")
             ch = chunk_ops(ops) if len(ops) else (ops, np.zeros(1, np.int64))
             if ch is None:
                 chunked = False
@@ -183,7 +187,7 @@ class HeatEnsemble:
             begin += len(ops)
         self.host_grid = grid
         allops = np.concatenate(tables) if begin else np.zeros(1, N.BC_OP_DTYPE)
-        self.needs_t = bool(np.any(allops["tmax"] > 0))
+        self.needs_t = bool(np.any((allops["tmax"] > 0) | (allops["vkind"] != N.BCV_CONST)))
         dev = self.device
         self.grid_dev = torch.from_numpy(grid.view(np.uint8).copy()).to(dev)
         self.ops_dev = torch.from_numpy(allops.view(np.uint8).copy()).to(dev)

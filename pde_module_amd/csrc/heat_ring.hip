@@ -83,17 +83,6 @@ __device__ __forceinline__ void heat_interior(const HeatRArgs &A, const HeatRGeo
     }
 }
 
-// op_value with the time value already loaded (t ? t[0] : 0): the same arithmetic
-__device__ __forceinline__ float op_value_tv(const pde_bc_op &op, float tv)
-{
-    if (op.tmax > 0.0f) {
-        float ramp = tv / op.tmax;
-        ramp = ramp < 1.0f ? ramp : 1.0f;
-        return ramp * op.value;
-    }
-    return op.value;
-}
-
 __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &pg, int pid,
                                           float *lds)
 {
@@ -135,7 +124,7 @@ __device__ __forceinline__ void heat_ring(const HeatRArgs &A, const HeatRGeom &p
     auto eval = [&](const pde_ring_entry &w, const pde_bc_op &op) {
         const int l = lidx(w.node);
         if (l < 0) return;
-        const float v = op_value_tv(op, tv);
+        const float v = op_value_tv(op, tv, A.t);
         const int ls = op.kind == PDE_BC_SET ? 0 : lidx(op.src);
         const float in = ls >= 0 ? U[ls] : 0.0f;
         float r;

@@ -75,18 +75,25 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblocks)
 
 // Value of one resolved boundary op (PDE_BC_* kinds) read from `in`; t is the
 // device time scalar of ramp values (null = 0).
-__device__ __forceinline__ float op_value(const pde_bc_op &op, const float *t)
+// The op's boundary value with the time tv = t[0] already loaded (0 for a
+// null t, the reference's default time array zeros(1),
+// functional_boundary.py:237).
+__device__ __forceinline__ float op_value_tv(const pde_bc_op &op, float tv, const float *t)
 {
+    if (op.vkind == PDE_BCV_SINE) return op.value * sinf(op.tmax * tv);
+    if (op.vkind == PDE_BCV_ARRAY) return t ? t[1 + op.slot] : 0.0f;
     if (op.tmax > 0.0f) {
-        // wp.min(t / T_max, 1) * const  (boundary_functions.py:85-89).  A null
-        // t is the reference's default time array, zeros(1)
-        // (functional_boundary.py:237).
-        const float tv = t ? t[0] : 0.0f;
+        // wp.min(t / T_max, 1) * const  (boundary_functions.py:85-89)
         float ramp = tv / op.tmax;
         ramp = ramp < 1.0f ? ramp : 1.0f;
         return ramp * op.value;
     }
     return op.value;
+}
+
+__device__ __forceinline__ float op_value(const pde_bc_op &op, const float *t)
+{
+    return op_value_tv(op, t ? t[0] : 0.0f, t);
 }
 
 __device__ __forceinline__ float op_eval(const pde_bc_op &op, const float *__restrict__ in,

@@ -16,7 +16,9 @@ import numpy as np
 import torch
 
 from ..FDM.boundary.boundary import Boundary
+from ..FDM.boundary.boundary_functions import FunctionValue
 from ..FDM.boundary.device_table import DeviceBoundaryTable
+from ..FDM.boundary.function_values import FunctionGroup, FunctionSlots
 from ..FDM.boundary.tables import legacy_ops
 from ..stencil import before_forward, setup
 
@@ -77,10 +79,35 @@ class GridBoundary(Boundary):
         self.boundary_ijk_indices[self.groups[group], axis] += shift
         self.table = None
 
+    def _slot_bases(self):
+        bases, n = {}, 0
+        for key, fb in self.func_groups.items():
+            if isinstance(fb.func, FunctionValue):
+                bases[key] = n
+                n += len(fb.face_ids)
+        return bases
+
     def resolved_ops(self) -> np.ndarray:
+        bases = self._slot_bases()
+        fgs = [(fb.face_ids, fb.boundary_type, fb.varID, fb.func, bases.get(key))
+               for key, fb in self.func_groups.items()]
         return legacy_ops(self.grid_shape, self.inputs[0], self.boundary_ijk_indices,
                           self.boundary_interior, self.boundary_type, self.boundary_value,
-                          np.float32(self.dx))
+                          np.float32(self.dx), fgs)
+
+    def function_groups(self) -> list[FunctionGroup]:
+        """Function groups given by Python callables: their nodes (the boundary
+        node list's, shifted groups included) and slots."""
+        return [FunctionGroup(key, self.func_groups[key].func.func,
+                              self.boundary_ijk_indices[self.func_groups[key].face_ids], base, 1,
+                              self.func_groups[key].varID)
+                for key, base in self._slot_bases().items()]
+
+    def _build_table(self, device) -> None:
+        self.table = DeviceBoundaryTable(self.resolved_ops(), device)
+        fg = self.function_groups()
+        self._fslots = FunctionSlots(fg, device, np.float32(self.dx), self.grid_coordinates) \
+            if fg else None
 
     def __call__(self, input_array: torch.Tensor, t: float = 0.0, params=None):
         if params is None:
@@ -92,28 +119,62 @@ class GridBoundary(Boundary):
         self._check_input(input_array)
         if tuple(input_array.shape[:3]) != self.grid_shape:
             raise AssertionError("field shape differs from the one the boundary was built for")
-        self.t = torch.zeros(1, dtype=torch.float32, device=input_array.device)
-        self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
+        self._set_time(input_array, None)
+        self._build_table(input_array.device)
 
     @before_forward
     def set_default_params(self, input_array, t, params, **kwargs) -> None:
+        """Reference ``gridBoundary.py:182-191``: default params per function
+        group, then the time (a number, or a 1-element float32 device array)."""
+        for key in self.func_groups:
+            params.setdefault(key, 0)
+        self._params = params
         if self.table is None:
-            self.table = DeviceBoundaryTable(self.resolved_ops(), input_array.device)
+            self._build_table(input_array.device)
+        self._set_time(input_array, t)
+
+    def _set_time(self, u, t) -> None:
+        """t: None keeps the current time array; a tensor (the caller's
+        1-element float32 device array) becomes it; a number is written into
+        the boundary's own array (never into a caller's)."""
+        own = getattr(self, "_t_own", None)
+        if own is None or own.device != u.device:
+            own = self._t_own = torch.zeros(1, dtype=torch.float32, device=u.device)
+            if not hasattr(self, "t") or self.t.device != u.device:
+                self.t = own
+        if isinstance(t, torch.Tensor):
+            if t.numel() != 1 or t.dtype != torch.float32 or t.device != u.device:
+                raise AssertionError("t must be a 1-element float32 array on the field's device")
+            self.t = t
+        elif t is not None:
+            own.fill_(float(np.float32(t)))
+            self.t = own
+
+    def _values(self, u: torch.Tensor):
+        """None for a table of constants (the kernels need no time), else the
+        time array, followed by the function groups' values on u."""
+        if self._fslots is not None:
+            return self._fslots.evaluate(u, self.t, getattr(self, "_params", None))
+        return self.t if self.table.needs_t else None
 
     def forward(self, input_array, t=0.0, params=None, **kwargs):
-        return self.table.apply(input_array, self.output_array, None)
+        return self.table.apply(input_array, self.output_array, self._values(input_array))
 
     def _ensure_table(self, u: torch.Tensor) -> None:
         self._check_input(u)
         if self.table is None or self.table.device != u.device:
-            self.table = DeviceBoundaryTable(self.resolved_ops(), u.device)
+            self._build_table(u.device)
 
     def _time_tensor(self, u: torch.Tensor, t=None):
-        return None  # legacy tables hold constants only
+        """The device array the boundary values read (None when every value is
+        a constant)."""
+        self._ensure_table(u)
+        self._set_time(u, t)
+        return self._values(u)
 
-    def apply_inplace(self, u: torch.Tensor, t=0.0) -> torch.Tensor:
-        """Apply the boundary to u itself (no copy), as the fused steps do."""
+    def apply_inplace(self, u: torch.Tensor, t=None) -> torch.Tensor:
+        """Apply the boundary to u itself (no copy), as the fused steps do
+        (t: None keeps the time of the last call)."""
         self._check_input(u)
-        if self.table is None:
-            self.table = DeviceBoundaryTable(self.resolved_ops(), u.device)
-        return self.table.apply_inplace(u, None)
+        tt = self._time_tensor(u, t)          # builds the table when groups changed
+        return self.table.apply_inplace(u, tt)

@@ -1,18 +1,23 @@
 """Test-side helpers (not part of the package)."""
 import numpy as np
 
-from pde_module_amd._native import BC_COPY, BC_GHOST_D, BC_SET
+from pde_module_amd._native import BC_COPY, BC_GHOST_D, BC_SET, BCV_ARRAY, BCV_SINE, bc_slots
 
 
-def apply_ops_host(u, ops, t=0.0):
+def apply_ops_host(u, ops, t=0.0, values=None):
     """Interpret a resolved boundary op table on the host in FP32, to check the
-    table resolution against the oracle without a GPU."""
+    table resolution against the oracle without a GPU.  ``values``: the time
+    array's function values (t[1:]) for array-valued ops."""
     fin = np.ascontiguousarray(u, dtype=np.float32).ravel()
     out = fin.copy()
     t = np.float32(t)
     for op in ops:
         v = op["value"]
-        if op["tmax"] > 0:
+        if op["vkind"] == BCV_SINE:
+            v = op["value"] * np.float32(np.sin(np.float64(op["tmax"] * t)))
+        elif op["vkind"] == BCV_ARRAY:
+            v = np.float32(values[int(bc_slots(np.array([op]))[0])])
+        elif op["tmax"] > 0:
             v = min(t / op["tmax"], np.float32(1.0)) * op["value"]
         k = op["kind"]
         if k == BC_SET:

@@ -60,12 +60,21 @@ def test_version_and_status_strings():
 def test_bc_op_record_layout():
     text = open(os.path.join(ROOT, "include", "pde_hip.h")).read()
     body = re.search(r"typedef struct pde_bc_op \{(.*?)\} pde_bc_op;", text, flags=re.S).group(1)
-    fields = re.findall(r"(int64_t|int32_t|float)\s+(\w+);", body)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    union = re.search(r"union \{(.*?)\};", body, flags=re.S).group(1)
+    members = re.findall(r"(int32_t|float)\s+(\w+);", union)
+    assert [f for _, f in members] == ["value", "slot"]      # 4 bytes shared
+    flat = re.sub(r"union \{.*?\};", "float value;", body, flags=re.S)
+    fields = re.findall(r"(int64_t|int32_t|int16_t|float)\s+(\w+);", flat)
     assert [f for _, f in fields] == list(N.BC_OP_DTYPE.names)
-    sizes = {"int64_t": 8, "int32_t": 4, "float": 4}
+    sizes = {"int64_t": 8, "int32_t": 4, "int16_t": 2, "float": 4}
     assert sum(sizes[t] for t, _ in fields) == N.BC_OP_DTYPE.itemsize == 32
-    kinds = dict(re.findall(r"#define (PDE_BC_\w+) (\d+)", text))
+    kinds = dict(re.findall(r"#define (PDE_BCV?_\w+) (\d+)", text))
     assert int(kinds["PDE_BC_SET"]) == N.BC_SET and int(kinds["PDE_BC_GHOST_N"]) == N.BC_GHOST_N
+    assert (int(kinds["PDE_BCV_CONST"]), int(kinds["PDE_BCV_SINE"]),
+            int(kinds["PDE_BCV_ARRAY"])) == (N.BCV_CONST, N.BCV_SINE, N.BCV_ARRAY)
+    assert np.array_equal(N.bc_slots(np.array([(0, 0, 0, 2, N.bc_slot_bits([7])[0], 0, 0)],
+                                              dtype=N.BC_OP_DTYPE)), [7])
 
 
 def test_argument_errors_need_no_gpu():

@@ -108,14 +108,18 @@ def test_bad_arguments_raise():
     with pytest.raises(AssertionError):
         bc.dirichlet_BC("+W", 0.0)
     with pytest.raises(TypeError):
-        bc.dirichlet_BC("ALL", lambda *a: 0.0)
+        bc.dirichlet_BC("ALL", "not a value")
+    bc.dirichlet_BC("ALL", lambda *a: 0.0)          # a boundary function: accepted
     with pytest.raises(AssertionError):
         bc.dirichlet_BC("ALL", 0.0, 5)
     with pytest.raises(AssertionError):
         GridBoundary(field((6, 6, 1), 3), 0.1, 1).impermeable("-X")
     leg = LegacyGridBoundary(field((6, 6, 1), 1), 0.1, 1)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(AssertionError):             # one component per function group
         leg.dirichlet_BC("ALL", lambda *a: 0.0)
+    leg.dirichlet_BC("ALL", lambda *a: 0.0, 0)
+    with pytest.raises(AssertionError):
+        leg.dirichlet_BC("ALL", lambda *a: 0.0, 1)
     with pytest.raises(AssertionError):
         LegacyGridBoundary(field((6, 6, 1), 1), 0.1, 0)
 
