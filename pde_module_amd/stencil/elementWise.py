@@ -7,7 +7,7 @@ from typing import Optional
 import torch
 
 from .. import _native as N
-from .element_ops import ELEMENT_OPS, ElementOp, _elem
+from .element_ops import ELEMENT_OPS, ElementOp, FunctionElementOp, _elem
 from .hooks import setup
 from .stencil import Stencil
 
@@ -15,20 +15,26 @@ from .stencil import Stencil
 def resolve_element_op(element_op) -> ElementOp:
     if isinstance(element_op, ElementOp):
         return element_op
-    if isinstance(element_op, str) and element_op in ELEMENT_OPS:
-        return ELEMENT_OPS[element_op]
-    raise TypeError(
-        f"unsupported element op {element_op!r}: the element operations are compiled into "
-        f"libpdehip.so ({', '.join(sorted(ELEMENT_OPS))}); arbitrary functions cannot run")
+    if isinstance(element_op, str):
+        if element_op in ELEMENT_OPS:
+            return ELEMENT_OPS[element_op]
+        raise TypeError(f"unknown element op {element_op!r}: the compiled ones are "
+                        f"{', '.join(sorted(ELEMENT_OPS))}; any other is given as a function")
+    if callable(element_op):
+        return FunctionElementOp(element_op)
+    raise TypeError(f"an element op is a compiled op's name, an ElementOp or a function, not "
+                    f"{type(element_op).__name__}")
 
 
 class ElementWise(Stencil):
     """``ElementWise(element_op, output_dtype=None)``; ``module(A, B)`` returns
     the fixed output holding ``element_op(A[n], B[n])`` at every padded node.
 
-    ``element_op`` is one of ``pde_module_amd.stencil.element_ops``' ops (or
-    its name); ``output_dtype`` is the output's element shape (default: the one
-    the op produces)."""
+    ``element_op`` is one of ``pde_module_amd.stencil.element_ops``' compiled
+    ops (or its name), or any Python function over whole fields, element axes
+    last, run as torch operations on the device; ``output_dtype`` is the
+    output's element shape (default: the one the op produces; A's for a
+    function, as the reference defaults to A's dtype)."""
 
     def __init__(self, element_op, output_dtype: Optional[tuple] = None) -> None:
         self.element_op = resolve_element_op(element_op)
@@ -44,6 +50,8 @@ class ElementWise(Stencil):
         produced = self.element_op.out_shape(self.array_A_dtype, self.array_B_dtype)
         if self.output_dtype is None:
             self.output_dtype = produced
+        elif isinstance(self.element_op, FunctionElementOp):
+            pass                                  # the function decides its shape
         elif self.output_dtype != produced:
             raise AssertionError(f"output dtype {self.output_dtype} but {self.element_op.name} "
                                  f"produces {produced}")

@@ -176,6 +176,48 @@ def component_shifted_scale(comp: int) -> MapOp:
 MAP_OPS = {op.name: op for op in (shifted_scale, divide_by_scalar)}
 
 
+def _copy_result(r, out: torch.Tensor, name: str) -> None:
+    r = torch.as_tensor(r, dtype=out.dtype, device=out.device)
+    try:
+        out.copy_(r.broadcast_to(out.shape))
+    except RuntimeError:
+        raise AssertionError(f"{name} returned shape {tuple(r.shape)}; the output is "
+                             f"{tuple(out.shape)}")
+
+
+class FunctionMap(MapOp):
+    """A user map written over whole fields: ``func(A, *args)`` with the
+    element axes last (so ``rho[..., 0:1]`` is the reference's ``rho[0]``),
+    evaluated as torch operations on the field's device and stream into the
+    module's fixed output.  The reference compiles the element function into a
+    Warp kernel (``pm/stencil/mapWise.py:28-30``)."""
+
+    def __init__(self, func):
+        self.func = func
+        self.name = getattr(func, "__name__", "function")
+
+    def launch(self, a, out, *args):
+        _copy_result(self.func(a, *args), out, f"map {self.name}")
+
+
+class FunctionElementOp(ElementOp):
+    """A user element operation written over whole fields: ``func(A, B)``
+    with the element axes last, evaluated as torch operations on the device
+    into the module's fixed output (reference ``pm/stencil/elementWise.py:61-81``
+    builds a Warp kernel calling it per node).  The output's element shape is
+    A's unless the module is given ``output_dtype``."""
+
+    def __init__(self, func):
+        self.func = func
+        self.name = getattr(func, "__name__", "function")
+
+    def out_shape(self, ea, eb):
+        return ea
+
+    def launch(self, a, b, out):
+        _copy_result(self.func(a, b), out, f"element op {self.name}")
+
+
 def linear_combination(out: torch.Tensor, *terms) -> torch.Tensor:
     """``out = c0*x0 + c1*x1 + ...`` for up to four ``(c, x)`` terms in one
     pass, accumulated left to right; with coefficients +-1 it equals the Warp

@@ -4,7 +4,7 @@ from __future__ import annotations
 import torch
 
 from .. import _native as N
-from .element_ops import MAP_OPS, MapOp
+from .element_ops import MAP_OPS, FunctionMap, MapOp
 from .hooks import setup
 from .stencil import Stencil
 
@@ -12,17 +12,25 @@ from .stencil import Stencil
 def resolve_map_op(element_op) -> MapOp:
     if isinstance(element_op, MapOp):
         return element_op
-    if isinstance(element_op, str) and element_op in MAP_OPS:
-        return MAP_OPS[element_op]
-    raise TypeError(
-        f"unsupported map {element_op!r}: the maps are compiled into libpdehip.so "
-        f"({', '.join(sorted(MAP_OPS))}); arbitrary functions cannot run")
+    if isinstance(element_op, str):
+        if element_op in MAP_OPS:
+            return MAP_OPS[element_op]
+        raise TypeError(f"unknown map {element_op!r}: the compiled maps are "
+                        f"{', '.join(sorted(MAP_OPS))}; any other map is given as a function")
+    if callable(element_op):
+        return FunctionMap(element_op)
+    raise TypeError(f"a map is a compiled map's name, a MapOp or a function, not "
+                    f"{type(element_op).__name__}")
 
 
 class MapWise(Stencil):
     """``MapWise(element_op)``; ``module(A, *args)`` returns the fixed output
     (same shape as A) holding ``element_op(A[n], *args)``.  The extra arguments
-    are the map's own (scalars, or a field for ``divide_by_scalar``)."""
+    are the map's own (scalars, or a field for ``divide_by_scalar``).
+
+    ``element_op`` is a compiled map (``shifted_scale``, ``divide_by_scalar``:
+    one HIP kernel each) or any Python function over whole fields, element
+    axes last, run as torch operations on the device (``FunctionMap``)."""
 
     def __init__(self, element_op) -> None:
         self.element_op = resolve_map_op(element_op)

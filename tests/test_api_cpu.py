@@ -93,13 +93,22 @@ def test_grid_shape_may_carry_element_dims():
         Divergence(3, (23, 23, 1, 3), 0.05, 1)
 
 
-def test_element_ops_reject_arbitrary_functions():
+def test_element_ops_take_functions_and_refuse_unknown_names():
+    """Compiled ops by name; any other function runs as device torch
+    operations; unknown names and non-callables raise; CPU fields are refused
+    (no host fallback) for functions too."""
     from pde_module_amd.stencil import ElementWise, MapWise, element_ops
-    with pytest.raises(TypeError):
-        ElementWise(lambda a, b: a + b)
+    fe = ElementWise(lambda a, b: a + b)
+    assert isinstance(fe.element_op, element_ops.FunctionElementOp)
+    assert isinstance(MapWise(lambda m, rho: m / rho[..., 0:1]).element_op,
+                      element_ops.FunctionMap)
     with pytest.raises(TypeError):
         MapWise("not_a_map")
+    with pytest.raises(TypeError):
+        ElementWise(3.0)
     assert ElementWise("outer").element_op is element_ops.outer
     x = torch.zeros(4, 4, 1, 2)
     with pytest.raises(RuntimeError):
         ElementWise("add")(x, x)
+    with pytest.raises(RuntimeError):
+        fe(x, x)

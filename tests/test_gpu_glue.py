@@ -144,3 +144,45 @@ def test_acm_cavity_step_matches_oracle(dev, corc):
     assert same_bits(u_conv.cpu().numpy(), o_conv, mu)
     assert same_bits(u_next.cpu().numpy(), o_un, mu)
     assert same_bits(p_next.cpu().numpy(), o_pn, mp)
+
+
+def test_function_maps_and_element_ops(dev):
+    """User functions in MapWise / ElementWise (the cylinder example's
+    ``get_p_op`` and ``m / rho[0]`` maps, examples/Cylinder/Cylinder_Transient.py:170-176,
+    written over whole fields) run on the device into the modules' fixed
+    outputs: equal to the oracle's restatements of the same maps (the
+    subtract-multiply bit for bit; torch's device division within one ulp),
+    the same output object on every call, and replayed from a captured graph
+    with the same bits as eagerly."""
+    print("function maps and element ops", flush=True)
+    grid = (37, 29, 1)
+    rho = _r(grid + (1,), 7, 0.5, 2.0)
+    m = _r(grid + (2,), 8)
+    get_p = MapWise(lambda r, c2, density: c2 * (r - density))
+    p = get_p(T(rho, dev), 100.0, 1.0)
+    assert same_bits(p.cpu().numpy(), O.shifted_scale(rho, 100.0, 1.0))
+    assert get_p(T(rho, dev), 100.0, 1.0) is p
+    get_u = MapWise(lambda mm, r: mm / r[..., 0:1])
+    u = get_u(T(m, dev), T(rho, dev)).cpu().numpy()
+    want = O.divide_by_scalar(m, rho)
+    assert np.all(np.abs(u.view(np.int32).astype(np.int64) - want.view(np.int32)) <= 1)
+    prod = ElementWise(lambda a, b: a * b[..., 0:1] + 1.0)
+    got = prod(T(m, dev), T(rho, dev)).cpu().numpy()
+    assert got.shape == grid + (2,)
+    assert same_bits(got, m * rho + np.float32(1.0))
+    # an output shape other than A's, given explicitly
+    norm = ElementWise(lambda a, b: (a * a).sum(-1, keepdim=True) + b, output_dtype=(1,))
+    got = norm(T(m, dev), T(rho, dev)).cpu().numpy()
+    assert np.allclose(got, (m * m).sum(-1, keepdims=True) + rho, rtol=1e-6)
+    # graph replay: the maps' torch operations are captured with their output
+    a, b = T(m, dev), T(rho, dev)
+    eager = prod(a, b).clone()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = prod(a, b)
+    out.zero_()
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)
