@@ -507,6 +507,51 @@ int pde_heat_batch_step_f32(const float *u, float *un, const pde_batch_grid *gri
                             int64_t ngrids, const int32_t *tiles, int64_t ntiles,
                             int32_t rows_per_lane, int32_t waves, int64_t blocks, void *stream);
 
+/* Several heat steps per launch on LDS-resident tiles.  A tile is a region
+ * of one grid (rows [r0, r0+rh), columns [c0, c0+rw) of its padded field, at
+ * most PDE_HEAT_TILE_ROWS x PDE_HEAT_TILE_PITCH cells, inside the grid) that
+ * one workgroup loads once and steps `steps` times, writing back its output
+ * rectangle [o_r0, o_r1) x [o_c0, o_c1).  Its boundary ops are
+ * tops[op_beg .. op_end) (at most PDE_HEAT_TILE_MAXOPS): region-local cells
+ * (row * PDE_HEAT_TILE_PITCH + column) of the written scalar and of its
+ * source (-1 for a PDE_BC_SET), and the index of the full op in ops (its kind
+ * and value; PDE_BCV_ARRAY values are not allowed: they are evaluated once
+ * per step on the host side). */
+#define PDE_HEAT_TILE_PITCH 128
+#define PDE_HEAT_TILE_ROWS 128
+#define PDE_HEAT_TILE_MAXOPS 4096
+
+typedef struct pde_heat_tile {
+    int32_t grid;               /* index into grids                          */
+    int32_t r0, c0, rh, rw;     /* LDS region                                */
+    int32_t o_r0, o_r1;         /* output rows    [o_r0, o_r1)               */
+    int32_t o_c0, o_c1;         /* output columns [o_c0, o_c1)               */
+    int32_t op_beg, op_end;     /* the tile's boundary ops in tops           */
+    int32_t reserved;
+} pde_heat_tile; /* 48 bytes */
+
+typedef struct pde_heat_tile_op {
+    int32_t ldst, lsrc;         /* region-local cells (lsrc -1 for a SET)    */
+    int32_t op;                 /* index into ops                            */
+    int32_t reserved;
+} pde_heat_tile_op; /* 16 bytes */
+
+/* `steps` heat steps of every grid (each the in-place boundary of
+ * pde_bc_apply_inplace_f32, t as there, then pde_heat_step_f32), from u into
+ * un (u is not written; u != un); the same bits as `steps` pairs of those
+ * launches when the tiles' outputs cover every cell of every grid exactly
+ * once, each output lies `steps` cells inside every side where its region was
+ * cut from the grid, and every op's source cell stays inside the valid part
+ * of the region as long as its written cell does (the host planner builds and
+ * checks this).  The sub-steps share one time array.  No reference
+ * counterpart as one call: it replaces `steps` iterations of GridBoundary +
+ * Laplacian + ForwardEuler (examples/2D_Heat_Equation.py:58-62), for one grid
+ * or a batch. */
+int pde_heat_multistep_f32(const float *u, float *un, const pde_batch_grid *grids,
+                           int64_t ngrids, const pde_heat_tile *tiles, int64_t ntiles,
+                           const pde_heat_tile_op *tops, const pde_bc_op *ops, const float *t,
+                           int32_t steps, void *stream);
+
 /* ------------------------------------------------------------ node-wise glue */
 /* The element-wise operations between stencil outputs.  `nodes` counts padded
  * nodes (ghosts included: the reference's element-wise launches cover the

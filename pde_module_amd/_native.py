@@ -43,6 +43,7 @@ SIGNATURES = {
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
     "pde_bc_apply_batch_inplace_f32": [_P, _P, _I64, _P, _P, _P],
     "pde_batch_bc_capacity": [],
+    "pde_heat_multistep_f32": [_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I32, _P],
     "pde_heat_batch_step_f32": [_P, _P, _P, _I64, _P, _I64, _I32, _I32, _I64, _P],
     "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
     "pde_scalar_mul_f32": [_P, _P, _P, _I64, _I32, _P],
@@ -122,6 +123,18 @@ BATCH_GRID_DTYPE = np.dtype([("offset", "<i8"), ("op_begin", "<i8"), ("op_end", 
                              ("reserved0", "<i4"), ("wm", "<f4"), ("wc", "<f4"), ("wp", "<f4"),
                              ("alpha", "<f4"), ("dt", "<f4"), ("reserved1", "<f4")])
 assert BATCH_GRID_DTYPE.itemsize == 64
+
+# struct pde_heat_tile / pde_heat_tile_op (pde_heat_multistep_f32)
+HEAT_TILE_DTYPE = np.dtype([("grid", "<i4"), ("r0", "<i4"), ("c0", "<i4"), ("rh", "<i4"),
+                            ("rw", "<i4"), ("o_r0", "<i4"), ("o_r1", "<i4"), ("o_c0", "<i4"),
+                            ("o_c1", "<i4"), ("op_beg", "<i4"), ("op_end", "<i4"),
+                            ("reserved", "<i4")])
+assert HEAT_TILE_DTYPE.itemsize == 48
+HEAT_TILE_OP_DTYPE = np.dtype([("ldst", "<i4"), ("lsrc", "<i4"), ("op", "<i4"),
+                               ("reserved", "<i4")])
+assert HEAT_TILE_OP_DTYPE.itemsize == 16
+HEAT_TILE_PITCH = HEAT_TILE_ROWS = 128
+HEAT_TILE_MAXOPS = 4096
 
 # struct pde_ring_piece / pde_ring_entry / pde_ring_plan
 RING_PIECE_DTYPE = np.dtype([("r0", "<i4"), ("r1", "<i4"), ("c0", "<i4"), ("c1", "<i4"),

@@ -36,6 +36,7 @@ from .time_step import HeatStep
 ALIGN = 64            # floats: every instance starts on a 256-byte boundary
 ROWS_PER_LANE = (1, 2, 4, 8)
 WAVES = (1, 2, 4)
+MULTISTEP_CANDIDATES = (2, 4, 8, 16)
 
 
 def _plane(shape3):
@@ -114,9 +115,13 @@ class TileConfig:
     waves: int = 4
     blocks: int = 0          # 0: one block per tile; else a persistent grid
     xcd_slots: int = 8       # tile order: strips aligned to XCDs (8) or row-major (1)
+    # > 1: that many steps per launch on LDS-resident tiles (pde_heat_multistep_f32,
+    # the boundary applied inside); the other fields then do not apply
+    steps_per_launch: int = 1
 
     def key(self):
-        return (self.rows_per_lane, self.waves, self.blocks, self.xcd_slots)
+        return (self.rows_per_lane, self.waves, self.blocks, self.xcd_slots,
+                self.steps_per_launch)
 
 
 class HeatEnsemble:
@@ -158,6 +163,7 @@ class HeatEnsemble:
         from .FDM.boundary.tables import chunk_ops
         cap = N.load().pde_batch_bc_capacity()
         tables, begin = [], 0
+        self._local_ops = []       # per instance, chunk order, indices into its own field
         chunk_starts, chunked = [np.zeros(1, np.int64)], True
         grid = np.zeros(self.n, N.BATCH_GRID_DTYPE)
         for i, bc in enumerate(boundaries):
@@ -178,6 +184,7 @@ class HeatEnsemble:
             idx = np.concatenate([ops["dst"], rd]) if len(ops) else np.zeros(0, np.int64)
             if len(idx) and (idx.min() < 0 or idx.max() >= sizes[i]):
                 raise IndexError(f"instance {i}: boundary op index outside its field")
+            self._local_ops.append(ops.copy())
             ops["dst"] += offs[i]
             ops["src"] += offs[i]
             tables.append(ops)
@@ -202,6 +209,7 @@ class HeatEnsemble:
         self.graph_chunk = max(2, int(graph_chunk) // 2 * 2)
         self._graph = None
         self._tile_cache = {}
+        self._mplans = {}
         self.config = tiles or TileConfig()
 
     # ----------------------------------------------------------------- state
@@ -237,6 +245,34 @@ class HeatEnsemble:
                                      len(t))
         return self._tile_cache[key]
 
+    def _mstep_plan(self, halo: int):
+        """Device tiles and tile ops of the multi-step launch with this halo
+        (cached); raises multistep.PlanError when some instance's boundary
+        cannot take it."""
+        from .multistep import check_coverage, plan_batch
+        if halo not in self._mplans:
+            tiles, tops = plan_batch(self.p1, self.p2, self._local_ops, halo)
+            check_coverage(tiles, self.p1, self.p2)
+            dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(
+                self.device)
+            self._mplans[halo] = (dev(tiles), len(tiles), dev(tops))
+        return self._mplans[halo]
+
+    def multistep_ok(self, steps_per_launch: int) -> bool:
+        """Whether every instance's boundary allows this many steps per launch."""
+        from .multistep import PlanError
+        try:
+            self._mstep_plan(int(steps_per_launch))
+            return True
+        except PlanError:
+            return False
+
+    def _launch_multistep(self, src, dst, k: int, halo: int) -> None:
+        tiles, nt, tops = self._mstep_plan(halo)
+        N.call("pde_heat_multistep_f32", src.data_ptr(), dst.data_ptr(), self.grid_dev.data_ptr(),
+               self.n, tiles.data_ptr(), nt, tops.data_ptr(), self.ops_dev.data_ptr(),
+               self.t.data_ptr() if self.needs_t else None, int(k), N.stream_of(src))
+
     def _step(self, src: torch.Tensor, dst: torch.Tensor, cfg: TileConfig) -> None:
         st = N.stream_of(src)
         tp = self.t.data_ptr() if self.needs_t else None
@@ -252,10 +288,26 @@ class HeatEnsemble:
                cfg.waves, cfg.blocks, st)
 
     def _steps(self, k: int, cfg: TileConfig) -> None:
-        """k steps from bufs[cur]; k even leaves the state in bufs[cur]."""
+        """k steps from bufs[cur], one launch (pair of launches) per step or
+        steps_per_launch per launch; cur follows the launches."""
+        spl = int(cfg.steps_per_launch)
+        if spl > 1:
+            while k > 0:
+                n = min(spl, k)
+                self._launch_multistep(self.bufs[self.cur], self.bufs[1 - self.cur], n, spl)
+                self.cur = 1 - self.cur
+                k -= n
+            return
         for _ in range(k):
             self._step(self.bufs[self.cur], self.bufs[1 - self.cur], cfg)
             self.cur = 1 - self.cur
+
+    def _chunk(self, cfg: TileConfig) -> int:
+        """Steps per captured graph: an even number of launches, so a replay
+        leaves the state where it started."""
+        spl = max(1, int(cfg.steps_per_launch))
+        per = 2 * spl
+        return max(per, (self.graph_chunk + per - 1) // per * per)
 
     def set_time(self, t: float) -> None:
         """Time the ramp boundary values read (a device scalar, so graphs see it)."""
@@ -268,11 +320,14 @@ class HeatEnsemble:
         if not use_graph:
             self._steps(steps, cfg)
             return
-        k = self.graph_chunk
+        k = self._chunk(cfg)
         full = steps // k
         if full:
             if self._graph is None or self._graph[0] != (cfg.key(), self.cur):
-                self._tiles(cfg)              # upload before capture
+                if cfg.steps_per_launch > 1:
+                    self._mstep_plan(int(cfg.steps_per_launch))   # upload before capture
+                else:
+                    self._tiles(cfg)              # upload before capture
                 torch.cuda.synchronize(self.device)
                 g = torch.cuda.CUDAGraph()
                 start = self.cur
@@ -292,22 +347,28 @@ class HeatEnsemble:
             sms = torch.cuda.get_device_properties(self.device).multi_processor_count
             candidates = [TileConfig(r, w, b, x) for r in ROWS_PER_LANE for w in WAVES
                           for b in (0, 8 * sms) for x in (8, 1)]
+            candidates += [TileConfig(steps_per_launch=s) for s in MULTISTEP_CANDIDATES
+                           if self.multistep_ok(s)]
         saved = [b.clone() for b in self.bufs]
         cur = self.cur
         best, best_t = None, float("inf")
         timings = {}
         for cfg in candidates:
-            self._tiles(cfg)
+            if cfg.steps_per_launch > 1:
+                self._mstep_plan(int(cfg.steps_per_launch))
+            else:
+                self._tiles(cfg)
+            n = max(steps, 2 * int(cfg.steps_per_launch))
             self._steps(2, cfg)
             times = []
             for _ in range(reps):
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                self._steps(steps, cfg)
+                self._steps(n, cfg)
                 e1.record()
                 e1.synchronize()
-                times.append(e0.elapsed_time(e1) / steps)
+                times.append(e0.elapsed_time(e1) / n)
             timings[cfg.key()] = min(times)
             if min(times) < best_t:
                 best, best_t = cfg, min(times)

@@ -44,6 +44,68 @@ class HeatStep:
         # (key, DeviceRingPlan or None, table): the table is held so that its id,
         # part of the key, cannot be reused by a later table while cached
         self._ring = None
+        # the same for the multi-step tile plan: (key, device arrays or None, table)
+        self._mplan = None
+
+    def _multistep_plan(self, u: torch.Tensor, halo: int):
+        """Device tiles, tile ops and grid record of the multi-step launch for
+        u's shape, the boundary's current table and this halo, or None when
+        the field or its boundary cannot take it (2D scalar fields only)."""
+        shape = tuple(int(s_) for s_ in u.shape[:3])
+        if u.shape[3] != 1 or sum(1 for s_ in shape if s_ > 1) != 2:
+            return None
+        self.boundary._ensure_table(u)
+        table = self.boundary.table
+        key = (shape, u.device, id(table), int(halo))
+        if self._mplan is None or self._mplan[0] != key:
+            from ..multistep import PlanError, check_coverage, plan_grid
+            p1, p2 = [s_ for s_ in shape if s_ > 1]
+            try:
+                tiles, tops = plan_grid(p1, p2, table.host_ops, halo)
+                check_coverage(tiles, [p1], [p2])
+            except PlanError:
+                self._mplan = (key, None, table)
+                return None
+            grid = np.zeros(1, N.BATCH_GRID_DTYPE)
+            wm, wc, wp = self.weights
+            grid[0] = (0, 0, 0, p1, p2, self.ghost_cells, 0, wm, wc, wp, self.alpha, self.dt, 0.0)
+            dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(
+                u.device)
+            if not len(tops):
+                tops = np.zeros(1, N.HEAT_TILE_OP_DTYPE)
+            self._mplan = (key, (dev(tiles), len(tiles), dev(tops), dev(grid)), table)
+        return self._mplan[1]
+
+    def run(self, u: torch.Tensor, steps: int, t=None, steps_per_launch: int = 8) -> torch.Tensor:
+        """``steps`` steps from u with the same time t for every step: the
+        values of calling the step ``steps`` times.  On 2D scalar fields whose
+        boundary allows it, ``steps_per_launch`` steps run per launch on
+        LDS-resident tiles (``pde_heat_multistep_f32``: each launch reads the
+        state once and writes it once); otherwise one step per launch.
+        Returns the final state (one of the stepper's buffers); u may be
+        modified."""
+        N.check_field(u, "u")
+        steps = int(steps)
+        u = self._own(u)
+        spl = int(steps_per_launch)
+        plan = self._multistep_plan(u, spl) if spl > 1 and steps > 1 else None
+        if plan is None:
+            for _ in range(steps):
+                u = self(u, t)
+            return u
+        tiles, nt, tops, grid = plan
+        tt = self.boundary._time_tensor(u, t)
+        table = self.boundary.table
+        while steps > 0:
+            n = min(spl, steps)
+            nxt = self._bufs[1] if u.data_ptr() == self._bufs[0].data_ptr() else self._bufs[0]
+            N.call("pde_heat_multistep_f32", N.dev_ptr(u), N.dev_ptr(nxt), grid.data_ptr(), 1,
+                   tiles.data_ptr(), nt, tops.data_ptr(),
+                   table.ops.data_ptr(), tt.data_ptr() if tt is not None else None, n,
+                   N.stream_of(u))
+            u = nxt
+            steps -= n
+        return u
 
     def _ring_plan(self, u: torch.Tensor):
         """The ring plan of the one-launch step for u's shape and the
