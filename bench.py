@@ -41,6 +41,7 @@ WAVE_PAIR_BYTES = 44
 
 
 PREWARM_S = 0.3     # untimed seconds of the step before the warm-up steps
+MULTI_SPL = 12      # heat steps per launch on LDS-resident tiles (256^2 workload)
 
 
 def parse():
@@ -252,7 +253,7 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
     baseline (one HeatStep per grid, launched grid by grid) is timed over
     ``baseline_steps`` steps of every grid."""
     import torch
-    from pde_module_amd.batch import HeatEnsemble, PerInstanceHeat
+    from pde_module_amd.batch import HeatEnsemble, PerInstanceHeat, TileConfig
     from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
     from pde_module_amd import _native as NV
     from pde_module_amd.workloads import heat_ensemble_config
@@ -273,8 +274,12 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
     ens.run(steps)
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
-    # kernel-only time of the batched step (HIP events on the launch stream)
-    tiles, nt = ens._tiles(best)
+    # kernel-only time per step (HIP events on the launch stream): the best
+    # one-step-per-launch tile config's step kernel, and the multi-step launch
+    # divided by its steps when that is the tuned config
+    one = min((k for k in ens.timings if k[4] == 1), key=lambda k: ens.timings[k])
+    one_cfg = TileConfig(*one)
+    tiles, nt = ens._tiles(one_cfg)
     a, b = ens.bufs
     stream = torch.cuda.current_stream(dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -282,11 +287,20 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
     e0.record(stream)
     for _ in range(reps):
         NV.call("pde_heat_batch_step_f32", a.data_ptr(), b.data_ptr(), ens.grid_dev.data_ptr(),
-                ens.n, tiles.data_ptr(), nt, best.rows_per_lane, best.waves, best.blocks,
-                NV.stream_of(a))
+                ens.n, tiles.data_ptr(), nt, one_cfg.rows_per_lane, one_cfg.waves,
+                one_cfg.blocks, NV.stream_of(a))
     e1.record(stream)
     e1.synchronize()
     t_kernel = e0.elapsed_time(e1) / reps * 1e-3
+    t_multi = None
+    if best.steps_per_launch > 1:
+        spl = best.steps_per_launch
+        e0.record(stream)
+        for _ in range(reps):
+            ens._launch_multistep(a, b, spl, spl)
+        e1.record(stream)
+        e1.synchronize()
+        t_multi = e0.elapsed_time(e1) / reps * 1e-3
     padded = int((ens.p1 * ens.p2).sum())
     ref = PerInstanceHeat(mk(), dxs, cfg.g, alphas, dts)
     ref.load(fields)
@@ -301,14 +315,30 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
             "per_instance_ms_per_step": round(t_ref * 1e3, 4),
             "speedup": round(t_ref / t_batch, 2),
             "batched_glups": round(ens.points / t_batch / 1e9, 1),
-            "tile_config": {"rows_per_lane": best.rows_per_lane, "waves": best.waves,
-                            "blocks": best.blocks, "xcd_slots": best.xcd_slots},
-            "autotune_ms_per_step": {f"{k[0]}x{k[1]}/{k[2]}/{k[3]}": round(v, 4)
+            "tile_config": ({"steps_per_launch": best.steps_per_launch}
+                            if best.steps_per_launch > 1 else
+                            {"rows_per_lane": best.rows_per_lane, "waves": best.waves,
+                             "blocks": best.blocks, "xcd_slots": best.xcd_slots}),
+            "autotune_ms_per_step": {(f"{k[4]} steps/launch" if k[4] > 1 else
+                                      f"{k[0]}x{k[1]}/{k[2]}/{k[3]}"): round(v, 4)
                                      for k, v in sorted(ens.timings.items())},
-            "step_kernel_us": round(t_kernel * 1e6, 1),
-            "step_kernel_GBs": round(8 * padded / t_kernel / 1e9, 1),
-            "note": "batched: boundary + step kernels for all grids per step, 50-step graphs; "
-                    "per-instance: HeatStep per grid (boundary + fused step launches), eager"}
+            "multistep_kernel": (None if t_multi is None else {
+                "kernel": "heat_mstep_kernel", "steps_per_launch": best.steps_per_launch,
+                "us_per_launch": round(t_multi * 1e6, 1),
+                "us_per_step": round(t_multi / best.steps_per_launch * 1e6, 1),
+                "effective_GBs": round(8 * padded * best.steps_per_launch / t_multi / 1e9, 1),
+                "note": "effective = 8 B per padded node and step over the launch time; each "
+                        "launch reads and writes every field once for all its steps "
+                        "(LDS-resident tiles), so this exceeds the one-step HBM roofline"}),
+            "one_step_kernel": {"tile_config": {"rows_per_lane": one_cfg.rows_per_lane,
+                                                "waves": one_cfg.waves, "blocks": one_cfg.blocks,
+                                                "xcd_slots": one_cfg.xcd_slots},
+                                "us": round(t_kernel * 1e6, 1),
+                                "GBs": round(8 * padded / t_kernel / 1e9, 1),
+                                "ms_per_step_with_boundary": round(ens.timings[one], 4)},
+            "note": "batched: the autotuned form (several steps per launch on LDS-resident "
+                    "tiles with the boundary inside, or boundary + step kernels per step), "
+                    "graph-replayed; per-instance: HeatStep per grid, eager"}
 
 
 def heat_config_timing(dev, steps=1000, cpu_steps=100):
@@ -353,6 +383,29 @@ def heat_config_timing(dev, steps=1000, cpu_steps=100):
             snaps[per * (k + 1)] = a.cpu().numpy()
     torch.cuda.synchronize()
     t_fused = (time.perf_counter() - t0) / steps
+    # several steps per launch on LDS-resident tiles (HeatStep.run): the whole
+    # 1000-step loop captured as one graph (84 launches of up to 12 steps)
+    multi = HeatStep(mk_bc(), cfg.dx, g, cfg.alpha, cfg.dt)
+    multi.boundary.dirichlet_BC("ALL", 0.0)
+    ms = multi.run(u0.clone(), steps, steps_per_launch=MULTI_SPL)   # plan, buffers, warm-up
+    torch.cuda.synchronize()
+    ma = multi._bufs[0]
+    mgraph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(mgraph):
+        mout = multi.run(ma, steps, steps_per_launch=MULTI_SPL)
+    ma.copy_(u0)
+    torch.cuda.synchronize()
+    mgraph.replay()
+    torch.cuda.synchronize()
+    same_as_fused = bool(torch.equal(mout.view(torch.int32), a.view(torch.int32)))
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ma.copy_(u0)
+        mgraph.replay()
+    torch.cuda.synchronize()
+    t_multi = (time.perf_counter() - t0) / (reps * steps)
+    mplan = multi._mplan[0][4], multi._mplan[1][1]
     # module-by-module (the reference's loop shape), eager, 100 steps
     bcm = mk_bc()
     bcm.dirichlet_BC("ALL", 0.0)
@@ -395,13 +448,20 @@ def heat_config_timing(dev, steps=1000, cpu_steps=100):
             errs[str(step)] = float(np.max(np.abs(s[mask] - u64[mask])) / np.max(np.abs(u64[mask])))
     lups = cfg.n * cfg.n
     return {"grid": [cfg.n, cfg.n], "steps": steps,
+            "multistep_graph_us_per_step": round(t_multi * 1e6, 2),
+            "multistep_glups": round(lups / t_multi / 1e9, 2),
+            "multistep": {"steps_per_launch": MULTI_SPL, "region": list(mplan[0]),
+                          "tiles": int(mplan[1]), "state_after_1000_steps_bitwise_equal_"
+                          "to_one_step_per_launch": same_as_fused},
             "fused_graph_us_per_step": round(t_fused * 1e6, 2),
             "fused_glups": round(lups / t_fused / 1e9, 2),
             "module_loop_us_per_step": round(t_mod * 1e6, 2),
             "cpu_1thread_us_per_step": round(t_cpu * 1e6, 1),
             "cpu_glups": round(lups / t_cpu / 1e9, 4),
             "rel_error_vs_fp64": errs,
-            "note": "cache-resident (266 KB field): launch- and latency-bound, not an HBM figure"}
+            "note": "cache-resident (266 KB field): launch- and latency-bound, not an HBM "
+                    "figure; multistep: several steps per launch on LDS-resident tiles "
+                    "(pde_heat_multistep_f32), fused: one launch per step"}
 
 
 def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=10):

@@ -519,7 +519,7 @@ int pde_heat_batch_step_f32(const float *u, float *un, const pde_batch_grid *gri
  * per step on the host side). */
 #define PDE_HEAT_TILE_PITCH 128
 #define PDE_HEAT_TILE_ROWS 128
-#define PDE_HEAT_TILE_MAXOPS 4096
+#define PDE_HEAT_TILE_MAXOPS 2048
 
 typedef struct pde_heat_tile {
     int32_t grid;               /* index into grids                          */

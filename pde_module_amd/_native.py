@@ -134,7 +134,7 @@ HEAT_TILE_OP_DTYPE = np.dtype([("ldst", "<i4"), ("lsrc", "<i4"), ("op", "<i4"),
                                ("reserved", "<i4")])
 assert HEAT_TILE_OP_DTYPE.itemsize == 16
 HEAT_TILE_PITCH = HEAT_TILE_ROWS = 128
-HEAT_TILE_MAXOPS = 4096
+HEAT_TILE_MAXOPS = 2048
 
 # struct pde_ring_piece / pde_ring_entry / pde_ring_plan
 RING_PIECE_DTYPE = np.dtype([("r0", "<i4"), ("r1", "<i4"), ("c0", "<i4"), ("c1", "<i4"),

@@ -36,7 +36,7 @@ from .time_step import HeatStep
 ALIGN = 64            # floats: every instance starts on a 256-byte boundary
 ROWS_PER_LANE = (1, 2, 4, 8)
 WAVES = (1, 2, 4)
-MULTISTEP_CANDIDATES = (2, 4, 8, 16)
+MULTISTEP_CANDIDATES = (4, 8, 12, 16)
 
 
 def _plane(shape3):

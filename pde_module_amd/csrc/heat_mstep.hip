@@ -24,12 +24,53 @@ namespace pde {
 
 constexpr int HT_PITCH = PDE_HEAT_TILE_PITCH;   // LDS row pitch (cells): region width cap
 constexpr int HT_ROWS = PDE_HEAT_TILE_ROWS;     // region height cap
-constexpr int HT_TX = 64, HT_TY = 16;           // 1024 threads
-constexpr int HT_W = HT_PITCH / HT_TX;          // columns per thread
-constexpr int HT_H = HT_ROWS / HT_TY;           // rows per thread (a contiguous band)
-constexpr int HT_OPT = PDE_HEAT_TILE_MAXOPS / (HT_TX * HT_TY);   // ops per thread
+constexpr int HT_NT = 1024;                     // threads per workgroup
+constexpr int HT_RPB = 4;                       // rows per thread band (at most)
+constexpr int HT_OPT = PDE_HEAT_TILE_MAXOPS / HT_NT;   // ops per thread
 
-__global__ __launch_bounds__(HT_TX *HT_TY) void heat_mstep_kernel(
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l receives
+// lane l-1's value (from_left) or lane l+1's (from_right); 0 past the ends.
+__device__ __forceinline__ float ht_from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float ht_from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+struct f4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ float f4el(const f4 &v, int i)
+{
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+// The heat update of one cell from its u2 neighbours, in heat_rows' order
+// (tile2d.hpp acc5: rows x-1, x, x+1, then columns k-1, k, k+1).
+__device__ __forceinline__ float ht_cell(float up, float mid, float dn, float zl, float zr,
+                                         float wm, float wc, float wp, float alpha, float dt)
+{
+    float acc = 0.0f;
+    acc += wm * up;
+    acc += wc * mid;
+    acc += wp * dn;
+    acc += wm * zl;
+    acc += wc * mid;
+    acc += wp * zr;
+    return mid + dt * (alpha * acc);
+}
+
+// Thread layout: a band of `lanes` consecutive threads (a power of two, at
+// most 32, at least the region's float4 columns) covers one band of up to
+// HT_RPB rows, thread c4 of the band the 4 columns 4*c4 .. 4*c4+3; bands never
+// straddle a wave, so the DPP shifts bring each chunk its left and right
+// neighbours (the band's first and last chunks only border region-edge
+// cells, which are never stepped).
+__global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     const float *__restrict__ u, float *__restrict__ un, const pde_batch_grid *__restrict__ grids,
     int64_t ngrids, const pde_heat_tile *__restrict__ tiles,
     const pde_heat_tile_op *__restrict__ tops, const pde_bc_op *__restrict__ ops,
@@ -46,61 +87,87 @@ __global__ __launch_bounds__(HT_TX *HT_TY) void heat_mstep_kernel(
         T.r0 + rh > G.p1 || T.c0 + rw > G.p2 || T.op_end - T.op_beg > PDE_HEAT_TILE_MAXOPS ||
         T.op_end < T.op_beg)
         return;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * HT_TX + tx;
+    const int tid = threadIdx.x;
     const int64_t P2 = G.p2;
     const float *ug = u + G.offset;
 
-    for (int lr = ty; lr < rh; lr += HT_TY) {
-        const float *row = ug + (int64_t)(T.r0 + lr) * P2 + T.c0;
-        for (int lc = tx; lc < rw; lc += HT_TX) A[lr * HT_PITCH + lc] = row[lc];
+    for (int c = tid; c < rh * HT_PITCH; c += HT_NT) {
+        const int lr = c / HT_PITCH, lc = c - lr * HT_PITCH;
+        if (lc < rw) A[c] = ug[(int64_t)(T.r0 + lr) * P2 + T.c0 + lc];
     }
 
-    // this thread's boundary ops: value (constant over the sub-steps: one
-    // time array per launch), kind, source and destination cells
+    // this thread's boundary ops, folded for the sub-steps (one time array per
+    // launch): SET r = b; COPY r = in + (-0); GHOST_D r = 2v + (-in);
+    // GHOST_N r = coef*v + in -- each the same rounding as op_eval's
     const int nop = T.op_end - T.op_beg;
     const float tv = t ? t[0] : 0.0f;
-    float ov[HT_OPT];
-    int ok[HT_OPT], osrc[HT_OPT], odst[HT_OPT];
-    float oc[HT_OPT];
+    int odst[HT_OPT], osrc[HT_OPT];
+    float ob[HT_OPT];
+    bool oneg[HT_OPT];
 #pragma unroll
     for (int j = 0; j < HT_OPT; ++j) {
-        const int e = tid + j * HT_TX * HT_TY;
+        const int e = tid + j * HT_NT;
         odst[j] = -1;
+        osrc[j] = -1;
+        ob[j] = 0.0f;
+        oneg[j] = false;
         if (e < nop) {
             const pde_heat_tile_op m = tops[T.op_beg + e];
             const pde_bc_op op = ops[m.op];
             const int cap = rh * HT_PITCH;
             if (m.ldst >= 0 && m.ldst < cap && m.lsrc < cap &&
                 (op.kind == PDE_BC_SET || m.lsrc >= 0)) {
+                const float v = op_value_tv(op, tv, t);
                 odst[j] = m.ldst;
-                osrc[j] = m.lsrc;
-                ok[j] = op.kind;
-                ov[j] = op_value_tv(op, tv, t);
-                oc[j] = op.coef;
+                osrc[j] = op.kind == PDE_BC_SET ? -1 : m.lsrc;
+                switch (op.kind) {
+                case PDE_BC_SET: ob[j] = v; break;
+                case PDE_BC_COPY: ob[j] = -0.0f; break;
+                case PDE_BC_GHOST_D: ob[j] = 2.0f * v; oneg[j] = true; break;
+                default: ob[j] = op.coef * v; break;
+                }
             }
         }
     }
 
-    // rows of this thread's band and their position in the grid
-    const int lr0 = ty * HT_H;
-    const int g = G.ghost, P1 = G.p1;
+    // this thread's chunk: band, rows, columns and which of its cells step
+    const int ncol4 = (rw + 3) >> 2;
+    int lanes = 1;
+    while (lanes < ncol4) lanes <<= 1;
+    const int bands = HT_NT / lanes;
+    const int rpb = (rh + bands - 1) / bands;   // <= HT_RPB (the planner's regions fit)
+    const int band = tid / lanes, c4 = tid - band * lanes;
+    const int lc0 = 4 * c4, lr0 = band * rpb;
+    const bool active = c4 < ncol4 && lr0 < rh && rpb <= HT_RPB;
+    const int g = G.ghost;
+    bool cm[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int lc = lc0 + i, k = T.c0 + lc;
+        cm[i] = lc > 0 && lc < rw - 1 && k >= g && k < G.p2 - g;
+    }
+    bool rm[HT_RPB];
+#pragma unroll
+    for (int h = 0; h < HT_RPB; ++h) {
+        const int lr = lr0 + h, x = T.r0 + lr;
+        rm[h] = active && h < rpb && lr > 0 && lr < rh - 1 && x >= g && x < G.p1 - g;
+    }
     const float wm = G.wm, wc = G.wc, wp = G.wp, alpha = G.alpha, dt = G.dt;
+    const f4 *A4 = reinterpret_cast<const f4 *>(A);
+    f4 *W4 = reinterpret_cast<f4 *>(A);
+    auto row4 = [&](int lr) {
+        // rows outside the region read a clamped row (results masked)
+        const int r = lr < 0 ? 0 : (lr > HT_ROWS - 1 ? HT_ROWS - 1 : lr);
+        return A4[r * (HT_PITCH / 4) + (c4 < HT_PITCH / 4 ? c4 : HT_PITCH / 4 - 1)];
+    };
     __syncthreads();
     for (int s = 0; s < steps; ++s) {
         if (nop > 0) {
             float val[HT_OPT];
 #pragma unroll
             for (int j = 0; j < HT_OPT; ++j) {
-                if (odst[j] < 0) continue;
-                const float in = ok[j] == PDE_BC_SET ? 0.0f : A[osrc[j]];
-                float r;
-                switch (ok[j]) {
-                case PDE_BC_SET: r = ov[j]; break;
-                case PDE_BC_COPY: r = in; break;
-                case PDE_BC_GHOST_D: r = 2.0f * ov[j] - in; break;
-                default: r = oc[j] * ov[j] + in; break;
-                }
-                val[j] = r;
+                const float in = osrc[j] >= 0 ? A[osrc[j]] : 0.0f;
+                val[j] = osrc[j] < 0 ? ob[j] : ob[j] + (oneg[j] ? -in : in);
             }
             __syncthreads();
 #pragma unroll
@@ -108,48 +175,33 @@ __global__ __launch_bounds__(HT_TX *HT_TY) void heat_mstep_kernel(
                 if (odst[j] >= 0) A[odst[j]] = val[j];
             __syncthreads();
         }
-        // the step on u2, each thread walking its band down every column
-        float nv[HT_W][HT_H];
+        f4 nv[HT_RPB];
+        f4 up = row4(lr0 - 1), mid = row4(lr0);
 #pragma unroll
-        for (int w = 0; w < HT_W; ++w) {
-            const int lc = tx + w * HT_TX;
-            const bool col_ok = lc < rw;
-            const int k = T.c0 + lc;
-            const bool kin = lc > 0 && lc < rw - 1 && k >= g && k < G.p2 - g;
-            float up = col_ok && lr0 > 0 && lr0 - 1 < rh ? A[(lr0 - 1) * HT_PITCH + lc] : 0.0f;
-            float mid = col_ok && lr0 < rh ? A[lr0 * HT_PITCH + lc] : 0.0f;
-#pragma unroll
-            for (int h = 0; h < HT_H; ++h) {
-                const int lr = lr0 + h;
-                const float dn = col_ok && lr + 1 < rh ? A[(lr + 1) * HT_PITCH + lc] : 0.0f;
-                float o = mid;
-                const int x = T.r0 + lr;
-                if (kin && lr > 0 && lr < rh - 1 && x >= g && x < P1 - g) {
-                    const float zl = A[lr * HT_PITCH + lc - 1];
-                    const float zr = A[lr * HT_PITCH + lc + 1];
-                    float acc = 0.0f;
-                    acc += wm * up;
-                    acc += wc * mid;
-                    acc += wp * dn;
-                    acc += wm * zl;
-                    acc += wc * mid;
-                    acc += wp * zr;
-                    o = mid + dt * (alpha * acc);
-                }
-                nv[w][h] = o;
-                up = mid;
-                mid = dn;
+        for (int h = 0; h < HT_RPB; ++h) {
+            const f4 dn = row4(lr0 + h + 1);
+            // every lane shifts (no divergence around the DPP moves)
+            const float left = ht_from_left(mid.w), right = ht_from_right(mid.x);
+            f4 o = mid;
+            if (rm[h]) {
+                const float n0 = ht_cell(up.x, mid.x, dn.x, left, mid.y, wm, wc, wp, alpha, dt);
+                const float n1 = ht_cell(up.y, mid.y, dn.y, mid.x, mid.z, wm, wc, wp, alpha, dt);
+                const float n2 = ht_cell(up.z, mid.z, dn.z, mid.y, mid.w, wm, wc, wp, alpha, dt);
+                const float n3 = ht_cell(up.w, mid.w, dn.w, mid.z, right, wm, wc, wp, alpha, dt);
+                o.x = cm[0] ? n0 : mid.x;
+                o.y = cm[1] ? n1 : mid.y;
+                o.z = cm[2] ? n2 : mid.z;
+                o.w = cm[3] ? n3 : mid.w;
             }
+            nv[h] = o;
+            up = mid;
+            mid = dn;
         }
         __syncthreads();
+        if (active) {
 #pragma unroll
-        for (int w = 0; w < HT_W; ++w) {
-            const int lc = tx + w * HT_TX;
-#pragma unroll
-            for (int h = 0; h < HT_H; ++h) {
-                const int lr = lr0 + h;
-                if (lc < rw && lr < rh) A[lr * HT_PITCH + lc] = nv[w][h];
-            }
+            for (int h = 0; h < HT_RPB; ++h)
+                if (h < rpb && lr0 + h < rh) W4[(lr0 + h) * (HT_PITCH / 4) + c4] = nv[h];
         }
         __syncthreads();
     }
@@ -158,9 +210,10 @@ __global__ __launch_bounds__(HT_TX *HT_TY) void heat_mstep_kernel(
     const int orow0 = T.o_r0 - T.r0, orow1 = T.o_r1 - T.r0;
     const int ocol0 = T.o_c0 - T.c0, ocol1 = T.o_c1 - T.c0;
     if (orow0 < 0 || orow1 > rh || ocol0 < 0 || ocol1 > rw) return;
-    for (int lr = orow0 + ty; lr < orow1; lr += HT_TY) {
-        float *row = ung + (int64_t)(T.r0 + lr) * P2 + T.c0;
-        for (int lc = ocol0 + tx; lc < ocol1; lc += HT_TX) row[lc] = A[lr * HT_PITCH + lc];
+    const int ow = ocol1 - ocol0;
+    for (int c = tid; c < (orow1 - orow0) * ow; c += HT_NT) {
+        const int r = c / ow, lc = ocol0 + (c - r * ow), lr = orow0 + r;
+        ung[(int64_t)(T.r0 + lr) * P2 + T.c0 + lc] = A[lr * HT_PITCH + lc];
     }
 }
 
@@ -176,7 +229,7 @@ extern "C" int pde_heat_multistep_f32(const float *u, float *un, const pde_batch
     if (!u || !un || u == un || !grids || ngrids < 1 || ntiles < 0 || steps < 1) return PDE_EINVAL;
     if (ntiles == 0) return PDE_OK;
     if (!tiles || ntiles > 0x7fffffff || ngrids > 0x7fffffff) return PDE_EINVAL;
-    hipLaunchKernelGGL(heat_mstep_kernel, dim3((unsigned)ntiles), dim3(HT_TX, HT_TY), 0,
+    hipLaunchKernelGGL(heat_mstep_kernel, dim3((unsigned)ntiles), dim3(HT_NT), 0,
                        as_stream(stream), u, un, grids, ngrids, tiles, tops, ops, t, steps);
     return launch_status();
 }

@@ -56,12 +56,17 @@ def split_axis(P: int, cap: int, halo: int):
     raise PlanError("cannot split the axis")
 
 
-def plan_grid(P1: int, P2: int, ops: np.ndarray, halo: int, grid: int = 0, op_base: int = 0):
+def plan_grid(P1: int, P2: int, ops: np.ndarray, halo: int, grid: int = 0, op_base: int = 0,
+              rows_cap: int = ROWS, cols_cap: int = PITCH):
     """Tiles (HEAT_TILE_DTYPE) and tile ops (HEAT_TILE_OP_DTYPE, op indices
     op_base + index into ``ops``) of one scalar 2D grid of padded extents
     P1 x P2 (column index fastest) whose boundary is the resolved op table
-    ``ops`` (BC_OP_DTYPE, flat indices into the grid)."""
+    ``ops`` (BC_OP_DTYPE, flat indices into the grid).  Regions are at most
+    rows_cap x cols_cap cells (smaller regions give a small grid more
+    workgroups, at the price of more halo cells)."""
     P1, P2, halo = int(P1), int(P2), int(halo)
+    if not (1 <= rows_cap <= ROWS and 1 <= cols_cap <= PITCH):
+        raise PlanError(f"region caps must be at most {ROWS} x {PITCH}")
     if halo < 1:
         raise PlanError("halo must be at least 1")
     ops = np.asarray(ops)
@@ -77,8 +82,8 @@ def plan_grid(P1: int, P2: int, ops: np.ndarray, halo: int, grid: int = 0, op_ba
     dx_, dy_ = np.divmod(dst, P2)
     sx_, sy_ = np.divmod(src, P2)
     tiles, tops = [], []
-    for (or0, or1, r0, r1) in split_axis(P1, ROWS, halo):
-        for (oc0, oc1, c0, c1) in split_axis(P2, PITCH, halo):
+    for (or0, or1, r0, r1) in split_axis(P1, rows_cap, halo):
+        for (oc0, oc1, c0, c1) in split_axis(P2, cols_cap, halo):
             rh, rw = r1 - r0, c1 - c0
             sel = np.flatnonzero((dx_ >= r0) & (dx_ < r1) & (dy_ >= c0) & (dy_ < c1))
             inf = np.int64(1 << 40)
@@ -122,12 +127,13 @@ def plan_grid(P1: int, P2: int, ops: np.ndarray, halo: int, grid: int = 0, op_ba
         (np.concatenate(tops) if tops else np.zeros(0, N.HEAT_TILE_OP_DTYPE))
 
 
-def plan_batch(p1s, p2s, op_tables, halo: int):
+def plan_batch(p1s, p2s, op_tables, halo: int, rows_cap: int = ROWS, cols_cap: int = PITCH):
     """Tiles and tile ops of many grids: grid i's tiles carry grid = i, op
     indices into the concatenation of ``op_tables`` (the packed table)."""
     tiles, tops, base, nt = [], [], 0, 0
     for i, (P1, P2, ops) in enumerate(zip(p1s, p2s, op_tables)):
-        t, o = plan_grid(P1, P2, ops, halo, grid=i, op_base=base)
+        t, o = plan_grid(P1, P2, ops, halo, grid=i, op_base=base, rows_cap=rows_cap,
+                         cols_cap=cols_cap)
         t["op_beg"] += nt
         t["op_end"] += nt
         nt += len(o)

@@ -123,8 +123,9 @@ CASES = [((40, 37), 1, _legacy_mixed, 8), ((128, 128), 1, _functional, 4),
          ((260, 261), 1, _legacy_shift, 16), ((129, 40), 1, _legacy_shift, 3)]
 
 
+@pytest.mark.parametrize("caps", [(128, 128), (64, 48)])
 @pytest.mark.parametrize("shape,g,make,k", CASES)
-def test_emulated_tiles_equal_sequential_steps(shape, g, make, k):
+def test_emulated_tiles_equal_sequential_steps(shape, g, make, k, caps):
     P1, P2 = shape
     rng = np.random.default_rng(P1 * P2)
     u = rng.uniform(-1, 1, (P1, P2)).astype(F)
@@ -132,9 +133,14 @@ def test_emulated_tiles_equal_sequential_steps(shape, g, make, k):
     dt = 0.2 * dx * dx / alpha
     bc = make(torch.from_numpy(u.reshape(P1, P2, 1, 1)), dx, g)
     ops = bc.resolved_ops()
-    tiles, tops = plan_grid(P1, P2, ops, k)
+    if 2 * k >= min(caps):
+        with pytest.raises(PlanError):
+            plan_grid(P1, P2, ops, k, rows_cap=caps[0], cols_cap=caps[1])
+        return
+    tiles, tops = plan_grid(P1, P2, ops, k, rows_cap=caps[0], cols_cap=caps[1])
     check_coverage(tiles, [P1], [P2])
-    if P1 <= 128 and P2 <= 128:
+    assert np.all(tiles["rh"] <= caps[0]) and np.all(tiles["rw"] <= caps[1])
+    if P1 <= caps[0] and P2 <= caps[1]:
         assert len(tiles) == 1 and tiles["rh"][0] == P1 and tiles["rw"][0] == P2
     w = O.laplacian_weights(F(dx))
     for steps in (k, max(1, k // 2)):      # a plan for k serves any count up to k
