@@ -68,7 +68,7 @@ class Boundary(ExplicitUniformGridStencil):
                 raise AssertionError("output ids out of range")
             return output_ids
         raise TypeError(
-            f"Valid Types are: int|np.ndarray|list|tuple|None got {type(output_ids)} instead")
+            f"output ids are an int, a sequence of ints or None, not {type(output_ids).__name__}")
 
     def set_BC(self, face_ids, value: float, boundary_type: int, outputs_ids) -> None:
         if isinstance(face_ids, str):
@@ -141,15 +141,15 @@ class Boundary(ExplicitUniformGridStencil):
     def no_slip(self, group) -> None:
         if self.inputs[0] != self.dimension:
             raise AssertionError(
-                "Valid only when input_dtype is vector with same length equal to dimension of field")
+                "this needs a velocity field: one component per grid dimension")
         self.dirichlet_BC(group, 0.0)
 
     def impermeable(self, group: str) -> None:
         if self.inputs[0] != self.dimension:
             raise AssertionError(
-                "Valid only when input_dtype is vector with same length equal to dimension of field")
+                "this needs a velocity field: one component per grid dimension")
         if group not in self.groups or group not in {"-X", "+X", "-Y", "+Y", "-Z", "+Z"}:
-            raise AssertionError("{'-X','+X','-Y','+Y','-Z','+Z'} are valid groups")
+            raise AssertionError("impermeable takes one face group: -X, +X, -Y, +Y, -Z or +Z")
         axis = "XYZ".index(group[-1])
         self.vonNeumann_BC(group, 0.0)
         self.dirichlet_BC(group, 0.0, axis)

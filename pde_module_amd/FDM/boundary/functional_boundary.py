@@ -114,9 +114,9 @@ class GridBoundary(ExplicitUniformGridStencil):
     def impermeable(self, group: str) -> None:
         if self.inputs[0] != self.dimension:
             raise AssertionError(
-                "Valid only when input_dtype is vector with same length equal to dimension of field")
+                "impermeable needs a velocity field: one component per grid dimension")
         if group not in {"-X", "+X", "-Y", "+Y", "-Z", "+Z"}:
-            raise AssertionError("{'-X','+X','-Y','+Y','-Z','+Z'} are valid groups")
+            raise AssertionError("impermeable takes one face group: -X, +X, -Y, +Y, -Z or +Z")
         axis = "XYZ".index(group[-1])
         self.vonNeumann_BC(group, 0.0)
         self.dirichlet_BC(group, 0.0, axis)

@@ -33,16 +33,16 @@ class StencilKernel:
 def _check_stencil(stencil):
     w = tuple(np.float32(x) for x in stencil)
     if len(w) % 2 != 1:
-        raise AssertionError("stencil must be odd sized")
+        raise AssertionError("the stencil needs an odd number of weights")
     if len(w) != 3:
-        raise ValueError("Custom stencil not implemented yet")
+        raise ValueError("only the default central stencil is supported")
     return w
 
 
 def create_Laplacian_kernel(input_shape, grid_shape, stencil, ghost_cells):
     """Reference ``pm/FDM/kernel/laplacian.py:7-61``: vector fields only."""
     if len(input_shape) != 1:
-        raise AssertionError("Input type must be of vector")
+        raise AssertionError("the input must be a vector field")
     return StencilKernel("laplacian", _check_stencil(stencil), tuple(input_shape),
                          tuple(input_shape), tuple(grid_shape), ghost_cells)
 
@@ -51,10 +51,10 @@ def create_Grad_kernel(input_shape, output_shape, grid_shape, stencil, ghost_cel
     """Reference ``pm/FDM/kernel/grad.py:8-102`` (and ``create_grad_op:56-102``)."""
     dims, _ = eligible_dims_and_shift(grid_shape, ghost_cells)
     if len(input_shape) != 1:
-        raise AssertionError("Grad Only For Vectors")
+        raise AssertionError("the gradient takes vector fields")
     if len(output_shape) == 1:
         if input_shape[0] != 1:
-            raise AssertionError("Grad Kernel with Vector output only availiable for vectors of length 1")
+            raise AssertionError("a vector-valued gradient needs a scalar (length-1) input; use the matrix form")
         if output_shape[0] != len(dims):
             raise AssertionError("vector output length must equal the grid dimension")
     else:
@@ -70,7 +70,7 @@ def create_Divergence_kernel(input_shape, output_shape, grid_shape, stencil, gho
     dims, _ = eligible_dims_and_shift(grid_shape, ghost_cells)
     if len(input_shape) == 2:
         if input_shape[1] != len(dims):
-            raise AssertionError("Dimensions of field and num col in matrix must match")
+            raise AssertionError("the matrix must have one column per grid dimension")
     elif input_shape[0] != len(dims):
         raise AssertionError("vector input length must equal the grid dimension")
     return StencilKernel("divergence", _check_stencil(stencil), tuple(input_shape),

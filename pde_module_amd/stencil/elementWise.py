@@ -45,7 +45,7 @@ class ElementWise(Stencil):
     def initialize_kernel(self, array_A: torch.Tensor, array_B: torch.Tensor, *args,
                           **kwargs) -> None:
         if tuple(array_A.shape[:3]) != tuple(array_B.shape[:3]):
-            raise AssertionError("input arrays must be the same Shape!")
+            raise AssertionError("both fields must have the same grid shape")
         self.array_A_dtype, self.array_B_dtype = _elem(array_A), _elem(array_B)
         produced = self.element_op.out_shape(self.array_A_dtype, self.array_B_dtype)
         if self.output_dtype is None:

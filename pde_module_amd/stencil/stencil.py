@@ -48,7 +48,7 @@ class Stencil:
 
     def forward(self, *args, **kwargs) -> Any:
         raise NotImplementedError(
-            "forward method in Stencil Object must be implemented by the user")
+            "a Stencil subclass defines forward()")
 
     def _set_registry(self, call: str) -> str:
         name = f"_{call}_list"
