@@ -59,6 +59,9 @@ def parse():
                    help="profiling runs: only the 2048^2 wave workload, its JSON on stdout")
     p.add_argument("--wave-steps", type=int, default=2000,
                    help="steps of the wave workload (profiling runs shorten it)")
+    p.add_argument("--only-heat", action="store_true",
+                   help="profiling runs: only the heat workloads (512-grid ensemble, 256^2 "
+                        "loop), their JSON on stdout")
     return p.parse_args()
 
 
@@ -292,6 +295,14 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
     e1.record(stream)
     e1.synchronize()
     t_kernel = e0.elapsed_time(e1) / reps * 1e-3
+    # a plain copy of the packed buffer: the bandwidth yardstick, and the
+    # calibration of counter runs over this workload
+    e0.record(stream)
+    for _ in range(reps):
+        NV.call("pde_copy_f32", a.data_ptr(), b.data_ptr(), a.numel(), NV.stream_of(a))
+    e1.record(stream)
+    e1.synchronize()
+    t_copy = e0.elapsed_time(e1) / reps * 1e-3
     t_multi = None
     if best.steps_per_launch > 1:
         spl = best.steps_per_launch
@@ -330,6 +341,9 @@ def heat_ensemble_timing(dev, steps=500, baseline_steps=10, seed=3):
                 "note": "effective = 8 B per padded node and step over the launch time; each "
                         "launch reads and writes every field once for all its steps "
                         "(LDS-resident tiles), so this exceeds the one-step HBM roofline"}),
+            "copy_of_packed_buffer": {"bytes": 8 * int(a.numel()),
+                                      "us": round(t_copy * 1e6, 1),
+                                      "GBs": round(8 * a.numel() / t_copy / 1e9, 1)},
             "one_step_kernel": {"tile_config": {"rows_per_lane": one_cfg.rows_per_lane,
                                                 "waves": one_cfg.waves, "blocks": one_cfg.blocks,
                                                 "xcd_slots": one_cfg.xcd_slots},
@@ -558,6 +572,10 @@ def main():
     if args.only_wave:
         print(json.dumps({"wave2048_fwd_vs_fwd_adjoint":
                           wave_adjoint_timing(dev, steps=args.wave_steps)}))
+        return
+    if args.only_heat:
+        print(json.dumps({"heat_ensemble_512": heat_ensemble_timing(dev),
+                          "heat_256": heat_config_timing(dev)}))
         return
     gen = torch.Generator(device=dev).manual_seed(0)
     s = torch.rand((P, P, P, 1), device=dev, generator=gen) * 2 - 1
