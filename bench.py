@@ -478,15 +478,19 @@ def heat_config_timing(dev, steps=1000, cpu_steps=100):
                     "(pde_heat_multistep_f32), fused: one launch per step"}
 
 
-def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=10):
+def projection_cavity_timing(dev, n=1024, steps=2000, batch_steps=20, seeds=16,
+                             checkpoint_every=10):
     """Projection-method cavity workload (1024^2, Re = 100, 50 Jacobi sweeps per
-    step, seeded divergence-free u0 of amplitude 1e-3).  One instance: the
-    graph-replayed forward, then the kinetic-energy gradient dJ/du0 through the
-    transposed steps (CavityRollout: every step's velocity kept, and
-    checkpoints every ``checkpoint_every`` steps with recompute), each timed
-    after a warm-up run that captures the graphs.  Batch: ``seeds`` instances
-    (seeds 0..15) as 16 instances in one captured graph, and as one
-    ProjectionCavityBatch (forward and batched gradient), timed per step."""
+    step, seeded divergence-free u0 of amplitude 1e-3).  One instance over all
+    ``steps`` steps: the graph-replayed forward, then the kinetic-energy
+    gradient dJ/du0 through the transposed steps (CavityRollout: every step's
+    velocity kept, 17 GB at 2000 steps, and checkpoints every
+    ``checkpoint_every`` steps with recompute), each timed after a warm-up run
+    that captures the graphs.  Batch: ``seeds`` instances (seeds 0..15) as 16
+    instances in one captured graph, and as one ProjectionCavityBatch (forward
+    and batched gradient, every state kept), timed per step over
+    ``batch_steps`` steps (every state of 16 seeds for 2000 steps would be
+    270 GB)."""
     import torch
     from pde_module_amd.flow import CavityRollout, ProjectionCavity, ProjectionCavityBatch
     from pde_module_amd.workloads import ns_cavity_u0
@@ -525,17 +529,18 @@ def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=1
     gr.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(batch_steps):
         gr.replay()
     torch.cuda.synchronize()
-    out["16_instances_one_graph_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    out["16_instances_one_graph_ms_per_step"] = round((time.perf_counter() - t0) / batch_steps
+                                                      * 1e3, 3)
     del gr, us, ps
     # the same 16 seeds as one ProjectionCavityBatch (every instance's sweeps in
     # one launch per sweep), forward and the batched gradient
     bat = ProjectionCavityBatch(n, seeds, jacobi_sweeps=50, device=dev)
     ub = torch.from_numpy(np.stack([ns_cavity_u0(sd, n) for sd in range(seeds)])).to(dev)
     pb = torch.zeros((seeds,) + cav.shape + (1,), device=dev)
-    ro = CavityRollout(bat, steps, checkpoint_every=None, use_graph=True)
+    ro = CavityRollout(bat, batch_steps, checkpoint_every=None, use_graph=True)
     for rep in range(2):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -545,7 +550,8 @@ def projection_cavity_timing(dev, n=1024, steps=20, seeds=16, checkpoint_every=1
         ro.backward_states(area * uN * cav.mask, torch.zeros_like(pb))
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-    fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    fwd, bwd = (t1 - t0) / batch_steps, (t2 - t1) / batch_steps
+    out["batch_steps"] = batch_steps
     out["batch_of_16_ms_per_step"] = round(fwd * 1e3, 3)
     out["batch_of_16_fwd_plus_dKE_du0_ms_per_step"] = round((fwd + bwd) * 1e3, 3)
     out["batch_speedup_vs_16_instances"] = round(out["16_instances_one_graph_ms_per_step"] /
