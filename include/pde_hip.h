@@ -33,6 +33,7 @@ extern "C" {
 #define PDE_OK 0
 #define PDE_EINVAL (-1)   /* null pointer, bad count or unsupported layout */
 #define PDE_ESHAPE (-2)   /* grid too small for the ghost width / stencil  */
+#define PDE_ETOOBIG (-3)  /* larger than a fixed-size scratch array         */
 
 /* ---------------------------------------------------------------- operators */
 
@@ -232,6 +233,21 @@ int pde_bc_apply_f32(const float *in, float *out, int64_t n, const pde_bc_op *op
  * GridBoundary used by the fused steps. */
 int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t nops, const float *t,
                              float *staging, void *stream);
+
+/* pde_bc_apply_inplace_f32 with no staging argument and no allocation: a
+ * table of more than 4096 ops stages its values in a file-scope device array
+ * of PDE_BC_SCRATCH_FLOATS floats (4 MiB) that each GPU the library runs on
+ * holds until the process ends (its address is looked up once per device, on
+ * the first call with that device current).  Same values as
+ * pde_bc_apply_inplace_f32.  PDE_ETOOBIG when nops > PDE_BC_SCRATCH_FLOATS.
+ * Every call on a GPU shares that GPU's array (each slot is written before it
+ * is read, so nothing needs clearing): calls that overlap on one GPU, on
+ * different streams or threads, race.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+#define PDE_BC_SCRATCH_FLOATS (1 << 20)
+int pde_bc_apply_inplace_noalloc_f32(float *u, const pde_bc_op *ops, int64_t nops,
+                                     const float *t, void *stream);
 
 /* Periodic ghost fill, in place: for each eligible axis ascending, ghost layer
  * i < g takes interior layer i + E and ghost layer i >= P - g takes i - E

@@ -43,6 +43,7 @@ SIGNATURES = {
     "pde_scale_f32": [_P, _P, _I64, _F, _P],
     "pde_bc_apply_batch_inplace_f32": [_P, _P, _I64, _P, _P, _P],
     "pde_batch_bc_capacity": [],
+    "pde_bc_apply_inplace_noalloc_f32": [_P, _P, _I64, _P, _P],
     "pde_heat_multistep_f32": [_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I32, _P],
     "pde_heat_batch_step_f32": [_P, _P, _P, _I64, _P, _I64, _I32, _I32, _I64, _P],
     "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],

@@ -9,6 +9,13 @@
 // launch here reproduces either semantics exactly.
 #include "pde_common.hpp"
 
+#include <mutex>
+
+// Staging of pde_bc_apply_inplace_noalloc_f32: one copy per GPU, for the
+// process's lifetime.  In host code this name is only a handle for
+// hipGetSymbolAddress: kernels receive the per-device address it returns.
+static __device__ __align__(256) float scratch[PDE_BC_SCRATCH_FLOATS];
+
 namespace pde {
 
 __global__ __launch_bounds__(256) void bc_apply_kernel(const float *__restrict__ in,
@@ -115,6 +122,46 @@ extern "C" int pde_bc_apply_inplace_f32(float *u, const pde_bc_op *ops, int64_t 
     if (st) return st;
     hipLaunchKernelGGL(bc_scatter_kernel, grid, block, 0, s, u, ops, nops, staging);
     return launch_status();
+}
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+float *scratch_ptr[kScratchDevices] = {};
+
+// The current device's copy of `scratch` (looked up once per device).
+int scratch_of_current_device(float **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kScratchDevices) return PDE_EINVAL;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return PDE_EINVAL;
+        scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return PDE_OK;
+}
+
+}  // namespace
+
+extern "C" int pde_bc_apply_inplace_noalloc_f32(float *u, const pde_bc_op *ops, int64_t nops,
+                                                const float *t, void *stream)
+{
+    if (!u || nops < 0 || (nops > 0 && !ops)) return PDE_EINVAL;
+    if (nops > PDE_BC_SCRATCH_FLOATS) return PDE_ETOOBIG;
+    if (nops <= 4096)      // one workgroup stages in LDS: no scratch needed
+        return pde_bc_apply_inplace_f32(u, ops, nops, t, nullptr, stream);
+    float *staging = nullptr;
+    int st = scratch_of_current_device(&staging);
+    if (st) return st;
+    return pde_bc_apply_inplace_f32(u, ops, nops, t, staging, stream);
 }
 
 extern "C" int pde_bc_apply_chunked_inplace_f32(float *u, const pde_bc_op *ops,

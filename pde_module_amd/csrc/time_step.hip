@@ -260,6 +260,8 @@ extern "C" const char *pde_status_string(int status)
     if (status == PDE_OK) return "ok";
     if (status == PDE_EINVAL) return "invalid argument";
     if (status == PDE_ESHAPE) return "grid too small for the ghost width or launch limits";
+    if (status == PDE_ETOOBIG)
+        return "table larger than the allocation-free entry point's scratch array";
     if (status > 0) return hipGetErrorString((hipError_t)status);
     return "unknown status";
 }

@@ -120,3 +120,14 @@ def test_ring_plan_record_layout_and_op_count_check():
         ctypes.c_void_p(96), ctypes.c_void_p(112), ctypes.c_void_p(128), None, 34, 34, 1, 1,
         1.0, -2.0, 1.0, 1.0, 0.1, step.ctypes.data, step[1:].ctypes.data, None, None, save,
         len(ops), rec.ctypes.data, None) == -1
+
+
+def test_allocation_free_boundary_refuses_oversize_tables_on_the_host():
+    """pde_bc_apply_inplace_noalloc_f32 returns PDE_ETOOBIG (-3) for a table
+    larger than its fixed scratch array before touching the device."""
+    lib = N.load()
+    fake = ctypes.c_void_p(16)
+    assert lib.pde_bc_apply_inplace_noalloc_f32(fake, fake, (1 << 20) + 1, None, None) == -3
+    assert b"scratch" in lib.pde_status_string(-3)
+    assert lib.pde_bc_apply_inplace_noalloc_f32(None, fake, 10, None, None) == -1
+    assert lib.pde_bc_apply_inplace_noalloc_f32(fake, None, 10, None, None) == -1
