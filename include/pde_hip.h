@@ -689,6 +689,35 @@ int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, float *grhs
                                        int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
                                        float h2, float inv, void *stream);
 
+/* Most sweeps one multi-sweep Jacobi launch runs. */
+#define PDE_JACOBI_MAX_SWEEPS 48
+
+/* `sweeps` (1 .. PDE_JACOBI_MAX_SWEEPS) folded Neumann-0 sweeps
+ * (pde_jacobi_neumann0_f32) in one launch, for `batch` instances of a 2D
+ * grid stored one after another (arguments as pde_jacobi_neumann0_x2_f32):
+ * each workgroup loads a tile of the grid into LDS (the node's output
+ * rectangle grown by `sweeps` nodes on every side inside the grid), sweeps
+ * it there and stores the rectangle.  out = the interior after `sweeps`
+ * sweeps from p (out != p, != rhs; out's ghosts are not written).  The values
+ * of `sweeps` launches of pde_jacobi_neumann0_f32 bit for bit.  PDE_ESHAPE
+ * for grids that are not 2D (or have an interior side under 2) or batch >
+ * 65535.  No reference counterpart (the projection cavity's pressure solve). */
+int pde_jacobi_neumann0_multisweep_f32(const float *p, const float *rhs, float *out,
+                                       int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                       int32_t ghost, float h2, float inv, int32_t sweeps,
+                                       void *stream);
+
+/* `sweeps` transposed sweeps (pde_jacobi_neumann0_adjoint_f32) in one
+ * launch, tiled as pde_jacobi_neumann0_multisweep_f32: gp = the transpose
+ * applied `sweeps` times to g (0 on ghosts), grhs += each sweep's
+ * -h2 * (inv * input) on interior nodes, in sweep order.  g, gp and grhs
+ * distinct.  The values of `sweeps` launches of
+ * pde_jacobi_neumann0_adjoint_f32 bit for bit.  No reference counterpart. */
+int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, float *gp, float *grhs,
+                                               int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                               int32_t ghost, float h2, float inv,
+                                               int32_t sweeps, void *stream);
+
 /* The projection cavity's predictor on a 2D vec2 field in one launch:
  * us = u2 + dt * (-(grad u2) u2 + nu * lap(u2)) with the values of
  * pde_grad_f32 (alpha 1) + pde_matvec_f32 + pde_laplacian_f32 (alpha nu) +

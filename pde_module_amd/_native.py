@@ -44,6 +44,10 @@ SIGNATURES = {
     "pde_bc_apply_batch_inplace_f32": [_P, _P, _I64, _P, _P, _P],
     "pde_batch_bc_capacity": [],
     "pde_bc_apply_inplace_noalloc_f32": [_P, _P, _I64, _P, _P],
+    "pde_jacobi_neumann0_multisweep_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F,
+                                           _I32, _P],
+    "pde_jacobi_neumann0_adjoint_multisweep_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32,
+                                                   _F, _F, _I32, _P],
     "pde_heat_multistep_f32": [_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I32, _P],
     "pde_heat_batch_step_f32": [_P, _P, _P, _I64, _P, _I64, _I32, _I32, _I64, _P],
     "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],
@@ -136,6 +140,7 @@ HEAT_TILE_OP_DTYPE = np.dtype([("ldst", "<i4"), ("lsrc", "<i4"), ("op", "<i4"),
 assert HEAT_TILE_OP_DTYPE.itemsize == 16
 HEAT_TILE_PITCH = HEAT_TILE_ROWS = 128
 HEAT_TILE_MAXOPS = 2048
+JACOBI_MAX_SWEEPS = 48        # PDE_JACOBI_MAX_SWEEPS
 
 # struct pde_ring_piece / pde_ring_entry / pde_ring_plan
 RING_PIECE_DTYPE = np.dtype([("r0", "<i4"), ("r1", "<i4"), ("c0", "<i4"), ("c1", "<i4"),

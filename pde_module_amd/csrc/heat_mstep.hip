@@ -16,38 +16,16 @@
 //
 // The host builds the tiles (outputs covering every cell of every grid
 // once) and each tile's op list, and checks that every op's source cell
-// stays valid as long as its destination does; see ring/multistep planning
-// in pde_module_amd/multistep.py.
-#include "pde_common.hpp"
+// stays valid as long as its destination does (pde_module_amd/multistep.py).
+#include "mstep.hpp"
 
 namespace pde {
 
-constexpr int HT_PITCH = PDE_HEAT_TILE_PITCH;   // LDS row pitch (cells): region width cap
-constexpr int HT_ROWS = PDE_HEAT_TILE_ROWS;     // region height cap
-constexpr int HT_NT = 1024;                     // threads per workgroup
-constexpr int HT_RPB = 4;                       // rows per thread band (at most)
+constexpr int HT_PITCH = MS_PITCH;
+constexpr int HT_ROWS = MS_ROWS;
+constexpr int HT_NT = MS_NT;
+constexpr int HT_RPB = MS_RPB;
 constexpr int HT_OPT = PDE_HEAT_TILE_MAXOPS / HT_NT;   // ops per thread
-
-// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l receives
-// lane l-1's value (from_left) or lane l+1's (from_right); 0 past the ends.
-__device__ __forceinline__ float ht_from_left(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-
-__device__ __forceinline__ float ht_from_right(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-
-struct f4 {
-    float x, y, z, w;
-};
-
-__device__ __forceinline__ float f4el(const f4 &v, int i)
-{
-    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
-}
 
 // The heat update of one cell from its u2 neighbours, in heat_rows' order
 // (tile2d.hpp acc5: rows x-1, x, x+1, then columns k-1, k, k+1).
@@ -64,12 +42,7 @@ __device__ __forceinline__ float ht_cell(float up, float mid, float dn, float zl
     return mid + dt * (alpha * acc);
 }
 
-// Thread layout: a band of `lanes` consecutive threads (a power of two, at
-// most 32, at least the region's float4 columns) covers one band of up to
-// HT_RPB rows, thread c4 of the band the 4 columns 4*c4 .. 4*c4+3; bands never
-// straddle a wave, so the DPP shifts bring each chunk its left and right
-// neighbours (the band's first and last chunks only border region-edge
-// cells, which are never stepped).
+// Thread layout and DPP neighbours: mstep.hpp.
 __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     const float *__restrict__ u, float *__restrict__ un, const pde_batch_grid *__restrict__ grids,
     int64_t ngrids, const pde_heat_tile *__restrict__ tiles,
@@ -81,12 +54,7 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     if (T.grid < 0 || T.grid >= ngrids) return;
     const pde_batch_grid G = grids[T.grid];
     const int rh = T.rh, rw = T.rw;
-    // a malformed record (never produced by the host planner) cannot reach
-    // outside the LDS array or the grid
-    if (rh < 1 || rh > HT_ROWS || rw < 1 || rw > HT_PITCH || T.r0 < 0 || T.c0 < 0 ||
-        T.r0 + rh > G.p1 || T.c0 + rw > G.p2 || T.op_end - T.op_beg > PDE_HEAT_TILE_MAXOPS ||
-        T.op_end < T.op_beg)
-        return;
+    if (!ms_tile_ok(T, G)) return;
     const int tid = threadIdx.x;
     const int64_t P2 = G.p2;
     const float *ug = u + G.offset;
@@ -131,14 +99,9 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     }
 
     // this thread's chunk: band, rows, columns and which of its cells step
-    const int ncol4 = (rw + 3) >> 2;
-    int lanes = 1;
-    while (lanes < ncol4) lanes <<= 1;
-    const int bands = HT_NT / lanes;
-    const int rpb = (rh + bands - 1) / bands;   // <= HT_RPB (the planner's regions fit)
-    const int band = tid / lanes, c4 = tid - band * lanes;
-    const int lc0 = 4 * c4, lr0 = band * rpb;
-    const bool active = c4 < ncol4 && lr0 < rh && rpb <= HT_RPB;
+    const MsLayout L = ms_layout(tid, rh, rw);
+    const int rpb = L.rpb, c4 = L.c4, lc0 = L.lc0, lr0 = L.lr0;
+    const bool active = L.active;
     const int g = G.ghost;
     bool cm[4];
 #pragma unroll
@@ -153,13 +116,7 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
         rm[h] = active && h < rpb && lr > 0 && lr < rh - 1 && x >= g && x < G.p1 - g;
     }
     const float wm = G.wm, wc = G.wc, wp = G.wp, alpha = G.alpha, dt = G.dt;
-    const f4 *A4 = reinterpret_cast<const f4 *>(A);
-    f4 *W4 = reinterpret_cast<f4 *>(A);
-    auto row4 = [&](int lr) {
-        // rows outside the region read a clamped row (results masked)
-        const int r = lr < 0 ? 0 : (lr > HT_ROWS - 1 ? HT_ROWS - 1 : lr);
-        return A4[r * (HT_PITCH / 4) + (c4 < HT_PITCH / 4 ? c4 : HT_PITCH / 4 - 1)];
-    };
+    auto row4 = [&](int lr) { return ms_row4(A, lr, c4); };
     __syncthreads();
     for (int s = 0; s < steps; ++s) {
         if (nop > 0) {
@@ -175,14 +132,14 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
                 if (odst[j] >= 0) A[odst[j]] = val[j];
             __syncthreads();
         }
-        f4 nv[HT_RPB];
-        f4 up = row4(lr0 - 1), mid = row4(lr0);
+        ms_f4 nv[HT_RPB];
+        ms_f4 up = row4(lr0 - 1), mid = row4(lr0);
 #pragma unroll
         for (int h = 0; h < HT_RPB; ++h) {
-            const f4 dn = row4(lr0 + h + 1);
+            const ms_f4 dn = row4(lr0 + h + 1);
             // every lane shifts (no divergence around the DPP moves)
-            const float left = ht_from_left(mid.w), right = ht_from_right(mid.x);
-            f4 o = mid;
+            const float left = ms_from_left(mid.w), right = ms_from_right(mid.x);
+            ms_f4 o = mid;
             if (rm[h]) {
                 const float n0 = ht_cell(up.x, mid.x, dn.x, left, mid.y, wm, wc, wp, alpha, dt);
                 const float n1 = ht_cell(up.y, mid.y, dn.y, mid.x, mid.z, wm, wc, wp, alpha, dt);
@@ -201,7 +158,7 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
         if (active) {
 #pragma unroll
             for (int h = 0; h < HT_RPB; ++h)
-                if (h < rpb && lr0 + h < rh) W4[(lr0 + h) * (HT_PITCH / 4) + c4] = nv[h];
+                if (h < rpb && lr0 + h < rh) ms_store4(A, lr0 + h, c4, nv[h]);
         }
         __syncthreads();
     }

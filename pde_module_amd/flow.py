@@ -35,6 +35,9 @@ from .deprecated import GridBoundary as LegacyGridBoundary
 from .FDM.module.grad import central_weights
 from .FDM.module.laplacian import laplacian_weights
 
+# the most Jacobi sweeps one multi-sweep launch runs
+MAX_SWEEPS_PER_LAUNCH = N.JACOBI_MAX_SWEEPS
+
 
 def _grid(x):
     return tuple(int(s) for s in x.shape[:3])
@@ -130,17 +133,26 @@ class _KineticEnergy(torch.autograd.Function):
 
 class ProjectionCavity:
     """``ProjectionCavity(n, nu=0.01, U=1.0, dt=None, jacobi_sweeps=50,
-    ghost_cells=1, device='cuda')``: n x n nodes on the unit square, dx =
-    1/(n-1), dt = min(0.5 dx / U, 0.25 dx^2 / nu) by default."""
+    ghost_cells=1, device='cuda', sweeps_per_launch=10)``: n x n nodes on the
+    unit square, dx = 1/(n-1), dt = min(0.5 dx / U, 0.25 dx^2 / nu) by
+    default.  The Jacobi sweeps (and their transposes) run ``sweeps_per_launch``
+    (at most 48) per launch on LDS-resident tiles
+    (pde_jacobi_neumann0_multisweep_f32; 2 or fewer keeps the two-sweep
+    row-tile kernels); the values do not depend on it."""
 
     def __init__(self, n, nu=0.01, U=1.0, dt=None, jacobi_sweeps=50, ghost_cells=1,
-                 device="cuda"):
+                 device="cuda", sweeps_per_launch=10):
         self.n, self.g = int(n), int(ghost_cells)
         self.dx = 1.0 / (self.n - 1)
         self.nu, self.U = float(nu), float(U)
         self.dt = float(dt) if dt is not None else min(0.5 * self.dx / self.U,
                                                        0.25 * self.dx ** 2 / self.nu)
         self.sweeps = int(jacobi_sweeps)
+        self.sweeps_per_launch = max(1, int(sweeps_per_launch))
+        if self.sweeps_per_launch > MAX_SWEEPS_PER_LAUNCH:
+            raise ValueError(f"sweeps_per_launch {self.sweeps_per_launch} exceeds "
+                             f"{MAX_SWEEPS_PER_LAUNCH}")
+        self._multi = self.sweeps_per_launch > 2
         self.device = torch.device(device)
         P = self.n + 2 * self.g
         self.shape = (P, P, 1)
@@ -202,23 +214,36 @@ class ProjectionCavity:
         N.call("pde_divergence_f32", us2.data_ptr(), rhs.data_ptr(), *grid, 1, g, *self.wc,
                np.float32(1.0 / self.dt), s)
 
+    def _sweep_plan(self):
+        """Sweeps per launch, in order (the last launch takes the remainder)."""
+        k = self.sweeps_per_launch
+        if not self._multi:
+            return [2] * (self.sweeps // 2) + [1] * (self.sweeps % 2)
+        return [k] * (self.sweeps // k) + ([self.sweeps % k] if self.sweeps % k else [])
+
     def _sweeps(self, p, rhs, p_out, pf, p0, batch):
         """The Jacobi sweeps from p into p_out (``batch`` instances stored one
         after another).  Neumann-0 walls are folded into each sweep
         (pde_jacobi_neumann0_f32: the same values as applying p_bc, then
-        sweeping), two sweeps per launch (pde_jacobi_neumann0_x2_f32, the
-        same values again) and a single one when the count is odd.  Launches
-        alternate between p_out and pf so that none writes what it reads and
-        the last one writes p_out."""
+        sweeping).  With sweeps_per_launch k > 2, k sweeps per launch on
+        LDS-resident tiles (pde_jacobi_neumann0_multisweep_f32) and the
+        remainder in one more; otherwise two per launch
+        (pde_jacobi_neumann0_x2_f32) and a single one when the count is odd.
+        Every form gives the same values.  Launches alternate between p_out
+        and pf so that none writes what it reads and the last one writes
+        p_out."""
         s = N.stream_of(p)
-        plan = [2] * (self.sweeps // 2) + [1] * (self.sweeps % 2)
+        plan = self._sweep_plan()
         cur = p
         for j, m in enumerate(plan):
             dst = p_out if (len(plan) - 1 - j) % 2 == 0 else pf
             if dst is cur:                       # p_out is p on the first launch
                 N.call("pde_copy_f32", cur.data_ptr(), p0.data_ptr(), cur.numel(), s)
                 cur = p0
-            if m == 2:
+            if self._multi and m > 1:
+                N.call("pde_jacobi_neumann0_multisweep_f32", cur.data_ptr(), rhs.data_ptr(),
+                       dst.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, m, s)
+            elif m == 2:
                 N.call("pde_jacobi_neumann0_x2_f32", cur.data_ptr(), rhs.data_ptr(),
                        dst.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
             elif batch == 1:
@@ -310,7 +335,7 @@ class ProjectionCavity:
         return gu, gp
 
     def _sweep_launches(self):
-        return self.sweeps // 2 + self.sweeps % 2
+        return len(self._sweep_plan())
 
     def _sweep_buffers(self, gp, b0, b1):
         """(first, spare) for the transposed sweeps so that the last launch
@@ -330,11 +355,27 @@ class ProjectionCavity:
 
     def _adj_sweeps(self, g, spare, grhs, batch):
         """The sweeps p <- J(B_p(p), rhs), transposed last to first, from g
-        (``batch`` instances one after another), two per launch; rhs's
-        gradient accumulates in grhs.  Returns the buffer (g or spare) holding
-        p's gradient."""
+        (``batch`` instances one after another), launched as the forward's
+        launches are; rhs's gradient accumulates in grhs.  Returns the buffer
+        (g or spare) holding p's gradient."""
         s = N.stream_of(g)
         cur, nxt = g, spare
+        if self._multi:
+            # the forward's launches transposed, last to first (each launch's
+            # sweeps are one linear map; transposes compose in reverse)
+            for m in reversed(self._sweep_plan()):
+                if m == 1:
+                    name = ("pde_jacobi_neumann0_adjoint_f32" if batch == 1 else
+                            "pde_jacobi_neumann0_adjoint_batch_f32")
+                    extra = () if batch == 1 else (batch,)
+                    N.call(name, cur.data_ptr(), nxt.data_ptr(), grhs.data_ptr(), *extra,
+                           *self.shape, self.g, self.h2, self.inv, s)
+                else:
+                    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", cur.data_ptr(),
+                           nxt.data_ptr(), grhs.data_ptr(), batch, *self.shape, self.g,
+                           self.h2, self.inv, m, s)
+                cur, nxt = nxt, cur
+            return cur
         for _ in range(self.sweeps // 2):          # two transposed sweeps per launch
             N.call("pde_jacobi_neumann0_adjoint_x2_f32", cur.data_ptr(), nxt.data_ptr(),
                    grhs.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
@@ -527,9 +568,10 @@ class ProjectionCavityBatch:
     """``batch`` cavities of one grid and one set of parameters stepped
     together (one per seed): u (batch, X, Y, 1, 2) and p (batch, X, Y, 1, 1)
     with the instances one after another.  The Jacobi sweeps of every instance
-    run as one launch per sweep (pde_jacobi_neumann0_batch_f32); the predictor
-    and the correction run per instance on shared work buffers.  Each
-    instance gets the same bits as ``ProjectionCavity.step`` alone."""
+    run in the same launches (several sweeps per launch, as in
+    ``ProjectionCavity``); the predictor and the correction run per instance
+    on shared work buffers.  Each instance gets the same bits as
+    ``ProjectionCavity.step`` alone."""
 
     def __init__(self, n, batch, **kw):
         self.cav = ProjectionCavity(n, **kw)
@@ -561,7 +603,7 @@ class ProjectionCavityBatch:
 
     def step_adjoint(self, u, gu_out, gp_out, gu, gp):
         """Transpose of ``step`` for every instance (the instances' transposed
-        sweeps as one launch per sweep); same arguments as
+        sweeps in the same launches); same arguments as
         ``ProjectionCavity.step_adjoint`` with a leading batch axis."""
         cav = self.cav
         cav.check_fields((u, gu_out, gu), (gp_out, gp), (self.batch,))

@@ -77,6 +77,25 @@ def test_bc_op_record_layout():
                                               dtype=N.BC_OP_DTYPE)), [7])
 
 
+def test_multisweep_jacobi_arguments_need_no_gpu():
+    """The multi-sweep Jacobi entry points check the sweep count (1 ..
+    PDE_JACOBI_MAX_SWEEPS), aliasing and the 2D shape before any launch."""
+    text = open(os.path.join(ROOT, "include", "pde_hip.h")).read()
+    assert int(re.search(r"#define PDE_JACOBI_MAX_SWEEPS (\d+)", text).group(1)) == \
+        N.JACOBI_MAX_SWEEPS
+    lib = N.load()
+    a, b, c = (ctypes.c_void_p(16 * i) for i in (1, 2, 3))   # never dereferenced
+    for fn in (lib.pde_jacobi_neumann0_multisweep_f32,
+               lib.pde_jacobi_neumann0_adjoint_multisweep_f32):
+        assert fn(a, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, 0, None) == -1
+        assert fn(a, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, N.JACOBI_MAX_SWEEPS + 1, None) == -1
+        assert fn(a, b, a, 1, 66, 66, 1, 1, 1.0, 0.25, 4, None) == -1     # out aliases in
+        assert fn(a, b, c, 0, 66, 66, 1, 1, 1.0, 0.25, 4, None) == -1
+        assert fn(a, b, c, 1, 66, 66, 66, 1, 1.0, 0.25, 4, None) == -2     # 3D
+        assert fn(a, b, c, 1, 66, 3, 1, 1, 1.0, 0.25, 4, None) == -2       # 1 interior column
+        assert fn(a, b, c, 70000, 66, 66, 1, 1, 1.0, 0.25, 4, None) == -2
+
+
 def test_argument_errors_need_no_gpu():
     """Entry points validate arguments before touching the device."""
     lib = N.load()

@@ -537,3 +537,48 @@ def test_cavity_adjoint_correct_and_div_fused_equal_chains(dev, shape, g):
     N.call("pde_cavity_adjoint_div_f32", gr.data_ptr(), gu.data_ptr(), got_u.data_ptr(), *shape, g,
            *wc, alpha, st)
     assert same_bits(got_u.cpu().numpy(), want_u.cpu().numpy())
+
+
+@pytest.mark.parametrize("n,sweeps,spl,batch", [
+    (34, 23, 10, 1),     # one whole-grid tile, 3 sweeps left for the last launch
+    (200, 21, 10, 1),    # 64-cell regions cut from the grid, 1 sweep left
+    (130, 50, 7, 2),     # a batch of cut grids, 1 sweep left
+    (18, 12, 5, 3),      # a batch of whole-grid tiles, a 2-sweep last launch
+    (300, 40, 40, 1),    # 96-cell regions around 40-cell halos
+])
+def test_multisweep_launches_equal_two_sweep_kernels(dev, n, sweeps, spl, batch):
+    """Jacobi sweeps run ``spl`` per launch on LDS-resident tiles against the
+    two-sweep row kernels: the same bits for three forward steps and for the
+    transposed step (gradients of u, p and, through them, of rhs)."""
+    from pde_module_amd.flow import ProjectionCavityBatch
+    print(f"multisweep n={n} sweeps={sweeps} per_launch={spl} batch={batch}", flush=True)
+    kw = dict(jacobi_sweeps=sweeps, dt=1e-3, device=dev)
+    a = ProjectionCavityBatch(n, batch, sweeps_per_launch=spl, **kw)
+    b = ProjectionCavityBatch(n, batch, sweeps_per_launch=2, **kw)
+    rng = np.random.default_rng(n + sweeps)
+    shp = (batch,) + a.cav.shape
+    u0 = torch.from_numpy(np.stack([ns_cavity_u0(s, n) * 50.0 for s in range(batch)])).to(dev)
+    p0 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, shp + (1,)).astype(np.float32)).to(dev)
+    ua, pa = a.run(u0, p0, 3, use_graph=False)
+    ub, pb = b.run(u0, p0, 3, use_graph=False)
+    assert same_bits(ua.cpu().numpy(), ub.cpu().numpy())
+    assert same_bits(pa.cpu().numpy(), pb.cpu().numpy())
+    gu_out = torch.from_numpy(rng.standard_normal(shp + (2,)).astype(np.float32)).to(dev)
+    gp_out = torch.from_numpy(rng.standard_normal(shp + (1,)).astype(np.float32)).to(dev)
+    got = []
+    for c in (a, b):
+        gu, gp = torch.zeros_like(gu_out), torch.zeros_like(gp_out)
+        c.step_adjoint(ua, gu_out, gp_out, gu, gp)
+        got.append((gu.cpu().numpy(), gp.cpu().numpy()))
+    assert same_bits(got[0][0], got[1][0])
+    assert same_bits(got[0][1], got[1][1])
+    if batch == 1:
+        # the single-instance class takes the same launches
+        one = ProjectionCavity(n, sweeps_per_launch=spl, **kw)
+        u1, p1 = one.run(u0[0], p0[0], 3, use_graph=True)
+        assert same_bits(u1.cpu().numpy(), ub[0].cpu().numpy())
+        assert same_bits(p1.cpu().numpy(), pb[0].cpu().numpy())
+        gu, gp = torch.zeros_like(gu_out[0]), torch.zeros_like(gp_out[0])
+        one.step_adjoint(ua[0], gu_out[0], gp_out[0], gu, gp)
+        assert same_bits(gu.cpu().numpy(), got[1][0][0])
+        assert same_bits(gp.cpu().numpy(), got[1][1][0])
