@@ -216,7 +216,7 @@ __device__ __forceinline__ JmTile jm_tile(const JmPlan &P, int t)
 template <bool ADJ, int PITCH, int NT, int RPB>
 __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
     const float *__restrict__ in, float *__restrict__ out, const float *__restrict__ rhs,
-    float *__restrict__ grhs, JmPlan PL, float h2, float inv, int sweeps)
+    float *__restrict__ grhs, JmPlan PL, float h2, float inv, int sweeps, int accumulate)
 {
     using C = JmCfg<PITCH, NT, RPB>;
     __shared__ __align__(16) float S[2 * C::BUF];
@@ -245,7 +245,8 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
     };
     // Stage the region row by row (consecutive threads, consecutive
     // columns): the field into buffer 0, h2 * rhs (forward) or the running
-    // rhs gradient (transpose) of interior nodes into buffer 1, 0 elsewhere.
+    // rhs gradient (transpose; 0 when not accumulating) of interior nodes
+    // into buffer 1, 0 elsewhere.
     // Every load is issued before the first is used; each reads a valid
     // address (out-of-region ones the grid's first node, then discarded).
     constexpr int PER = C::ROWS * PITCH / NT;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
             const bool in_region = lr < rh && lc < rw;
             const int64_t a = in_region ? (int64_t)x * P2 + k : 0;
             f[j] = ig[a];
-            q[j] = rg[in_region && interior(x, k) ? a : 0];
+            q[j] = ADJ && !accumulate ? 0.0f : rg[in_region && interior(x, k) ? a : 0];
         }
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
@@ -349,7 +350,7 @@ constexpr int JM_NT64 = 512;
 template <bool ADJ>
 static int jm_launch(const float *in, float *out, const float *rhs, float *grhs, int64_t batch,
                      int64_t nx, int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
-                     int32_t sweeps, hipStream_t stream)
+                     int32_t sweeps, int32_t accumulate, hipStream_t stream)
 {
     if (sweeps < 1 || sweeps > PDE_JACOBI_MAX_SWEEPS || batch < 1) return PDE_EINVAL;
     if (batch > 65535) return PDE_ESHAPE;
@@ -378,10 +379,11 @@ static int jm_launch(const float *in, float *out, const float *rhs, float *grhs,
     const dim3 grid((unsigned)tiles, (unsigned)batch);
     if (small)
         hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 64, JM_NT64, 64 * 16 / JM_NT64>), grid,
-                           dim3(JM_NT64), 0, stream, in, out, rhs, grhs, PL, h2, inv, sweeps);
+                           dim3(JM_NT64), 0, stream, in, out, rhs, grhs, PL, h2, inv, sweeps,
+                           accumulate);
     else
         hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 128, 1024, 4>), grid, dim3(1024), 0,
-                           stream, in, out, rhs, grhs, PL, h2, inv, sweeps);
+                           stream, in, out, rhs, grhs, PL, h2, inv, sweeps, accumulate);
     return launch_status();
 }
 
@@ -395,7 +397,7 @@ extern "C" int pde_jacobi_neumann0_multisweep_f32(const float *p, const float *r
                                                   int32_t sweeps, void *stream)
 {
     if (!p || !rhs || !out || p == out || rhs == out) return PDE_EINVAL;
-    return jm_launch<false>(p, out, rhs, nullptr, batch, nx, ny, nz, ghost, h2, inv, sweeps,
+    return jm_launch<false>(p, out, rhs, nullptr, batch, nx, ny, nz, ghost, h2, inv, sweeps, 0,
                             as_stream(stream));
 }
 
@@ -403,9 +405,10 @@ extern "C" int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, float 
                                                           int64_t batch, int64_t nx, int64_t ny,
                                                           int64_t nz, int32_t ghost, float h2,
                                                           float inv, int32_t sweeps,
-                                                          void *stream)
+                                                          int32_t accumulate, void *stream)
 {
     if (!g || !gp || !grhs || g == gp || g == grhs || gp == grhs) return PDE_EINVAL;
+    if (accumulate != 0 && accumulate != 1) return PDE_EINVAL;
     return jm_launch<true>(g, gp, nullptr, grhs, batch, nx, ny, nz, ghost, h2, inv, sweeps,
-                           as_stream(stream));
+                           accumulate, as_stream(stream));
 }

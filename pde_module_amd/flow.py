@@ -215,11 +215,13 @@ class ProjectionCavity:
                np.float32(1.0 / self.dt), s)
 
     def _sweep_plan(self):
-        """Sweeps per launch, in order (the last launch takes the remainder)."""
+        """Sweeps per launch, in order.  With several per launch the
+        remainder goes first, so that the transposed launches (last to
+        first) start with a full multi-sweep one."""
         k = self.sweeps_per_launch
         if not self._multi:
             return [2] * (self.sweeps // 2) + [1] * (self.sweeps % 2)
-        return [k] * (self.sweeps // k) + ([self.sweeps % k] if self.sweeps % k else [])
+        return ([self.sweeps % k] if self.sweeps % k else []) + [k] * (self.sweeps // k)
 
     def _sweeps(self, p, rhs, p_out, pf, p0, batch):
         """The Jacobi sweeps from p into p_out (``batch`` instances stored one
@@ -329,7 +331,6 @@ class ProjectionCavity:
         self.prepare_adjoint(u, gp)
         first, spare = self._sweep_buffers(gp, a["gpb"], a["gpa"])
         self._adj_correct(gu_out, gp_out, first)
-        a["grhs"].zero_()
         self._adj_sweeps(first, spare, a["grhs"], 1)          # ends in gp
         self._adj_predict(u, gu_out, a["grhs"], gu)
         return gu, gp
@@ -356,14 +357,18 @@ class ProjectionCavity:
     def _adj_sweeps(self, g, spare, grhs, batch):
         """The sweeps p <- J(B_p(p), rhs), transposed last to first, from g
         (``batch`` instances one after another), launched as the forward's
-        launches are; rhs's gradient accumulates in grhs.  Returns the buffer
-        (g or spare) holding p's gradient."""
+        launches are; rhs's gradient gathers in grhs from 0 (the first
+        multi-sweep launch starts it, else it is cleared first; its ghosts
+        stay 0).  Returns the buffer (g or spare) holding p's gradient."""
         s = N.stream_of(g)
         cur, nxt = g, spare
         if self._multi:
             # the forward's launches transposed, last to first (each launch's
             # sweeps are one linear map; transposes compose in reverse)
-            for m in reversed(self._sweep_plan()):
+            plan = self._sweep_plan()
+            if not plan or plan[-1] == 1:       # no multi-sweep launch comes first
+                grhs.zero_()
+            for j, m in enumerate(reversed(plan)):
                 if m == 1:
                     name = ("pde_jacobi_neumann0_adjoint_f32" if batch == 1 else
                             "pde_jacobi_neumann0_adjoint_batch_f32")
@@ -373,9 +378,10 @@ class ProjectionCavity:
                 else:
                     N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", cur.data_ptr(),
                            nxt.data_ptr(), grhs.data_ptr(), batch, *self.shape, self.g,
-                           self.h2, self.inv, m, s)
+                           self.h2, self.inv, m, int(j > 0 or plan[-1] == 1), s)
                 cur, nxt = nxt, cur
             return cur
+        grhs.zero_()
         for _ in range(self.sweeps // 2):          # two transposed sweeps per launch
             N.call("pde_jacobi_neumann0_adjoint_x2_f32", cur.data_ptr(), nxt.data_ptr(),
                    grhs.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, s)
@@ -611,7 +617,6 @@ class ProjectionCavityBatch:
         first, spare = cav._sweep_buffers(gp, self.gpb, self.gpa)
         for i in range(self.batch):
             cav._adj_correct(gu_out[i], gp_out[i], first[i])
-        self.grhs.zero_()
         cav._adj_sweeps(first, spare, self.grhs, self.batch)     # ends in gp
         for i in range(self.batch):
             cav._adj_predict(u[i], gu_out[i], self.grhs[i], gu[i])
