@@ -62,6 +62,10 @@ def parse():
     p.add_argument("--only-heat", action="store_true",
                    help="profiling runs: only the heat workloads (512-grid ensemble, 256^2 "
                         "loop), their JSON on stdout")
+    p.add_argument("--only-cavity", action="store_true",
+                   help="profiling runs: only the 1024^2 projection cavity, its JSON on stdout")
+    p.add_argument("--cavity-steps", type=int, default=2000,
+                   help="steps of the single-cavity rollouts (profiling runs shorten it)")
     return p.parse_args()
 
 
@@ -582,6 +586,10 @@ def main():
     if args.only_heat:
         print(json.dumps({"heat_ensemble_512": heat_ensemble_timing(dev),
                           "heat_256": heat_config_timing(dev)}))
+        return
+    if args.only_cavity:
+        print(json.dumps({"projection_cavity_1024":
+                          projection_cavity_timing(dev, steps=args.cavity_steps)}))
         return
     gen = torch.Generator(device=dev).manual_seed(0)
     s = torch.rand((P, P, P, 1), device=dev, generator=gen) * 2 - 1
