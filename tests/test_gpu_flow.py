@@ -587,3 +587,30 @@ def test_multisweep_launches_equal_two_sweep_kernels(dev, n, sweeps, spl, batch)
         one.step_adjoint(ua[0], gu_out[0], gp_out[0], gu, gp)
         assert same_bits(gu.cpu().numpy(), got[0][0][0])
         assert same_bits(gp.cpu().numpy(), got[0][1][0])
+
+
+def test_multisweep_at_the_cavity_workload_size(dev):
+    """The 1024^2 cavity (50 sweeps, 10 per launch: 576 tiles of 64 x 64 nodes)
+    against the two-sweep row kernels: two forward steps and a transposed step,
+    bit for bit."""
+    n = 1024
+    print(f"multisweep at {n}^2", flush=True)
+    kw = dict(jacobi_sweeps=50, device=dev)
+    a = ProjectionCavity(n, sweeps_per_launch=10, **kw)
+    b = ProjectionCavity(n, sweeps_per_launch=2, **kw)
+    u0 = torch.from_numpy(ns_cavity_u0(0, n)).to(dev)
+    rng = np.random.default_rng(5)
+    p0 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, a.shape + (1,)).astype(np.float32)).to(dev)
+    ua, pa = a.run(u0, p0, 2, use_graph=False)
+    ub, pb = b.run(u0, p0, 2, use_graph=False)
+    assert torch.equal(ua.view(torch.int32), ub.view(torch.int32))
+    assert torch.equal(pa.view(torch.int32), pb.view(torch.int32))
+    gu_out = torch.from_numpy(rng.standard_normal(a.shape + (2,)).astype(np.float32)).to(dev)
+    gp_out = torch.from_numpy(rng.standard_normal(a.shape + (1,)).astype(np.float32)).to(dev)
+    got = []
+    for c in (a, b):
+        gu, gp = torch.zeros_like(gu_out), torch.zeros_like(gp_out)
+        c.step_adjoint(ua, gu_out, gp_out, gu, gp)
+        got.append((gu, gp))
+    assert torch.equal(got[0][0].view(torch.int32), got[1][0].view(torch.int32))
+    assert torch.equal(got[0][1].view(torch.int32), got[1][1].view(torch.int32))
