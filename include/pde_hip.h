@@ -85,6 +85,14 @@ int pde_divergence_f32(const float *in, float *out, int64_t nx, int64_t ny, int6
                        int32_t nrows, int32_t ghost, float w_minus, float w_center,
                        float w_plus, float alpha, void *stream);
 
+/* pde_divergence_f32 for `batch` (<= 65535) instances of a 1D or 2D grid
+ * stored one after another, one launch: instance b's values are those of
+ * pde_divergence_f32 on its own arrays.  PDE_ESHAPE for 3D grids.  No
+ * reference counterpart (the projection cavity's seeds stepped together). */
+int pde_divergence_batch_f32(const float *in, float *out, int64_t batch, int64_t nx, int64_t ny,
+                             int64_t nz, int32_t nrows, int32_t ghost, float w_minus,
+                             float w_center, float w_plus, float alpha, void *stream);
+
 /* Fused 3D operator pass over one scalar field s and one vec(3) field v on the
  * same padded grid: lap = alpha_lap * Laplacian(s), grad = alpha_grad * Grad(s)
  * (vec3 per node), div = alpha_div * Divergence(v), each bit-identical to the
@@ -769,6 +777,33 @@ int pde_cavity_adjoint_correct_f32(const float *gu_out, float *gpf, int64_t nx, 
 int pde_cavity_adjoint_div_f32(const float *grhs, const float *gu_out, float *gus2, int64_t nx,
                                int64_t ny, int64_t nz, int32_t ghost, float grad_wm,
                                float grad_wc, float grad_wp, float alpha, void *stream);
+
+/* The five cavity kernels above for `batch` (<= 65535) instances of the 2D
+ * grid stored one after another (every array holds batch instances; the
+ * batch goes before nx): one launch each, instance b's values those of the
+ * single-instance entry point on its own arrays.  No reference counterpart
+ * (the projection cavity's seeds stepped together). */
+int pde_cavity_predict_batch_f32(const float *u2, float *us, int64_t batch, int64_t nx,
+                                 int64_t ny, int64_t nz, int32_t ghost, float lap_wm,
+                                 float lap_wc, float lap_wp, float grad_wm, float grad_wc,
+                                 float grad_wp, float nu, float dt, void *stream);
+int pde_cavity_correct_batch_f32(const float *p, const float *us2, float *u_out, int64_t batch,
+                                 int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                 float grad_wm, float grad_wc, float grad_wp, float dt,
+                                 void *stream);
+int pde_cavity_adjoint_predict_batch_f32(const float *gus, const float *u2, float *gu2,
+                                         int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                         int32_t ghost, float lap_wm, float lap_wc,
+                                         float lap_wp, float grad_wm, float grad_wc,
+                                         float grad_wp, float nu, float dt, void *stream);
+int pde_cavity_adjoint_correct_batch_f32(const float *gu_out, float *gpf, int64_t batch,
+                                         int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                         float grad_wm, float grad_wc, float grad_wp, float dt,
+                                         void *stream);
+int pde_cavity_adjoint_div_batch_f32(const float *grhs, const float *gu_out, float *gus2,
+                                     int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                     int32_t ghost, float grad_wm, float grad_wc,
+                                     float grad_wp, float alpha, void *stream);
 
 /* out[n][j] = sum_i m[n][i][j] * g[n][i], i ascending: the transpose of
  * pde_matvec_f32 in its vector argument (the convection term's gradient). */

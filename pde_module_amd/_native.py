@@ -23,6 +23,8 @@ SIGNATURES = {
     "pde_laplacian_periodic_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_grad_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
     "pde_divergence_f32": [_P, _P, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F, _P],
+    "pde_divergence_batch_f32": [_P, _P, _I64, _I64, _I64, _I64, _I32, _I32, _F, _F, _F, _F,
+                                 _P],
     "pde_fused_lap_grad_div_f32": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32,
                                    _F, _F, _F, _F, _F, _F, _F, _F, _F, _P],
     "pde_forward_euler_f32": [_P, _P, _P, _I64, _F, _P],
@@ -88,6 +90,16 @@ SIGNATURES = {
     "pde_jacobi_neumann0_adjoint_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_neumann0_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],
     "pde_jacobi_neumann0_x2_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _P],
+    "pde_cavity_predict_batch_f32": [_P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F,
+                                     _F, _F, _F, _P],
+    "pde_cavity_correct_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _F, _F,
+                                     _P],
+    "pde_cavity_adjoint_correct_batch_f32": [_P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _F,
+                                             _F, _P],
+    "pde_cavity_adjoint_div_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F, _F,
+                                         _F, _P],
+    "pde_cavity_adjoint_predict_batch_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F,
+                                             _F, _F, _F, _F, _F, _F, _P],
     "pde_cavity_predict_f32": [_P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _F, _F, _F, _F,
                                _P],
     "pde_cavity_correct_f32": [_P, _P, _P, _I64, _I64, _I64, _I32, _F, _F, _F, _F, _P],

@@ -547,6 +547,9 @@ __global__ __launch_bounds__(256) void cavity_predict_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
+    const int64_t nb = (int64_t)blockIdx.z * P1 * P2;   // nodes of the instances before
+    u2 += 2 * nb;
+    us += 2 * nb;
     const int64_t S1 = 2 * P2, S2 = 2;
     for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
         const int64_t e = (x * P2 + k) * 2;
@@ -603,6 +606,10 @@ __global__ __launch_bounds__(256) void cavity_correct_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
+    const int64_t nb = (int64_t)blockIdx.z * P1 * P2;
+    p += nb;
+    us2 += 2 * nb;
+    u_out += 2 * nb;
     for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
         const int64_t n = x * P2 + k;
         float d[2] = {0.0f, 0.0f};
@@ -644,6 +651,10 @@ __global__ __launch_bounds__(256) void cavity_adj_predict_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
+    const int64_t nb = (int64_t)blockIdx.z * P1 * P2;
+    gus += 2 * nb;
+    u2 += 2 * nb;
+    gu2 += 2 * nb;
     typedef float f2 __attribute__((ext_vector_type(2)));
     const f2 *G2 = reinterpret_cast<const f2 *>(gus);
     const f2 *U2 = reinterpret_cast<const f2 *>(u2);
@@ -729,6 +740,9 @@ __global__ __launch_bounds__(256) void cavity_adj_correct_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
+    const int64_t nb = (int64_t)blockIdx.z * P1 * P2;
+    gu_out += 2 * nb;
+    gpf += nb;
     auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
     auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
     auto gg = [&](int64_t node, int t) { return ndt * gu_out[2 * node + t]; };
@@ -758,6 +772,10 @@ __global__ __launch_bounds__(256) void cavity_adj_div_kernel(
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= P2) return;
+    const int64_t nb = (int64_t)blockIdx.z * P1 * P2;
+    grhs += nb;
+    gu_out += 2 * nb;
+    gus2 += 2 * nb;
     auto inx = [&](int64_t v) { return v >= g && v < P1 - g; };
     auto ink = [&](int64_t v) { return v >= g && v < P2 - g; };
     for (int64_t x = blockIdx.y; x < P1; x += gridDim.y) {
@@ -976,20 +994,60 @@ extern "C" int pde_jacobi_neumann0_adjoint_x2_f32(const float *gout, float *gp, 
                                      as_stream(stream));
 }
 
+// Grid of the cavity kernels for `batch` instances of a 2D grid stored one
+// after another (thread per column, rows on grid.y, instances on grid.z).
+static int cavity_grid(int64_t batch, int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                       bool mirror, Geom *G, dim3 *grid)
+{
+    if (batch < 1) return PDE_EINVAL;
+    if (batch > 65535) return PDE_ESHAPE;
+    int st = make_geom(nx, ny, nz, ghost, G);
+    if (st) return st;
+    if (G->ndim != 2) return PDE_ESHAPE;
+    if (mirror && (G->E[1] < 2 || G->E[2] < 2)) return PDE_ESHAPE;   // a wall node's mirror
+    *grid = dim3(clamp_grid((G->P[2] + 255) / 256, 0x7fffffff), clamp_grid(G->P[1], 65535),
+                 (unsigned)batch);
+    return PDE_OK;
+}
+
+extern "C" int pde_cavity_predict_batch_f32(const float *u2, float *us, int64_t batch,
+                                            int64_t nx, int64_t ny, int64_t nz, int32_t ghost,
+                                            float lap_wm, float lap_wc, float lap_wp,
+                                            float grad_wm, float grad_wc, float grad_wp, float nu,
+                                            float dt, void *stream)
+{
+    if (!u2 || !us || u2 == us) return PDE_EINVAL;
+    Geom G;
+    dim3 grid;
+    int st = cavity_grid(batch, nx, ny, nz, ghost, false, &G, &grid);
+    if (st) return st;
+    hipLaunchKernelGGL(cavity_predict_kernel, grid, dim3(256), 0, as_stream(stream), u2, us,
+                       G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm, grad_wc,
+                       grad_wp, nu, dt);
+    return launch_status();
+}
+
 extern "C" int pde_cavity_predict_f32(const float *u2, float *us, int64_t nx, int64_t ny,
                                       int64_t nz, int32_t ghost, float lap_wm, float lap_wc,
                                       float lap_wp, float grad_wm, float grad_wc, float grad_wp,
                                       float nu, float dt, void *stream)
 {
-    if (!u2 || !us || u2 == us) return PDE_EINVAL;
+    return pde_cavity_predict_batch_f32(u2, us, 1, nx, ny, nz, ghost, lap_wm, lap_wc, lap_wp,
+                                        grad_wm, grad_wc, grad_wp, nu, dt, stream);
+}
+
+extern "C" int pde_cavity_correct_batch_f32(const float *p, const float *us2, float *u_out,
+                                            int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                                            int32_t ghost, float grad_wm, float grad_wc,
+                                            float grad_wp, float dt, void *stream)
+{
+    if (!p || !us2 || !u_out || u_out == p) return PDE_EINVAL;
     Geom G;
-    int st = make_geom(nx, ny, nz, ghost, &G);
+    dim3 grid;
+    int st = cavity_grid(batch, nx, ny, nz, ghost, true, &G, &grid);
     if (st) return st;
-    if (G.ndim != 2) return PDE_ESHAPE;
-    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
-    hipLaunchKernelGGL(cavity_predict_kernel, grid, dim3(256), 0, as_stream(stream), u2, us,
-                       G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm, grad_wc,
-                       grad_wp, nu, dt);
+    hipLaunchKernelGGL(cavity_correct_kernel, grid, dim3(256), 0, as_stream(stream), p, us2,
+                       u_out, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
     return launch_status();
 }
 
@@ -998,15 +1056,26 @@ extern "C" int pde_cavity_correct_f32(const float *p, const float *us2, float *u
                                       float grad_wm, float grad_wc, float grad_wp, float dt,
                                       void *stream)
 {
-    if (!p || !us2 || !u_out || u_out == p) return PDE_EINVAL;
+    return pde_cavity_correct_batch_f32(p, us2, u_out, 1, nx, ny, nz, ghost, grad_wm, grad_wc,
+                                        grad_wp, dt, stream);
+}
+
+extern "C" int pde_cavity_adjoint_predict_batch_f32(const float *gus, const float *u2,
+                                                    float *gu2, int64_t batch, int64_t nx,
+                                                    int64_t ny, int64_t nz, int32_t ghost,
+                                                    float lap_wm, float lap_wc, float lap_wp,
+                                                    float grad_wm, float grad_wc,
+                                                    float grad_wp, float nu, float dt,
+                                                    void *stream)
+{
+    if (!gus || !u2 || !gu2 || gu2 == gus || gu2 == u2) return PDE_EINVAL;
     Geom G;
-    int st = make_geom(nx, ny, nz, ghost, &G);
+    dim3 grid;
+    int st = cavity_grid(batch, nx, ny, nz, ghost, false, &G, &grid);
     if (st) return st;
-    if (G.ndim != 2) return PDE_ESHAPE;
-    if (G.E[1] < 2 || G.E[2] < 2) return PDE_ESHAPE;    // a wall node's mirror must be interior
-    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
-    hipLaunchKernelGGL(cavity_correct_kernel, grid, dim3(256), 0, as_stream(stream), p, us2,
-                       u_out, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
+    hipLaunchKernelGGL(cavity_adj_predict_kernel, grid, dim3(256), 0, as_stream(stream), gus, u2,
+                       gu2, G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm,
+                       grad_wc, grad_wp, nu, dt);
     return launch_status();
 }
 
@@ -1016,15 +1085,24 @@ extern "C" int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2,
                                               float grad_wm, float grad_wc, float grad_wp,
                                               float nu, float dt, void *stream)
 {
-    if (!gus || !u2 || !gu2 || gu2 == gus || gu2 == u2) return PDE_EINVAL;
+    return pde_cavity_adjoint_predict_batch_f32(gus, u2, gu2, 1, nx, ny, nz, ghost, lap_wm,
+                                                lap_wc, lap_wp, grad_wm, grad_wc, grad_wp, nu,
+                                                dt, stream);
+}
+
+extern "C" int pde_cavity_adjoint_correct_batch_f32(const float *gu_out, float *gpf,
+                                                    int64_t batch, int64_t nx, int64_t ny,
+                                                    int64_t nz, int32_t ghost, float grad_wm,
+                                                    float grad_wc, float grad_wp, float dt,
+                                                    void *stream)
+{
+    if (!gu_out || !gpf) return PDE_EINVAL;
     Geom G;
-    int st = make_geom(nx, ny, nz, ghost, &G);
+    dim3 grid;
+    int st = cavity_grid(batch, nx, ny, nz, ghost, false, &G, &grid);
     if (st) return st;
-    if (G.ndim != 2) return PDE_ESHAPE;
-    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
-    hipLaunchKernelGGL(cavity_adj_predict_kernel, grid, dim3(256), 0, as_stream(stream), gus, u2,
-                       gu2, G.P[1], G.P[2], (int64_t)ghost, lap_wm, lap_wc, lap_wp, grad_wm,
-                       grad_wc, grad_wp, nu, dt);
+    hipLaunchKernelGGL(cavity_adj_correct_kernel, grid, dim3(256), 0, as_stream(stream), gu_out,
+                       gpf, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
     return launch_status();
 }
 
@@ -1033,14 +1111,23 @@ extern "C" int pde_cavity_adjoint_correct_f32(const float *gu_out, float *gpf, i
                                               float grad_wm, float grad_wc, float grad_wp,
                                               float dt, void *stream)
 {
-    if (!gu_out || !gpf) return PDE_EINVAL;
+    return pde_cavity_adjoint_correct_batch_f32(gu_out, gpf, 1, nx, ny, nz, ghost, grad_wm,
+                                                grad_wc, grad_wp, dt, stream);
+}
+
+extern "C" int pde_cavity_adjoint_div_batch_f32(const float *grhs, const float *gu_out,
+                                                float *gus2, int64_t batch, int64_t nx,
+                                                int64_t ny, int64_t nz, int32_t ghost,
+                                                float grad_wm, float grad_wc, float grad_wp,
+                                                float alpha, void *stream)
+{
+    if (!grhs || !gu_out || !gus2 || gus2 == grhs) return PDE_EINVAL;
     Geom G;
-    int st = make_geom(nx, ny, nz, ghost, &G);
+    dim3 grid;
+    int st = cavity_grid(batch, nx, ny, nz, ghost, false, &G, &grid);
     if (st) return st;
-    if (G.ndim != 2) return PDE_ESHAPE;
-    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
-    hipLaunchKernelGGL(cavity_adj_correct_kernel, grid, dim3(256), 0, as_stream(stream), gu_out,
-                       gpf, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, -dt);
+    hipLaunchKernelGGL(cavity_adj_div_kernel, grid, dim3(256), 0, as_stream(stream), grhs, gu_out,
+                       gus2, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, alpha);
     return launch_status();
 }
 
@@ -1049,15 +1136,8 @@ extern "C" int pde_cavity_adjoint_div_f32(const float *grhs, const float *gu_out
                                           float grad_wm, float grad_wc, float grad_wp,
                                           float alpha, void *stream)
 {
-    if (!grhs || !gu_out || !gus2 || gus2 == grhs) return PDE_EINVAL;
-    Geom G;
-    int st = make_geom(nx, ny, nz, ghost, &G);
-    if (st) return st;
-    if (G.ndim != 2) return PDE_ESHAPE;
-    const dim3 grid(clamp_grid((G.P[2] + 255) / 256, 0x7fffffff), clamp_grid(G.P[1], 65535));
-    hipLaunchKernelGGL(cavity_adj_div_kernel, grid, dim3(256), 0, as_stream(stream), grhs, gu_out,
-                       gus2, G.P[1], G.P[2], (int64_t)ghost, grad_wm, grad_wc, grad_wp, alpha);
-    return launch_status();
+    return pde_cavity_adjoint_div_batch_f32(grhs, gu_out, gus2, 1, nx, ny, nz, ghost, grad_wm,
+                                            grad_wc, grad_wp, alpha, stream);
 }
 
 extern "C" int pde_matvec_t_f32(const float *m, const float *g, float *out, int64_t nodes,

@@ -936,6 +936,29 @@ extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64
     return launch_status();
 }
 
+extern "C" int pde_divergence_batch_f32(const float *in, float *out, int64_t batch, int64_t nx,
+                                        int64_t ny, int64_t nz, int32_t nrows, int32_t ghost,
+                                        float wm, float wc, float wp, float alpha, void *stream)
+{
+    if (!in || !out || nrows < 1 || batch < 1) return PDE_EINVAL;
+    if (batch > 65535) return PDE_ESHAPE;
+    Geom G;
+    int st = make_geom(nx, ny, nz, ghost, &G);
+    if (st) return st;
+    if (G.ndim == 0 || G.ndim == 3) return PDE_ESHAPE;
+    // canonical axis 0 is not a grid axis of a 1D or 2D grid: it walks the
+    // instances (stride = one instance's nodes)
+    RowGeom rg = row_geom(G, nrows);
+    rg.E0 = batch;
+    dim3 grid = row_grid(rg), block(256);
+    hipStream_t s = as_stream(stream);
+    if (G.ndim == 1)
+        hipLaunchKernelGGL(div_generic<1>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha);
+    else
+        hipLaunchKernelGGL(div_generic<2>, grid, block, 0, s, in, out, rg, wm, wc, wp, alpha);
+    return launch_status();
+}
+
 extern "C" int pde_fused_lap_grad_div_f32(const float *s_, const float *v, float *lap,
                                           float *grad, float *div, int64_t nx, int64_t ny,
                                           int64_t nz, int32_t ghost, float lm, float lc,

@@ -32,6 +32,7 @@ import torch.utils.checkpoint as cp
 from . import _native as N
 from . import autograd as A
 from .deprecated import GridBoundary as LegacyGridBoundary
+from .FDM.boundary.device_table import DeviceBoundaryTable
 from .FDM.module.grad import central_weights
 from .FDM.module.laplacian import laplacian_weights
 
@@ -41,6 +42,22 @@ MAX_SWEEPS_PER_LAUNCH = N.JACOBI_MAX_SWEEPS
 
 def _grid(x):
     return tuple(int(s) for s in x.shape[:3])
+
+
+def _repeat_table(table, batch, stride, device):
+    """A boundary table for ``batch`` instances of one grid stored one after
+    another: ``table``'s ops repeated per instance, their scalar indices
+    shifted by the instance's offset (``stride`` scalars per instance), in
+    the same order within each instance."""
+    ops = table.host_ops
+    reps = []
+    for i in range(batch):
+        o = ops.copy()
+        o["dst"] += i * stride
+        rd = o["kind"] != N.BC_SET
+        o["src"][rd] += i * stride
+        reps.append(o)
+    return DeviceBoundaryTable(np.concatenate(reps) if reps else ops, device)
 
 
 class _MatVec(torch.autograd.Function):
@@ -153,6 +170,7 @@ class ProjectionCavity:
             raise ValueError(f"sweeps_per_launch {self.sweeps_per_launch} exceeds "
                              f"{MAX_SWEEPS_PER_LAUNCH}")
         self._multi = self.sweeps_per_launch > 2
+        self._btabs = {}
         self.device = torch.device(device)
         P = self.n + 2 * self.g
         self.shape = (P, P, 1)
@@ -194,25 +212,40 @@ class ProjectionCavity:
         self._correct(p_out, b["us2"], u_out)
         return u_out, p_out
 
-    def _predict(self, u, us2, rhs, u_scratch=False):
-        """u -> the boundary-fixed predictor us2 and the pressure source rhs.
-        With u_scratch (u is the step's output, overwritten at its end) the
-        velocity boundary is applied to u in place instead of into a copy."""
-        b, g, s = self.b, self.g, N.stream_of(u)
+    def _tables(self, batch):
+        """(velocity, pressure) boundary tables for ``batch`` instances stored
+        one after another: the grid's own, or for a batch their ops repeated
+        per instance (cached).  The grid's tables must exist."""
+        if batch == 1:
+            return self.u_bc.table, self.p_bc.table
+        if batch not in self._btabs:
+            n = self.shape[0] * self.shape[1] * self.shape[2]
+            self._btabs[batch] = (_repeat_table(self.u_bc.table, batch, 2 * n, self.device),
+                                  _repeat_table(self.p_bc.table, batch, n, self.device))
+        return self._btabs[batch]
+
+    def _predict(self, u, us2, rhs, u_scratch=False, batch=1, u2buf=None):
+        """u -> the boundary-fixed predictor us2 and the pressure source rhs
+        (``batch`` instances one after another, one launch per stage).  With
+        u_scratch (u is the step's output, overwritten at its end) the velocity
+        boundary is applied to u in place instead of into a copy (u2buf, the
+        grid's own work buffer by default)."""
+        g, s = self.g, N.stream_of(u)
         grid = self.shape
-        self.u_bc._ensure_table(u)
+        self.u_bc._ensure_table(u[0] if u.dim() == 5 else u)     # one instance's shape
+        ut, _ = self._tables(batch)
         if u_scratch:
-            u2 = self.u_bc.table.apply_inplace(u, None)
+            u2 = ut.apply_inplace(u, None)
         else:
-            u2 = self.u_bc.table.apply(u, b["u2"], None)
+            u2 = ut.apply(u, self.b["u2"] if u2buf is None else u2buf, None)
         # grad, convection product, diffusion, F and the Euler update in one
         # launch (the values of the five separate kernels), then the boundary
         # on that result in place
-        N.call("pde_cavity_predict_f32", u2.data_ptr(), us2.data_ptr(), *grid, g,
+        N.call("pde_cavity_predict_batch_f32", u2.data_ptr(), us2.data_ptr(), batch, *grid, g,
                *self.wl, *self.wc, np.float32(self.nu), np.float32(self.dt), s)
-        self.u_bc.table.apply_inplace(us2, None)
-        N.call("pde_divergence_f32", us2.data_ptr(), rhs.data_ptr(), *grid, 1, g, *self.wc,
-               np.float32(1.0 / self.dt), s)
+        ut.apply_inplace(us2, None)
+        N.call("pde_divergence_batch_f32", us2.data_ptr(), rhs.data_ptr(), batch, *grid, 1, g,
+               *self.wc, np.float32(1.0 / self.dt), s)
 
     def _sweep_plan(self):
         """Sweeps per launch, in order.  With several per launch the
@@ -258,12 +291,13 @@ class ProjectionCavity:
         if cur is not p_out:
             N.call("pde_copy_f32", cur.data_ptr(), p_out.data_ptr(), p_out.numel(), s)
 
-    def _correct(self, p_out, us2, u_out):
+    def _correct(self, p_out, us2, u_out, batch=1):
         """u_out = us2 - dt * grad(B_p(p_out))."""
         # B_p (Neumann 0 on every face) folded into the gradient: one launch
         # with the values of boundary + grad + Euler
-        N.call("pde_cavity_correct_f32", p_out.data_ptr(), us2.data_ptr(), u_out.data_ptr(),
-               *self.shape, self.g, *self.wc, np.float32(self.dt), N.stream_of(p_out))
+        N.call("pde_cavity_correct_batch_f32", p_out.data_ptr(), us2.data_ptr(),
+               u_out.data_ptr(), batch, *self.shape, self.g, *self.wc, np.float32(self.dt),
+               N.stream_of(p_out))
 
     def run(self, u0, p0, steps, use_graph=True):
         """``steps`` steps from (u0, p0); returns the final (u, p)."""
@@ -343,14 +377,16 @@ class ProjectionCavity:
         writes gp: the launches alternate first -> spare -> first ..."""
         return (gp, b1) if self._sweep_launches() % 2 == 0 else (b0, gp)
 
-    def _adj_correct(self, gu_out, gp_out, gpb):
+    def _adj_correct(self, gu_out, gp_out, gpb, batch=1, gpf=None):
         """Transpose of the correction u_out = us2 - dt * grad(B_p(p_out)) in
-        p_out: gpb = B_p^T(grad^T(-dt * gu_out)) + gp_out."""
-        a, g, s = self.ga, self.g, N.stream_of(gu_out)
-        N.call("pde_cavity_adjoint_correct_f32", gu_out.data_ptr(), a["gpf"].data_ptr(),
+        p_out: gpb = B_p^T(grad^T(-dt * gu_out)) + gp_out (gpf: work buffer,
+        the grid's own by default)."""
+        g, s = self.g, N.stream_of(gu_out)
+        gpf = self.ga["gpf"] if gpf is None else gpf
+        N.call("pde_cavity_adjoint_correct_batch_f32", gu_out.data_ptr(), gpf.data_ptr(), batch,
                *self.shape, g, *self.wc, np.float32(self.dt), s)
-        self.p_bc.table.adjoint_inplace(a["gpf"])
-        N.call("pde_lincomb_f32", a["gpf"].data_ptr(), gp_out.data_ptr(), None, None,
+        self._tables(batch)[1].adjoint_inplace(gpf)
+        N.call("pde_lincomb_f32", gpf.data_ptr(), gp_out.data_ptr(), None, None,
                np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
                gpb.data_ptr(), gpb.numel(), s)
 
@@ -396,25 +432,29 @@ class ProjectionCavity:
             cur, nxt = nxt, cur
         return cur
 
-    def _adj_predict(self, u, gu_out, grhs, gu):
+    def _adj_predict(self, u, gu_out, grhs, gu, batch=1, u2buf=None, gus2=None):
         """Transpose of the predictor and the correction in us2: from rhs's
-        gradient grhs and gu_out, u's gradient gu (u is the step's input)."""
-        b, a, g, s = self.b, self.ga, self.g, N.stream_of(u)
+        gradient grhs and gu_out, u's gradient gu (u is the step's input;
+        u2buf and gus2: work buffers, the grid's own by default)."""
+        g, s = self.g, N.stream_of(u)
         grid = self.shape
+        u2buf = self.b["u2"] if u2buf is None else u2buf
+        gus2 = self.ga["gus2"] if gus2 is None else gus2
+        ut, _ = self._tables(batch)
         # the forward value this transpose reads: u2 = B_u(u)
-        self.u_bc.table.apply(u, b["u2"], None)
+        ut.apply(u, u2buf, None)
         # rhs = div(us2) / dt; u_out = us2 - ...; us2 = B_u(us)
-        N.call("pde_cavity_adjoint_div_f32", grhs.data_ptr(), gu_out.data_ptr(),
-               a["gus2"].data_ptr(), *grid, g, *self.wc, np.float32(1.0 / self.dt), s)
-        self.u_bc.table.adjoint_inplace(a["gus2"])               # now the gradient of us
+        N.call("pde_cavity_adjoint_div_batch_f32", grhs.data_ptr(), gu_out.data_ptr(),
+               gus2.data_ptr(), batch, *grid, g, *self.wc, np.float32(1.0 / self.dt), s)
+        ut.adjoint_inplace(gus2)                                  # now the gradient of us
         # us = u2 + dt * F, F = -(grad u2) u2 + nu lap(u2): the transposes of
         # the Euler update, F, the diffusion, the convection product and grad
         # in one launch (the values of the separate transposed kernels)
-        N.call("pde_cavity_adjoint_predict_f32", a["gus2"].data_ptr(), b["u2"].data_ptr(),
-               gu.data_ptr(), *grid, g, *self.wl, *self.wc, np.float32(self.nu),
+        N.call("pde_cavity_adjoint_predict_batch_f32", gus2.data_ptr(), u2buf.data_ptr(),
+               gu.data_ptr(), batch, *grid, g, *self.wl, *self.wc, np.float32(self.nu),
                np.float32(self.dt), s)
         # u2 = B_u(u)
-        self.u_bc.table.adjoint_inplace(gu)
+        ut.adjoint_inplace(gu)
 
     def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
         """J(u_N) = 1/2 sum |u_N|^2 dx^2 (interior), differentiable in u0 and
@@ -573,11 +613,11 @@ class _CavityRolloutFn(torch.autograd.Function):
 class ProjectionCavityBatch:
     """``batch`` cavities of one grid and one set of parameters stepped
     together (one per seed): u (batch, X, Y, 1, 2) and p (batch, X, Y, 1, 1)
-    with the instances one after another.  The Jacobi sweeps of every instance
-    run in the same launches (several sweeps per launch, as in
-    ``ProjectionCavity``); the predictor and the correction run per instance
-    on shared work buffers.  Each instance gets the same bits as
-    ``ProjectionCavity.step`` alone."""
+    with the instances one after another.  Every stage runs as one launch for
+    all instances: the predictor, divergence, Jacobi sweeps (several per
+    launch, as in ``ProjectionCavity``), correction, and the boundaries (the
+    grid's tables repeated per instance).  Each instance gets the same bits
+    as ``ProjectionCavity.step`` alone."""
 
     def __init__(self, n, batch, **kw):
         self.cav = ProjectionCavity(n, **kw)
@@ -586,40 +626,42 @@ class ProjectionCavityBatch:
             raise ValueError("batch must be >= 1")
         z = lambda *e: torch.zeros((self.batch,) + self.cav.shape + e, dtype=torch.float32,
                                    device=self.cav.device)
-        self.us2, self.rhs, self.pf, self.p0 = z(2), z(1), z(1), z(1)
+        self.us2, self.rhs, self.pf, self.p0, self.u2 = z(2), z(1), z(1), z(1), z(2)
+        # the boundary tables, built here (never inside a graph capture)
+        self.cav.u_bc._ensure_table(self.us2[0])
+        self.cav.p_bc._ensure_table(self.rhs[0])
+        self.cav._tables(self.batch)
 
     def step(self, u, p, u_out, p_out):
         """One step of every instance from (u, p) into (u_out, p_out) (which
         may be u / p)."""
-        cav = self.cav
-        cav.check_fields((u, u_out), (p, p_out), (self.batch,))
-        for i in range(self.batch):
-            cav._predict(u[i], self.us2[i], self.rhs[i], u_scratch=u is u_out)
-        cav._sweeps(p, self.rhs, p_out, self.pf, self.p0, self.batch)
-        for i in range(self.batch):
-            cav._correct(p_out[i], self.us2[i], u_out[i])
+        cav, B = self.cav, self.batch
+        cav.check_fields((u, u_out), (p, p_out), (B,))
+        cav._predict(u, self.us2, self.rhs, u_scratch=u is u_out, batch=B, u2buf=self.u2)
+        cav._sweeps(p, self.rhs, p_out, self.pf, self.p0, B)
+        cav._correct(p_out, self.us2, u_out, batch=B)
         return u_out, p_out
 
     def prepare_adjoint(self, like_u, like_p):
         self.cav.prepare_adjoint(like_u[0], like_p[0])
+        for t in self.cav._tables(self.batch):
+            t.prepare_adjoint()
         if getattr(self, "gpb", None) is None:
             z = lambda *e: torch.zeros((self.batch,) + self.cav.shape + e, dtype=torch.float32,
                                        device=self.cav.device)
-            self.gpb, self.gpa, self.grhs = z(1), z(1), z(1)
+            self.gpb, self.gpa, self.grhs, self.gpf, self.gus2 = z(1), z(1), z(1), z(1), z(2)
 
     def step_adjoint(self, u, gu_out, gp_out, gu, gp):
-        """Transpose of ``step`` for every instance (the instances' transposed
-        sweeps in the same launches); same arguments as
-        ``ProjectionCavity.step_adjoint`` with a leading batch axis."""
-        cav = self.cav
-        cav.check_fields((u, gu_out, gu), (gp_out, gp), (self.batch,))
+        """Transpose of ``step`` for every instance (each stage one launch for
+        all instances); same arguments as ``ProjectionCavity.step_adjoint``
+        with a leading batch axis."""
+        cav, B = self.cav, self.batch
+        cav.check_fields((u, gu_out, gu), (gp_out, gp), (B,))
         self.prepare_adjoint(u, gp)
         first, spare = cav._sweep_buffers(gp, self.gpb, self.gpa)
-        for i in range(self.batch):
-            cav._adj_correct(gu_out[i], gp_out[i], first[i])
-        cav._adj_sweeps(first, spare, self.grhs, self.batch)     # ends in gp
-        for i in range(self.batch):
-            cav._adj_predict(u[i], gu_out[i], self.grhs[i], gu[i])
+        cav._adj_correct(gu_out, gp_out, first, batch=B, gpf=self.gpf)
+        cav._adj_sweeps(first, spare, self.grhs, B)     # ends in gp
+        cav._adj_predict(u, gu_out, self.grhs, gu, batch=B, u2buf=self.u2, gus2=self.gus2)
         return gu, gp
 
     def kinetic_energy_rollout(self, u0, p0, steps, checkpoint_every=10, use_graph=False):
