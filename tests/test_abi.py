@@ -98,6 +98,22 @@ def test_multisweep_jacobi_arguments_need_no_gpu():
         assert fn(a, b, c, 70000, 66, 66, 1, 1, 1.0, 0.25, 4, None) == -2
 
 
+def test_batched_cavity_arguments_need_no_gpu():
+    """Batch counts outside 1..65535 and 3D grids are refused before a launch."""
+    lib = N.load()
+    a, b, c = (ctypes.c_void_p(16 * i) for i in (1, 2, 3))   # never dereferenced
+    f = lib.pde_divergence_batch_f32
+    assert f(a, b, 0, 1, 40, 40, 1, 1, 1.0, 0.0, 1.0, 1.0, None) == -1
+    assert f(a, b, 70000, 1, 40, 40, 1, 1, 1.0, 0.0, 1.0, 1.0, None) == -2
+    assert f(a, b, 2, 40, 40, 40, 1, 1, 1.0, 0.0, 1.0, 1.0, None) == -2
+    assert lib.pde_cavity_correct_batch_f32(a, b, c, 0, 40, 40, 1, 1, 1.0, 0.0, 1.0, 0.1,
+                                            None) == -1
+    assert lib.pde_cavity_predict_batch_f32(a, b, 70000, 40, 40, 1, 1, 1.0, -2.0, 1.0, 1.0,
+                                            0.0, 1.0, 0.01, 0.1, None) == -2
+    assert lib.pde_cavity_adjoint_div_batch_f32(a, b, a, 2, 40, 40, 1, 1, 1.0, 0.0, 1.0, 1.0,
+                                                None) == -1       # gus2 aliases grhs
+
+
 def test_argument_errors_need_no_gpu():
     """Entry points validate arguments before touching the device."""
     lib = N.load()
