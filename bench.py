@@ -665,6 +665,17 @@ def main():
     t_copy = event_time(lambda: NV.call("pde_copy_f32", s.data_ptr(), out_l.data_ptr(),
                                         s.numel(), NV.stream_of(s)), reps // 2)
     copy_gbs = 2 * 4 * s.numel() / t_copy / 1e9
+    # ForwardEuler over every padded scalar (read cur and k, write out: 12 B)
+    euler = pm.ForwardEuler()
+    t_euler = event_time(lambda: euler(s, out_l, 1e-3), reps // 2)
+    # functional GridBoundary, Dirichlet 0 on every face: the module call (copy
+    # of the field + the boundary writes) and the same writes in place
+    gbc = pm.GridBoundary(s, 1.0 / N, G)
+    gbc.dirichlet_BC("ALL", 0.0)
+    t_bc = event_time(lambda: gbc(s), reps // 2)
+    bc_field = s.clone()
+    t_bc_inplace = event_time(lambda: gbc.apply_inplace(bc_field), reps // 2)
+    del bc_field
 
     progress(f"operators timed: step {t_step_kernel * 1e6:.1f} us, drop-in laplacian "
              f"{t_lap * 1e6:.1f} us")
@@ -724,6 +735,19 @@ def main():
                                    "traffic": traffic.get("ops3d_band<7, false, 4>") if traffic else None,
                                    "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},
             "copy_GBs": round(copy_gbs, 1),
+            "forward_euler": {"us": round(t_euler * 1e6, 1),
+                              "GBs": round(12 * s.numel() / t_euler / 1e9, 1),
+                              "frac": round(12 * s.numel() / t_euler / 1e9 / HBM_PEAK_GBS, 4),
+                              "note": "out = cur + dt*k over the 514^3 padded array, 12 B per "
+                                      "scalar"},
+            "grid_boundary_dirichlet_all": {
+                "module_call_us": round(t_bc * 1e6, 1),
+                "module_call_GBs": round(8 * s.numel() / t_bc / 1e9, 1),
+                "in_place_us": round(t_bc_inplace * 1e6, 1),
+                "ops": int(gbc.table.nops),
+                "note": "functional GridBoundary, Dirichlet 0 on every face of 514^3: the "
+                        "module call copies the field (8 B per scalar) and applies the writes; "
+                        "in place only the O(surface) writes"},
             "wave2048_fwd_vs_fwd_adjoint": wave,
             "heat_ensemble_512": ensemble,
             "projection_cavity_1024": cavity,

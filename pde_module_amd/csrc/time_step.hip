@@ -28,21 +28,22 @@ __global__ __launch_bounds__(256) void copy_kernel(const float *__restrict__ in,
 }
 
 // ForwardEuler (pm/time_step/forwardEuler.py:79): new = cur + dt * k.  No
-// __restrict__: out may alias cur or k (element-wise, same index).
+// __restrict__: out may alias cur or k (element-wise, same index).  Every
+// byte is read or written once: streaming loads and stores, as in the copy.
 __global__ __launch_bounds__(256) void euler_kernel(const float *cur, const float *k, float *out,
                                                     int64_t n, float dt)
 {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t i = q * 4;
     if (i + 3 < n && aligned16(cur) && aligned16(k) && aligned16(out)) {
-        const float4 a = reinterpret_cast<const float4 *>(cur)[q];
-        const float4 b = reinterpret_cast<const float4 *>(k)[q];
-        float4 r;
+        const cp_f4 a = __builtin_nontemporal_load(reinterpret_cast<const cp_f4 *>(cur) + q);
+        const cp_f4 b = __builtin_nontemporal_load(reinterpret_cast<const cp_f4 *>(k) + q);
+        cp_f4 r;
         r.x = a.x + dt * b.x;
         r.y = a.y + dt * b.y;
         r.z = a.z + dt * b.z;
         r.w = a.w + dt * b.w;
-        reinterpret_cast<float4 *>(out)[q] = r;
+        __builtin_nontemporal_store(r, reinterpret_cast<cp_f4 *>(out) + q);
         return;
     }
     for (int64_t e = i; e < n && e < i + 4; ++e) out[e] = cur[e] + dt * k[e];
