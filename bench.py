@@ -130,6 +130,45 @@ def cpu_baseline(seconds):
     return res
 
 
+def wave_cpu_timing(seconds):
+    """The 2048^2 wave workload's forward step on the host: the C FP32
+    restatement of the boundary (legacy Dirichlet 0 on every face) and of the
+    u/v update (oracle/pde_oracle.c, the reference loop's arithmetic), single
+    thread and all threads, on the same grid and c^2, from the seeded initial
+    state (the source injection, a few nodes, is left out).  Steps are timed
+    until about `seconds` of CPU work per thread count."""
+    from oracle import c_oracle
+    from oracle import np_oracle as O
+    from pde_module_amd.workloads import wave_2d_config
+    cfg = wave_2d_config(seed=0, steps=1)
+    c2 = np.ascontiguousarray(cfg.c2, dtype=np.float32)
+    rng = np.random.default_rng(0)
+    u = rng.uniform(-1e-3, 1e-3, c2.shape).astype(np.float32)
+    v = np.zeros_like(u)
+    import torch
+    from pde_module_amd.deprecated import GridBoundary as LegacyGridBoundary
+    bc = LegacyGridBoundary(torch.zeros(c2.shape), cfg.dx, cfg.g)     # default Dirichlet 0
+    tabs = (bc.boundary_ijk_indices, bc.boundary_interior, bc.boundary_type, bc.boundary_value)
+    w = O.laplacian_weights(cfg.dx)
+    interior = (cfg.n) ** 2
+    res = {"grid": [cfg.n, cfg.n], "note": "C FP32 restatement (boundary + u/v update), the "
+           "source injection left out"}
+    for threads in (1, c_oracle.max_threads()):
+        uu, vv, reps, t0 = u.copy(), v.copy(), 0, time.perf_counter()
+        while True:
+            u2 = c_oracle.bc_legacy(uu, *tabs, cfg.dx)
+            uu, vv = c_oracle.wave_step_core(u2, vv, c2, w, 1.0, cfg.g, float(cfg.dt), reps == 0,
+                                             nthreads=threads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > seconds / 2 or reps >= 200:
+                break
+        res[f"threads_{threads}"] = {"ms_per_step": round(el / reps * 1e3, 3),
+                                     "glups": round(reps * interior / el / 1e9, 4),
+                                     "steps_timed": reps}
+    return res
+
+
 def wave_adjoint_timing(dev, steps=2000, K=50, rec_distance=None, variants=True):
     """2D acoustic workload (2048^2, smoothed random c, Ricker source at the
     centre, 64 receivers, Dirichlet-0 walls, 2000 steps): time of the plain
@@ -686,6 +725,9 @@ def main():
         wave["receivers_reached"] = wave_adjoint_timing(dev, steps=args.wave_steps,
                                                         rec_distance=400, variants=False)
         progress("wave gradient with receivers the front reaches: checked against recompute")
+        if not args.no_cpu:
+            wave["cpu_forward_step"] = wave_cpu_timing(args.cpu_seconds / 2)
+            progress("wave forward step timed on the host (C restatement)")
         ensemble = heat_ensemble_timing(dev)
         progress("heat ensemble timed")
         cavity = projection_cavity_timing(dev)

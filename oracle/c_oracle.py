@@ -41,6 +41,8 @@ def lib():
         L.orc_bc_functional_f32.argtypes = [_f32p, _f32p, _I64, _I64, _I64, _I, _I, _I, _i64p, _i32p,
                                             _i64p, _i32p, _i8p, _f32p, _f32p, _F, _F]
         L.orc_periodic_fill_f32.argtypes = [_f32p, _I64, _I64, _I64, _I, _I]
+        L.orc_wave_step_f32.argtypes = [_f32p, _f32p, C.c_void_p, _f32p, _f32p, _I64, _I64, _I64,
+                                        _I, _F, _F, _F, _F, _F, _I, _I]
         L.orc_max_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -118,3 +120,17 @@ def periodic_fill(u, g):
     X, Y, Z, Cc = u.shape
     lib().orc_periodic_fill_f32(u, X, Y, Z, Cc, g)
     return u
+
+
+def wave_step_core(u2, v, c2, w, alpha, g, dt_step, first, nthreads=1):
+    """np_oracle.wave_step_core in C: (un, vn) of the fused u/v wave update on
+    the boundary-applied u2 (scalar fields; c2 None means 1)."""
+    X, Y, Z, Cc = u2.shape
+    if Cc != 1:
+        raise ValueError("the C wave step takes scalar fields")
+    un, vn = np.empty_like(u2), np.empty_like(v)
+    c2p = None if c2 is None else np.ascontiguousarray(c2, dtype=np.float32)
+    lib().orc_wave_step_f32(u2, v, None if c2p is None else c2p.ctypes.data_as(C.c_void_p),
+                            un, vn, X, Y, Z, g, w[0], w[1], w[2], np.float32(alpha),
+                            np.float32(dt_step), int(bool(first)), nthreads)
+    return un, vn

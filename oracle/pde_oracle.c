@@ -149,6 +149,45 @@ void orc_forward_euler_f32(const float *cur, const float *k, float *out, int64_t
     for (int64_t i = 0; i < n; ++i) out[i] = cur[i] + dt * k[i];
 }
 
+/* The u/v wave step on a boundary-applied u2 — the reference loop
+ * examples/wave2d/standingWave.py:72-76 (v += dt * c2 * lap(u2); u = u2 + dt * v,
+ * the first step reading the old v), as np_oracle.wave_step_core fuses it: on
+ * interior nodes acc = the Laplacian sum (axes ascending, s = -1, 0, +1),
+ * rhs = alpha * acc, times c2[n] when c2 is given, vn = v + dt * rhs,
+ * un = u2 + dt * (first ? v : vn); elsewhere vn = v and un = u2.  Scalar
+ * fields (one component).                                                    */
+void orc_wave_step_f32(const float *u2, const float *v, const float *c2, float *un, float *vn,
+                       int64_t X, int64_t Y, int64_t Z, int g, float wm, float wc, float wp,
+                       float alpha, float dt, int first, int nthreads)
+{
+    geom_t G = make_geom(X, Y, Z, g);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < X; ++i)
+        for (int64_t j = 0; j < Y; ++j)
+            for (int64_t k = 0; k < Z; ++k) {
+                const int64_t e = (i * Y + j) * Z + k;
+                const int in = i >= G.lo[0] && i < G.hi[0] && j >= G.lo[1] && j < G.hi[1] &&
+                               k >= G.lo[2] && k < G.hi[2];
+                if (!in) {
+                    vn[e] = v[e];
+                    un[e] = u2[e];
+                    continue;
+                }
+                float acc = 0.0f;
+                for (int t = 0; t < G.ndim; ++t) {
+                    const int64_t s = G.sn[G.ax[t]];
+                    acc += wm * u2[e - s];
+                    acc += wc * u2[e];
+                    acc += wp * u2[e + s];
+                }
+                float rhs = alpha * acc;
+                if (c2) rhs = c2[e] * rhs;
+                const float vv = v[e] + dt * rhs;
+                vn[e] = vv;
+                un[e] = u2[e] + dt * (first ? v[e] : vv);
+            }
+}
+
 static inline float neg_sign_times_two(int s) { return (s > 0) ? -2.0f : 2.0f; }
 
 /* Legacy GridBoundary — reference pm/deprecated/gridBoundary.py:193-196 and

@@ -77,3 +77,22 @@ def test_periodic_fill_c_equals_numpy(corc):
         a = O.periodic_fill(u.copy(), g)
         b = corc.periodic_fill(u.copy(), g)
         assert same_bits(a, b)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("g", [1, 2])
+def test_wave_step_c_equals_numpy(corc, shape, g):
+    """The C wave update (the bench's CPU wave step) against the NumPy one, bit
+    for bit: per-node c2 and none, first step and later steps, 1 and all
+    threads."""
+    rng = np.random.default_rng(hash((shape, g, "wave")) % 2**32)
+    u2 = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    v = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    c2 = rng.uniform(0.5, 1.5, shape + (1,)).astype(np.float32)
+    w = O.laplacian_weights(0.07)
+    for cc in (c2, None):
+        for first in (True, False):
+            want = O.wave_step_core(u2, v, cc, w, np.float32(0.9), g, np.float32(0.01), first)
+            for threads in (1, corc.max_threads()):
+                got = corc.wave_step_core(u2, v, cc, w, 0.9, g, 0.01, first, nthreads=threads)
+                assert same_bits(got[0], want[0]) and same_bits(got[1], want[1])
