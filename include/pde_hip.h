@@ -717,16 +717,18 @@ int pde_jacobi_neumann0_multisweep_f32(const float *p, const float *rhs, float *
 
 /* `sweeps` transposed sweeps (pde_jacobi_neumann0_adjoint_f32) in one
  * launch, tiled as pde_jacobi_neumann0_multisweep_f32: gp = the transpose
- * applied `sweeps` times to g (0 on ghosts); on interior nodes grhs gathers
- * each sweep's -h2 * (inv * input), in sweep order, starting from grhs
- * (accumulate 1) or from 0 (accumulate 0: grhs's interior is overwritten,
- * its ghosts are not written).  g, gp and grhs distinct.  The values of
- * `sweeps` launches of pde_jacobi_neumann0_adjoint_f32 bit for bit (on a
- * zeroed grhs for accumulate 0).  No reference counterpart. */
-int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, float *gp, float *grhs,
-                                               int64_t batch, int64_t nx, int64_t ny, int64_t nz,
-                                               int32_t ghost, float h2, float inv,
-                                               int32_t sweeps, int32_t accumulate,
+ * applied `sweeps` times to g, or to g + g2 when g2 is not NULL (the values
+ * of pde_lincomb_f32 with coefficients 1, 1), 0 on ghosts; on interior nodes
+ * grhs gathers each sweep's -h2 * (inv * input), in sweep order, starting
+ * from grhs (accumulate 1) or from 0 (accumulate 0: grhs's interior is
+ * overwritten, its ghosts are not written).  gp and grhs distinct from each
+ * other and from g and g2.  The values of `sweeps` launches of
+ * pde_jacobi_neumann0_adjoint_f32 bit for bit (on a zeroed grhs for
+ * accumulate 0).  No reference counterpart. */
+int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, const float *g2, float *gp,
+                                               float *grhs, int64_t batch, int64_t nx,
+                                               int64_t ny, int64_t nz, int32_t ghost, float h2,
+                                               float inv, int32_t sweeps, int32_t accumulate,
                                                void *stream);
 
 /* The projection cavity's predictor on a 2D vec2 field in one launch:

@@ -48,8 +48,8 @@ SIGNATURES = {
     "pde_bc_apply_inplace_noalloc_f32": [_P, _P, _I64, _P, _P],
     "pde_jacobi_neumann0_multisweep_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32, _F, _F,
                                            _I32, _P],
-    "pde_jacobi_neumann0_adjoint_multisweep_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I32,
-                                                   _F, _F, _I32, _I32, _P],
+    "pde_jacobi_neumann0_adjoint_multisweep_f32": [_P, _P, _P, _P, _I64, _I64, _I64, _I64,
+                                                   _I32, _F, _F, _I32, _I32, _P],
     "pde_heat_multistep_f32": [_P, _P, _P, _I64, _P, _I64, _P, _P, _P, _I32, _P],
     "pde_heat_batch_step_f32": [_P, _P, _P, _I64, _P, _I64, _I32, _I32, _I64, _P],
     "pde_outer_f32": [_P, _P, _P, _I64, _I32, _I32, _P],

@@ -215,8 +215,9 @@ __device__ __forceinline__ JmTile jm_tile(const JmPlan &P, int t)
 
 template <bool ADJ, int PITCH, int NT, int RPB>
 __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
-    const float *__restrict__ in, float *__restrict__ out, const float *__restrict__ rhs,
-    float *__restrict__ grhs, JmPlan PL, float h2, float inv, int sweeps, int accumulate)
+    const float *__restrict__ in, const float *__restrict__ in2, float *__restrict__ out,
+    const float *__restrict__ rhs, float *__restrict__ grhs, JmPlan PL, float h2, float inv,
+    int sweeps, int accumulate)
 {
     using C = JmCfg<PITCH, NT, RPB>;
     __shared__ __align__(16) float S[2 * C::BUF];
@@ -251,9 +252,10 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
     // address (out-of-region ones the grid's first node, then discarded).
     constexpr int PER = C::ROWS * PITCH / NT;
     const float *ig = in + off;
+    const float *ig2 = in2 ? in2 + off : ig;      // the field is in + in2 when in2 is given
     const float *rg = ADJ ? grhs + off : rhs + off;
     {
-        float f[PER], q[PER];
+        float f[PER], f2[PER], q[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int e = tid + NT * j, lr = e / PITCH, lc = e % PITCH;
@@ -261,7 +263,12 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
             const bool in_region = lr < rh && lc < rw;
             const int64_t a = in_region ? (int64_t)x * P2 + k : 0;
             f[j] = ig[a];
+            f2[j] = in2 ? ig2[a] : 0.0f;
             q[j] = ADJ && !accumulate ? 0.0f : rg[in_region && interior(x, k) ? a : 0];
+        }
+        if (in2) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) f[j] = f[j] + f2[j];   // pde_lincomb_f32 (1, 1)
         }
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
@@ -348,9 +355,10 @@ static void jm_axis(int P, int cap, int halo, int *L, int *m)
 constexpr int JM_NT64 = 512;
 
 template <bool ADJ>
-static int jm_launch(const float *in, float *out, const float *rhs, float *grhs, int64_t batch,
-                     int64_t nx, int64_t ny, int64_t nz, int32_t ghost, float h2, float inv,
-                     int32_t sweeps, int32_t accumulate, hipStream_t stream)
+static int jm_launch(const float *in, const float *in2, float *out, const float *rhs,
+                     float *grhs, int64_t batch, int64_t nx, int64_t ny, int64_t nz,
+                     int32_t ghost, float h2, float inv, int32_t sweeps, int32_t accumulate,
+                     hipStream_t stream)
 {
     if (sweeps < 1 || sweeps > PDE_JACOBI_MAX_SWEEPS || batch < 1) return PDE_EINVAL;
     if (batch > 65535) return PDE_ESHAPE;
@@ -379,11 +387,11 @@ static int jm_launch(const float *in, float *out, const float *rhs, float *grhs,
     const dim3 grid((unsigned)tiles, (unsigned)batch);
     if (small)
         hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 64, JM_NT64, 64 * 16 / JM_NT64>), grid,
-                           dim3(JM_NT64), 0, stream, in, out, rhs, grhs, PL, h2, inv, sweeps,
-                           accumulate);
+                           dim3(JM_NT64), 0, stream, in, in2, out, rhs, grhs, PL, h2, inv,
+                           sweeps, accumulate);
     else
         hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 128, 1024, 4>), grid, dim3(1024), 0,
-                           stream, in, out, rhs, grhs, PL, h2, inv, sweeps, accumulate);
+                           stream, in, in2, out, rhs, grhs, PL, h2, inv, sweeps, accumulate);
     return launch_status();
 }
 
@@ -397,18 +405,20 @@ extern "C" int pde_jacobi_neumann0_multisweep_f32(const float *p, const float *r
                                                   int32_t sweeps, void *stream)
 {
     if (!p || !rhs || !out || p == out || rhs == out) return PDE_EINVAL;
-    return jm_launch<false>(p, out, rhs, nullptr, batch, nx, ny, nz, ghost, h2, inv, sweeps, 0,
-                            as_stream(stream));
+    return jm_launch<false>(p, nullptr, out, rhs, nullptr, batch, nx, ny, nz, ghost, h2, inv,
+                            sweeps, 0, as_stream(stream));
 }
 
-extern "C" int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, float *gp, float *grhs,
-                                                          int64_t batch, int64_t nx, int64_t ny,
-                                                          int64_t nz, int32_t ghost, float h2,
-                                                          float inv, int32_t sweeps,
-                                                          int32_t accumulate, void *stream)
+extern "C" int pde_jacobi_neumann0_adjoint_multisweep_f32(const float *g, const float *g2,
+                                                          float *gp, float *grhs, int64_t batch,
+                                                          int64_t nx, int64_t ny, int64_t nz,
+                                                          int32_t ghost, float h2, float inv,
+                                                          int32_t sweeps, int32_t accumulate,
+                                                          void *stream)
 {
     if (!g || !gp || !grhs || g == gp || g == grhs || gp == grhs) return PDE_EINVAL;
+    if (g2 && (g2 == gp || g2 == grhs)) return PDE_EINVAL;
     if (accumulate != 0 && accumulate != 1) return PDE_EINVAL;
-    return jm_launch<true>(g, gp, nullptr, grhs, batch, nx, ny, nz, ghost, h2, inv, sweeps,
+    return jm_launch<true>(g, g2, gp, nullptr, grhs, batch, nx, ny, nz, ghost, h2, inv, sweeps,
                            accumulate, as_stream(stream));
 }

@@ -364,8 +364,8 @@ class ProjectionCavity:
         a = self._adjoint_buffers()
         self.prepare_adjoint(u, gp)
         first, spare = self._sweep_buffers(gp, a["gpb"], a["gpa"])
-        self._adj_correct(gu_out, gp_out, first)
-        self._adj_sweeps(first, spare, a["grhs"], 1)          # ends in gp
+        gpf = self._adj_correct(gu_out)
+        self._adj_sweeps(first, spare, a["grhs"], 1, start=(gpf, gp_out))   # ends in gp
         self._adj_predict(u, gu_out, a["grhs"], gu)
         return gu, gp
 
@@ -377,31 +377,39 @@ class ProjectionCavity:
         writes gp: the launches alternate first -> spare -> first ..."""
         return (gp, b1) if self._sweep_launches() % 2 == 0 else (b0, gp)
 
-    def _adj_correct(self, gu_out, gp_out, gpb, batch=1, gpf=None):
+    def _adj_correct(self, gu_out, batch=1, gpf=None):
         """Transpose of the correction u_out = us2 - dt * grad(B_p(p_out)) in
-        p_out: gpb = B_p^T(grad^T(-dt * gu_out)) + gp_out (gpf: work buffer,
-        the grid's own by default)."""
+        p_out, up to the gradient p_out also receives from the step's output:
+        gpf = B_p^T(grad^T(-dt * gu_out)) (gpf: work buffer, the grid's own by
+        default), returned."""
         g, s = self.g, N.stream_of(gu_out)
         gpf = self.ga["gpf"] if gpf is None else gpf
         N.call("pde_cavity_adjoint_correct_batch_f32", gu_out.data_ptr(), gpf.data_ptr(), batch,
                *self.shape, g, *self.wc, np.float32(self.dt), s)
         self._tables(batch)[1].adjoint_inplace(gpf)
-        N.call("pde_lincomb_f32", gpf.data_ptr(), gp_out.data_ptr(), None, None,
-               np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
-               gpb.data_ptr(), gpb.numel(), s)
+        return gpf
 
-    def _adj_sweeps(self, g, spare, grhs, batch):
+    def _adj_sweeps(self, g, spare, grhs, batch, start=None):
         """The sweeps p <- J(B_p(p), rhs), transposed last to first, from g
         (``batch`` instances one after another), launched as the forward's
         launches are; rhs's gradient gathers in grhs from 0 (the first
         multi-sweep launch starts it, else it is cleared first; its ghosts
-        stay 0).  Returns the buffer (g or spare) holding p's gradient."""
+        stay 0).  With start = (gpf, gp_out) the sweeps start from gpf + gp_out
+        (pde_lincomb_f32's values), summed by the first multi-sweep launch as it
+        loads them, or into g first.  Returns the buffer (g or spare) holding
+        p's gradient."""
         s = N.stream_of(g)
         cur, nxt = g, spare
+        plan = self._sweep_plan()
+        fold = start is not None and self._multi and plan and plan[-1] > 1
+        if start is not None and not fold:
+            gpf, gp_out = start
+            N.call("pde_lincomb_f32", gpf.data_ptr(), gp_out.data_ptr(), None, None,
+                   np.float32(1.0), np.float32(1.0), np.float32(0), np.float32(0), 2,
+                   g.data_ptr(), g.numel(), s)
         if self._multi:
             # the forward's launches transposed, last to first (each launch's
             # sweeps are one linear map; transposes compose in reverse)
-            plan = self._sweep_plan()
             if not plan or plan[-1] == 1:       # no multi-sweep launch comes first
                 grhs.zero_()
             for j, m in enumerate(reversed(plan)):
@@ -412,9 +420,11 @@ class ProjectionCavity:
                     N.call(name, cur.data_ptr(), nxt.data_ptr(), grhs.data_ptr(), *extra,
                            *self.shape, self.g, self.h2, self.inv, s)
                 else:
-                    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", cur.data_ptr(),
-                           nxt.data_ptr(), grhs.data_ptr(), batch, *self.shape, self.g,
-                           self.h2, self.inv, m, int(j > 0 or plan[-1] == 1), s)
+                    src, src2 = (start if fold and j == 0 else (cur, None))
+                    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", src.data_ptr(),
+                           src2.data_ptr() if src2 is not None else None, nxt.data_ptr(),
+                           grhs.data_ptr(), batch, *self.shape, self.g, self.h2, self.inv, m,
+                           int(j > 0 or plan[-1] == 1), s)
                 cur, nxt = nxt, cur
             return cur
         grhs.zero_()
@@ -659,8 +669,8 @@ class ProjectionCavityBatch:
         cav.check_fields((u, gu_out, gu), (gp_out, gp), (B,))
         self.prepare_adjoint(u, gp)
         first, spare = cav._sweep_buffers(gp, self.gpb, self.gpa)
-        cav._adj_correct(gu_out, gp_out, first, batch=B, gpf=self.gpf)
-        cav._adj_sweeps(first, spare, self.grhs, B)     # ends in gp
+        gpf = cav._adj_correct(gu_out, batch=B, gpf=self.gpf)
+        cav._adj_sweeps(first, spare, self.grhs, B, start=(gpf, gp_out))     # ends in gp
         cav._adj_predict(u, gu_out, self.grhs, gu, batch=B, u2buf=self.u2, gus2=self.gus2)
         return gu, gp
 

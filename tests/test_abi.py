@@ -87,8 +87,9 @@ def test_multisweep_jacobi_arguments_need_no_gpu():
     a, b, c = (ctypes.c_void_p(16 * i) for i in (1, 2, 3))   # never dereferenced
     fwd = lib.pde_jacobi_neumann0_multisweep_f32
     adj = lib.pde_jacobi_neumann0_adjoint_multisweep_f32
-    assert adj(a, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, 4, 2, None) == -1   # accumulate 0 or 1
-    for fn in (fwd, lambda *x: adj(*x[:-1], 1, x[-1])):
+    assert adj(a, None, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, 4, 2, None) == -1   # accumulate 0/1
+    assert adj(a, b, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, 4, 1, None) == -1      # g2 aliases gp
+    for fn in (fwd, lambda x0, *x: adj(x0, None, *x[:-1], 1, x[-1])):
         assert fn(a, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, 0, None) == -1
         assert fn(a, b, c, 1, 66, 66, 1, 1, 1.0, 0.25, N.JACOBI_MAX_SWEEPS + 1, None) == -1
         assert fn(a, b, a, 1, 66, 66, 1, 1, 1.0, 0.25, 4, None) == -1     # out aliases in

@@ -68,7 +68,7 @@ def test_multisweep_equals_one_sweep_launches(dev, shape, g, k):
     gin = torch.from_numpy(_rand(rng, shape)).to(dev)
     g0 = _rand(rng, shape)
     gp, grhs = torch.zeros_like(gin), torch.from_numpy(g0.copy()).to(dev)
-    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), gp.data_ptr(),
+    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), None, gp.data_ptr(),
            grhs.data_ptr(), 1, *shape, g, h2, inv, k, 1, N.stream_of(gin))
     grhs_c = torch.from_numpy(g0.copy()).to(dev)
     gp_c = _chain_adjoint(dev, gin, grhs_c, shape, g, h2, inv, k)
@@ -78,13 +78,27 @@ def test_multisweep_equals_one_sweep_launches(dev, shape, g, k):
     # transposed, starting the rhs gradient from 0 (its ghosts are not written)
     junk = _rand(rng, shape)
     grhs0 = torch.from_numpy(junk.copy()).to(dev)
-    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), gp.data_ptr(),
+    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), None, gp.data_ptr(),
            grhs0.data_ptr(), 1, *shape, g, h2, inv, k, 0, N.stream_of(gin))
     zc = torch.zeros_like(gin)
     _chain_adjoint(dev, gin, zc, shape, g, h2, inv, k)
     got0, want0 = grhs0.cpu().numpy(), zc.cpu().numpy()
     assert np.array_equal(got0[mask].view(np.int32), want0[mask].view(np.int32))
     assert np.array_equal(got0[~mask].view(np.int32), junk[~mask].view(np.int32))
+    # transposed from the sum of two inputs: the chain after pde_lincomb_f32 (1, 1)
+    g2 = torch.from_numpy(_rand(rng, shape)).to(dev)
+    gp2, grhs2 = torch.zeros_like(gin), torch.zeros_like(gin)
+    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), g2.data_ptr(),
+           gp2.data_ptr(), grhs2.data_ptr(), 1, *shape, g, h2, inv, k, 1, N.stream_of(gin))
+    summed = torch.empty_like(gin)
+    N.call("pde_lincomb_f32", gin.data_ptr(), g2.data_ptr(), None, None, np.float32(1.0),
+           np.float32(1.0), np.float32(0), np.float32(0), 2, summed.data_ptr(), summed.numel(),
+           N.stream_of(gin))
+    grhs_s = torch.zeros_like(gin)
+    gp_s = _chain_adjoint(dev, summed, grhs_s, shape, g, h2, inv, k)
+    assert np.array_equal(gp2.cpu().numpy().view(np.int32), gp_s.cpu().numpy().view(np.int32))
+    assert np.array_equal(grhs2.cpu().numpy().view(np.int32),
+                          grhs_s.cpu().numpy().view(np.int32))
 
 
 def test_multisweep_batch_equals_instances(dev):
@@ -99,7 +113,7 @@ def test_multisweep_batch_equals_instances(dev):
            *shape, g, h2, inv, k, N.stream_of(p))
     gin = torch.from_numpy(_rand(rng, shape, B)).to(dev)
     gp, grhs = torch.zeros_like(gin), torch.zeros_like(gin)
-    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), gp.data_ptr(),
+    N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin.data_ptr(), None, gp.data_ptr(),
            grhs.data_ptr(), B, *shape, g, h2, inv, k, 1, N.stream_of(gin))
     for i in range(B):
         o1 = torch.zeros_like(p[i])
@@ -107,7 +121,7 @@ def test_multisweep_batch_equals_instances(dev):
                o1.data_ptr(), 1, *shape, g, h2, inv, k, N.stream_of(p))
         assert torch.equal(out[i].view(torch.int32), o1.view(torch.int32))
         g1, r1 = torch.zeros_like(gin[i]), torch.zeros_like(gin[i])
-        N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin[i].data_ptr(), g1.data_ptr(),
-               r1.data_ptr(), 1, *shape, g, h2, inv, k, 1, N.stream_of(gin))
+        N.call("pde_jacobi_neumann0_adjoint_multisweep_f32", gin[i].data_ptr(), None,
+               g1.data_ptr(), r1.data_ptr(), 1, *shape, g, h2, inv, k, 1, N.stream_of(gin))
         assert torch.equal(gp[i].view(torch.int32), g1.view(torch.int32))
         assert torch.equal(grhs[i].view(torch.int32), r1.view(torch.int32))
