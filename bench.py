@@ -756,25 +756,25 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic.get("ops3d_band<1, true, 4>") if traffic else None,
-                     "kernel": "ops3d_band<1, true, 4> (periodic Laplacian)",
+                     "traffic": traffic.get("ops3d_band<1, true, 4, 4>") if traffic else None,
+                     "kernel": "ops3d_band<1, true, 4, 4> (periodic Laplacian)",
                      "avg_launch_us": round(t_step_kernel * 1e6, 2),
                      "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
         "extra": {
             "dropin_laplacian_kernel": {
-                "kernel": "ops3d_band<1, false, 4>", "us": round(t_lap * 1e6, 2),
+                "kernel": "ops3d_band<1, false, 4, 4>", "us": round(t_lap * 1e6, 2),
                 "GBs": round(lap_gbs, 1), "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
                 "glups": round(LUPS / t_lap / 1e9, 1),
-                "traffic": traffic.get("ops3d_band<1, false, 4>") if traffic else None},
+                "traffic": traffic.get("ops3d_band<1, false, 4, 4>") if traffic else None},
             "dropin_step_fill_plus_laplacian": {
                 "us": round(t_dropin * 1e6, 2), "glups": round(LUPS / t_dropin / 1e9, 1)},
             "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1),
-                     "traffic": traffic.get("ops3d_band<2, false, 8>") if traffic else None},
+                     "traffic": traffic.get("ops3d_band<2, false, 8, 2>") if traffic else None},
             "div": {"us": round(t_div * 1e6, 1), "GBs": round(DIV_BYTES * LUPS / t_div / 1e9, 1),
-                    "traffic": traffic.get("ops3d_band<4, false, 8>") if traffic else None},
+                    "traffic": traffic.get("ops3d_band<4, false, 8, 16>") if traffic else None},
             "fused_lap_grad_div": {"us": round(t_fused * 1e6, 1),
                                    "GBs": round(FUSED_BYTES * LUPS / t_fused / 1e9, 1),
-                                   "traffic": traffic.get("ops3d_band<7, false, 4>") if traffic else None,
+                                   "traffic": traffic.get("ops3d_band<7, false, 4, 4>") if traffic else None,
                                    "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},
             "copy_GBs": round(copy_gbs, 1),
             "forward_euler": {"us": round(t_euler * 1e6, 1),

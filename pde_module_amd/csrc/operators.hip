@@ -70,8 +70,8 @@ __global__ __launch_bounds__(256) void lap_generic(const float *__restrict__ in,
 // 3D scalar Laplacian: x-marching row tiles in an XCD band order.
 //
 // A block owns TB rows (y; LB_TB = 4 by default, 8 for Grad and Div) x 512
-// points (z) of LB_NXC consecutive planes
-// (x) and walks them one plane per step.  Each lane holds 4 consecutive z
+// points (z) of NXC consecutive planes (x; LB_NXC = 4 by default, 2 for Grad,
+// 16 for Div on large grids) and walks them one plane per step.  Each lane holds 4 consecutive z
 // points, so every global access is a 16-byte load or store (unaligned by the
 // ghost offset, which the hardware serves).  Planes x-1, x, x+1 ride in
 // registers; plane x+2, the next plane's tile-edge halo rows and its row-end
@@ -255,7 +255,7 @@ __device__ __forceinline__ float axis3(float m, float c, float p, float wm, floa
 // register-heavy (84 and 137 VGPRs: 5 and 3 waves per SIMD); capping them at
 // 6 and 4 waves spills to scratch and measured slower (519 vs 454 us, 1136 vs
 // 957 us at 512^3), so no occupancy is forced.
-template <int OPS, bool PER, int TB = LB_TB>
+template <int OPS, bool PER, int TB = LB_TB, int NXC = LB_NXC>
 __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__ s,
                                                           const float *__restrict__ v,
                                                           float *__restrict__ lap,
@@ -299,8 +299,8 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
     // every lane of the wave holds 4 points (uniform over the wave)
     const bool wave_full = __all(mode == 2);
     const int64_t g = m.g;
-    const int64_t x0 = (int64_t)xt * LB_NXC;
-    const int nx = (int)(m.E0 - x0 < LB_NXC ? m.E0 - x0 : LB_NXC);
+    const int64_t x0 = (int64_t)xt * NXC;
+    const int nx = (int)(m.E0 - x0 < NXC ? m.E0 - x0 : NXC);
     // node offsets within a plane: the lane's centre, its tile-edge halo row,
     // its memory-read z neighbour
     const int64_t rc = (j + g) * m.SY + (k0 + g);
@@ -768,7 +768,14 @@ namespace pde {
 // rows); smaller grids take the generic kernels.
 static bool lap3d_band_fits(const Geom &G) { return G.E[1] >= LB_TB; }
 
-template <int OPS, bool PER, int TB = LB_TB>
+// Blocks of a band launch with TB-row tiles and NXC-plane chunks.
+static int64_t band_blocks(const Geom &G, int64_t TB, int64_t NXC)
+{
+    const int64_t T = ((G.E[2] + LB_TZ - 1) / LB_TZ) * ((G.E[1] + TB - 1) / TB);
+    return 8 * ((T + 7) / 8) * ((G.E[0] + NXC - 1) / NXC);
+}
+
+template <int OPS, bool PER, int TB = LB_TB, int NXC = LB_NXC>
 static int ops3d_band_launch(const float *s_, const float *v, float *lap, float *grad,
                              float *div, const Geom &G, int ghost, const BandW &W,
                              hipStream_t s)
@@ -785,13 +792,13 @@ static int ops3d_band_launch(const float *s_, const float *v, float *lap, float 
     const int64_t ty = (G.E[1] + TB - 1) / TB;
     const int64_t T = tz * ty;
     const int64_t per = (T + 7) / 8;
-    const int64_t nxt = (G.E[0] + LB_NXC - 1) / LB_NXC;
+    const int64_t nxt = (G.E[0] + NXC - 1) / NXC;
     const int64_t nb = 8 * per * nxt;
     if (ty > 0xffffffffLL || T > 0xffffffffLL || nb > 0x7fffffff) return PDE_ESHAPE;
     m.ty = (uint32_t)ty;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;
-    hipLaunchKernelGGL((ops3d_band<OPS, PER, TB>), dim3((unsigned)nb), dim3(128, TB), 0, s, s_, v,
+    hipLaunchKernelGGL((ops3d_band<OPS, PER, TB, NXC>), dim3((unsigned)nb), dim3(128, TB), 0, s, s_, v,
                        lap, grad, div, m, W);
     return launch_status();
 }
@@ -891,10 +898,12 @@ extern "C" int pde_grad_f32(const float *in, float *out, int64_t nx, int64_t ny,
     if (G.ndim == 3 && ncomp == 1 && lap3d_band_fits(G)) {
         const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, alpha, 0.0f};
         // 8-row tiles where the grid has 8 rows (fewer halo rows per output
-        // row; measured 415 vs 439 us at 512^3), else 4-row tiles
+        // row; measured 415 vs 439 us at 512^3), else 4-row tiles.  Grad
+        // writes three floats per node read: 2-plane chunks (twice the blocks
+        // of 4-plane ones) measured 384 against 393 us at 512^3, 1 plane 478
         if (G.E[1] >= 8)
-            return ops3d_band_launch<BAND_GRAD, false, 8>(in, nullptr, nullptr, out, nullptr, G,
-                                                          ghost, W, s);
+            return ops3d_band_launch<BAND_GRAD, false, 8, 2>(in, nullptr, nullptr, out, nullptr, G,
+                                                             ghost, W, s);
         return ops3d_band_launch<BAND_GRAD, false>(in, nullptr, nullptr, out, nullptr, G, ghost,
                                                    W, s);
     }
@@ -920,9 +929,18 @@ extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64
     hipStream_t s = as_stream(stream);
     if (G.ndim == 3 && nrows == 1 && lap3d_band_fits(G)) {
         const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, 0.0f, alpha};
-        if (G.E[1] >= 8)     // 8-row tiles: 450 vs 456 us at 512^3
+        if (G.E[1] >= 8) {   // 8-row tiles: 450 vs 456 us at 512^3
+            // the vec3 planes fill L2 three times as fast as a scalar's, so a
+            // chunk's boundary planes are mostly re-read from memory by the
+            // next chunk: 16-plane chunks re-read fewer of them (434 against
+            // 448 us at 512^3; 32 planes 433, 64 planes 431) wherever they
+            // still give every CU 4 blocks
+            if (band_blocks(G, 8, 16) >= 1024)
+                return ops3d_band_launch<BAND_DIV, false, 8, 16>(nullptr, in, nullptr, nullptr,
+                                                                 out, G, ghost, W, s);
             return ops3d_band_launch<BAND_DIV, false, 8>(nullptr, in, nullptr, nullptr, out, G,
                                                          ghost, W, s);
+        }
         return ops3d_band_launch<BAND_DIV, false>(nullptr, in, nullptr, nullptr, out, G, ghost, W,
                                                   s);
     }

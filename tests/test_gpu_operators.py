@@ -243,3 +243,23 @@ def test_full_size_grad_div_fused_match_oracle(dev, corc):
     assert eq(lo.cpu().numpy(), want_l)
     assert eq(go.cpu().numpy(), want_g)
     assert eq(do.cpu().numpy(), want_d)
+
+
+def test_long_chunk_grad_div_match_oracle(dev, corc):
+    """Grad and Divergence on a grid large enough for Divergence's 16-plane
+    chunks (interior 118 x 510 x 520: two z tiles, the second of 8 points,
+    64 row tiles with a shifted last one, 8 chunks of which the last has 6
+    planes; 8 x 16 x 8 = 1024 blocks) and Grad's 2-plane chunks, against the
+    OpenMP C restatement, bit for bit."""
+    shape, g, dx = (120, 512, 522), 1, 0.013
+    print(f"long-chunk grad/div {shape}", flush=True)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    s = torch.rand(shape + (1,), device=dev, generator=gen) * 2 - 1
+    v = torch.rand(shape + (3,), device=dev, generator=gen) * 2 - 1
+    nt = min(16, corc.max_threads())
+    grad = Grad(1, shape, dx, g)
+    want_g = corc.grad(s.cpu().numpy(), grad.stencil, np.float32(0.7), g, nthreads=nt)
+    assert same_bits(grad(s, 0.7).cpu().numpy(), want_g, ghost_free_mask(shape, g, (3,)))
+    div = Divergence(3, shape, dx, g)
+    want_d = corc.divergence(v.cpu().numpy(), div.stencil, np.float32(-1.1), g, nthreads=nt)
+    assert same_bits(div(v, -1.1).cpu().numpy(), want_d, ghost_free_mask(shape, g, (1,)))
