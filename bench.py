@@ -233,7 +233,8 @@ def wave_adjoint_timing(dev, steps=2000, K=50, rec_distance=None, variants=True)
            "mode": "checkpoint every K steps; the backward walks each segment back from the "
                    "checkpoint at its end, two transposed steps per launch, rebuilding the "
                    "states (boundary-overwritten scalars recorded by the forward) instead of "
-                   "recomputing the segment; the first segment is recomputed"}
+                   "recomputing the segment; steps 0 and 1 (plain Euler first step) are "
+                   "recomputed"}
     if variants:
         t_fa = timed(grad_of(ro, "rec"))
         ro_all = mk(None)

@@ -139,8 +139,9 @@ class WaveRollout:
     one (``pde_wave_reverse_adjoint_pair_f32``).  The gradients in u0 and v0
     are those of the recompute path bit for bit; the c2 gradient differs only
     by the rounding of the rebuilt states (checkpoints bound how far that
-    rounding can travel).  Segments of odd length and the first segment (the
-    plain-Euler first step is not reversible) use the recompute path."""
+    rounding can travel).  Segments of odd length use the recompute path, and
+    so do steps 0 and 1 of the first segment (the plain-Euler step 0 is not
+    reversible); the rest of the first segment is walked back by pairs."""
 
     def __init__(self, boundary, dx, ghost_cells, dt, steps, checkpoint_every=50, alpha=1.0,
                  src_idx=None, src_wavelet=None, rec_idx=None, use_graph=False, reverse=False):
@@ -489,18 +490,26 @@ class WaveRollout:
             for s in reversed(range(nseg)):
                 n0 = s * self.K
                 L = min(self.K, self.steps - n0)
-                if self.reverse and s > 0 and L % 2 == 0:
+                # steps walked back by pair launches: the whole segment, or in
+                # the first one every step but 0 and 1 (pairs rebuild states
+                # down to step 2; step 0 is the plain-Euler one, which is not
+                # reversible), recomputed below from the checkpoint
+                R = (L if s > 0 else L - 2) if self.reverse and L % 2 == 0 else 0
+                if R > 0:
                     self._ru[0].copy_(self._cu[s + 1])
                     self._rv[0].copy_(self._cv[s + 1])
                     self._set_counter(n0 + L - 1)
-                    self._run(("rev", L, c2 is None, need_c2), self._rev_segment, L, c2,
+                    self._run(("rev", R, c2 is None, need_c2), self._rev_segment, R, c2,
                               need_c2)
+                if R == L:
                     continue
+                Lr = L - R
                 self._hist[0].copy_(self._cu[s])
                 self._vt.copy_(self._cv[s])
                 self._set_counter(n0)
-                self._run(("rec", L, c2 is None), self._hist_segment, 0, L, c2, False)
-                self._run(("adj", L, c2 is None, need_c2), self._adj_segment, 0, L, c2, need_c2)
+                self._run(("rec", Lr, c2 is None), self._hist_segment, 0, Lr, c2, False)
+                self._run(("adj", Lr, c2 is None, need_c2), self._adj_segment, 0, Lr, c2,
+                          need_c2)
         return (self._gc2.clone() if need_c2 else None), self._lu.clone(), self._lv.clone()
 
     def __call__(self, c2, u0, v0):

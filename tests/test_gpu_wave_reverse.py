@@ -106,8 +106,11 @@ def _run(dev, name, steps, K, reverse, use_graph=False, c2_on=True, seed=5):
                 c2=c2, u0=u0, v0=v0, src=src, wav=wav, rec=rec)
 
 
+# the first segment walks back by pairs down to step 2 and recomputes steps 1
+# and 0 (K = 4: one pair; K = 7: odd segments, recomputed whole)
 @pytest.mark.parametrize("name,steps,K", [("square", 120, 10), ("wide", 60, 12),
-                                          ("functional", 80, 20), ("g2", 60, 12)])
+                                          ("functional", 80, 20), ("g2", 60, 12),
+                                          ("square", 24, 4), ("wide", 21, 7)])
 def test_reverse_matches_recompute(dev, name, steps, K):
     print(f"reverse vs recompute: {name} steps={steps} K={K}", flush=True)
     a = _run(dev, name, steps, K, reverse=False)
