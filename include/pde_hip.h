@@ -767,7 +767,8 @@ int pde_cavity_adjoint_predict_f32(const float *gus, const float *u2, float *gu2
 /* The correction's transpose in the pressure, up to the boundary's
  * transpose, in one launch: gpf = grad^T(-dt * gu_out) (gpf scalar, gu_out
  * vec2, 2D), the values of pde_scale_f32(-dt) + pde_grad_adjoint_f32
- * (alpha 1); every node of gpf written.  No reference counterpart. */
+ * (alpha 1); every node of gpf written; gpf may not alias gu_out.  No
+ * reference counterpart. */
 int pde_cavity_adjoint_correct_f32(const float *gu_out, float *gpf, int64_t nx, int64_t ny,
                                    int64_t nz, int32_t ghost, float grad_wm, float grad_wc,
                                    float grad_wp, float dt, void *stream);

@@ -1096,7 +1096,8 @@ extern "C" int pde_cavity_adjoint_correct_batch_f32(const float *gu_out, float *
                                                     float grad_wc, float grad_wp, float dt,
                                                     void *stream)
 {
-    if (!gu_out || !gpf) return PDE_EINVAL;
+    // gpf gathers gu_out's neighbours: it may not alias it
+    if (!gu_out || !gpf || gpf == gu_out) return PDE_EINVAL;
     Geom G;
     dim3 grid;
     int st = cavity_grid(batch, nx, ny, nz, ghost, false, &G, &grid);

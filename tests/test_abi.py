@@ -113,6 +113,10 @@ def test_batched_cavity_arguments_need_no_gpu():
                                             0.0, 1.0, 0.01, 0.1, None) == -2
     assert lib.pde_cavity_adjoint_div_batch_f32(a, b, a, 2, 40, 40, 1, 1, 1.0, 0.0, 1.0, 1.0,
                                                 None) == -1       # gus2 aliases grhs
+    assert lib.pde_cavity_adjoint_correct_batch_f32(a, a, 2, 40, 40, 1, 1, 1.0, 0.0, 1.0, 0.1,
+                                                    None) == -1   # gpf aliases gu_out
+    assert lib.pde_cavity_adjoint_correct_f32(a, a, 40, 40, 1, 1, 1.0, 0.0, 1.0, 0.1,
+                                              None) == -1
 
 
 def test_argument_errors_need_no_gpu():
