@@ -16,8 +16,10 @@
 // nodes (the last ones shorter; one piece along an axis that fits a
 // region), computed from the block index; a tile's region is its output
 // grown by `sweeps` nodes on every side, inside the grid.  A workgroup loads
-// its region once into one of two LDS buffers and sweeps between them, one
-// barrier per sweep.  Cells on a side where the region was cut out of its
+// its region once into LDS and sweeps it there: between two buffers, one
+// barrier per sweep, or (the forward of large batches) in one buffer, each
+// sweep's rows held in registers between two barriers, which halves the LDS
+// and raises the occupancy.  Cells on a side where the region was cut out of its
 // grid go stale one layer per sweep, so after the last sweep the output
 // rectangle holds exactly what the one-sweep launches reach, and is stored.
 //
@@ -141,14 +143,18 @@ __device__ __forceinline__ JmWall jm_wall(const JmGeom &g, int x, int k)
     return w;
 }
 
-// `sweeps` sweeps from buffer 0; returns the buffer holding the result.
-template <bool ADJ, bool EDGE, int PITCH, int RPB>
+// `sweeps` sweeps from buffer A; returns the buffer holding the result.  Two
+// buffers (ONE false): each sweep writes B, one barrier, the buffers swap.
+// One buffer (ONE true, B unused): each sweep keeps its rows in registers,
+// a barrier, writes them back over A, a second barrier.
+template <bool ADJ, bool EDGE, int PITCH, int RPB, bool ONE>
 __device__ __forceinline__ float *jm_sweeps(float *A, float *B, int sweeps, int lr0, int c4,
                                             float (&r)[RPB][4], float h2, float inv,
                                             const JmGeom &g)
 {
     const int k0 = g.c0 + 4 * c4;
     for (int s = 0; s < sweeps; ++s) {
+        ms_f4 nv[ONE ? RPB : 1];
         ms_f4 up = jm_rd<PITCH>(A, lr0 - 1, c4), mid = jm_rd<PITCH>(A, lr0, c4);
 #pragma unroll
         for (int h = 0; h < RPB; ++h) {
@@ -172,14 +178,23 @@ __device__ __forceinline__ float *jm_sweeps(float *A, float *B, int sweeps, int 
                     r[h][i] = r[h][i] + -h2 * (inv * cc[i]);
                 }
             }
-            jm_wr<PITCH>(B, lr0 + h, c4, ms_f4{o[0], o[1], o[2], o[3]});
+            if constexpr (ONE)
+                nv[h] = ms_f4{o[0], o[1], o[2], o[3]};
+            else
+                jm_wr<PITCH>(B, lr0 + h, c4, ms_f4{o[0], o[1], o[2], o[3]});
             up = mid;
             mid = dn;
         }
         __syncthreads();
-        float *t = A;
-        A = B;
-        B = t;
+        if constexpr (ONE) {
+#pragma unroll
+            for (int h = 0; h < RPB; ++h) jm_wr<PITCH>(A, lr0 + h, c4, nv[h]);
+            __syncthreads();
+        } else {
+            float *t = A;
+            A = B;
+            B = t;
+        }
     }
     return A;
 }
@@ -213,14 +228,15 @@ __device__ __forceinline__ JmTile jm_tile(const JmPlan &P, int t)
     return T;
 }
 
-template <bool ADJ, int PITCH, int NT, int RPB>
-__global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
+// MINW: the waves per SIMD the register allocation must allow.
+template <bool ADJ, int PITCH, int NT, int RPB, bool ONE, int MINW>
+__global__ __launch_bounds__(NT, MINW) void jacobi_mstep_kernel(
     const float *__restrict__ in, const float *__restrict__ in2, float *__restrict__ out,
     const float *__restrict__ rhs, float *__restrict__ grhs, JmPlan PL, float h2, float inv,
     int sweeps, int accumulate)
 {
     using C = JmCfg<PITCH, NT, RPB>;
-    __shared__ __align__(16) float S[2 * C::BUF];
+    __shared__ __align__(16) float S[(ONE ? 1 : 2) * C::BUF];
     const JmTile T = jm_tile(PL, blockIdx.x);
     const int rh = T.rh, rw = T.rw;
     // the host sizes the plan for the LDS arrays; a tile outside them is skipped
@@ -237,7 +253,7 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
     g.hi2 = PL.P2 - PL.ghost;
     g.r0 = T.r0;
     g.c0 = T.c0;
-    float *A = S, *B = S + C::BUF;
+    float *A = S, *B = ONE ? S : S + C::BUF;
 
     const int band = tid / C::LANES, c4 = tid - band * C::LANES, lr0 = band * RPB;
     const int lc0 = 4 * c4;
@@ -245,16 +261,21 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
         return x >= g.lo1 && x < g.hi1 && k >= g.lo2 && k < g.hi2;
     };
     // Stage the region row by row (consecutive threads, consecutive
-    // columns): the field into buffer 0, h2 * rhs (forward) or the running
-    // rhs gradient (transpose; 0 when not accumulating) of interior nodes
-    // into buffer 1, 0 elsewhere.
-    // Every load is issued before the first is used; each reads a valid
-    // address (out-of-region ones the grid's first node, then discarded).
+    // columns): the field into buffer A; with two buffers also h2 * rhs
+    // (forward) or the running rhs gradient (transpose; 0 when not
+    // accumulating) of interior nodes into buffer B, 0 elsewhere, from where
+    // each thread takes its own chunk; with one buffer each thread loads its
+    // chunk of those straight into registers.
+    // Each load reads a valid address (out-of-region ones the grid's first
+    // node, then discarded).
     constexpr int PER = C::ROWS * PITCH / NT;
     const float *ig = in + off;
     const float *ig2 = in2 ? in2 + off : ig;      // the field is in + in2 when in2 is given
     const float *rg = ADJ ? grhs + off : rhs + off;
-    {
+    float r[RPB][4];
+    bool edge = false;
+    if constexpr (!ONE) {
+        // every load issued before the first is used
         float f[PER], f2[PER], q[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
@@ -279,19 +300,40 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
             A[(lr + 1) * PITCH + lc] = in_region ? f[j] : 0.0f;
             B[(lr + 1) * PITCH + lc] = in_int ? (ADJ ? q[j] : h2 * q[j]) : 0.0f;
         }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < RPB; ++h) {
+            const ms_f4 v = jm_rd<PITCH>(B, lr0 + h, c4);
+            r[h][0] = v.x;
+            r[h][1] = v.y;
+            r[h][2] = v.z;
+            r[h][3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + NT * j, lr = e / PITCH, lc = e % PITCH;
+            const bool in_region = lr < rh && lc < rw;
+            const int64_t a = in_region ? (int64_t)(T.r0 + lr) * P2 + T.c0 + lc : 0;
+            float f = ig[a];
+            if (in2) f = f + ig2[a];                  // pde_lincomb_f32 (1, 1)
+            A[(lr + 1) * PITCH + lc] = in_region ? f : 0.0f;
+        }
+#pragma unroll
+        for (int h = 0; h < RPB; ++h) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int lr = lr0 + h, lc = lc0 + i, x = T.r0 + lr, k = T.c0 + lc;
+                const bool in_int = lr < rh && lc < rw && interior(x, k);
+                const float q =
+                    ADJ && !accumulate ? 0.0f : rg[in_int ? (int64_t)x * P2 + k : 0];
+                r[h][i] = in_int ? (ADJ ? q : h2 * q) : 0.0f;
+            }
+        }
     }
-    __syncthreads();
-    // the thread's chunk of buffer 1 into registers, and whether any of its
-    // interior nodes lies within two of a wall
-    float r[RPB][4];
-    bool edge = false;
+    // whether any of the thread's interior nodes lies within two of a wall
 #pragma unroll
     for (int h = 0; h < RPB; ++h) {
-        const ms_f4 v = jm_rd<PITCH>(B, lr0 + h, c4);
-        r[h][0] = v.x;
-        r[h][1] = v.y;
-        r[h][2] = v.z;
-        r[h][3] = v.w;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int lr = lr0 + h, lc = lc0 + i, x = T.r0 + lr, k = T.c0 + lc;
@@ -299,17 +341,19 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
                             (x - 2 < g.lo1 || x + 2 >= g.hi1 || k - 2 < g.lo2 || k + 2 >= g.hi2));
         }
     }
-    __syncthreads();   // buffer 1 is the first sweep's destination
+    __syncthreads();   // buffer B is the first sweep's destination; A is complete
     const bool wave_edge = __ballot(edge) != 0;   // uniform over the wave
-    float *R = wave_edge ? jm_sweeps<ADJ, true, PITCH, RPB>(A, B, sweeps, lr0, c4, r, h2, inv, g)
-                         : jm_sweeps<ADJ, false, PITCH, RPB>(A, B, sweeps, lr0, c4, r, h2, inv, g);
-    float *W = R == A ? B : A;   // the last sweep's source, free after its barrier
+    float *R = wave_edge
+                   ? jm_sweeps<ADJ, true, PITCH, RPB, ONE>(A, B, sweeps, lr0, c4, r, h2, inv, g)
+                   : jm_sweeps<ADJ, false, PITCH, RPB, ONE>(A, B, sweeps, lr0, c4, r, h2, inv, g);
+    float *W = R == A ? B : A;   // two buffers: the last sweep's source, free after its barrier
 
     // The output rectangle, row by row: the forward stores interior nodes
     // only (the folded sweep never writes ghosts); the transpose every node
-    // (0 on ghosts) and, staged through the free buffer, the rhs gradient of
-    // its interior nodes.
-    if (ADJ) {
+    // (0 on ghosts) and the rhs gradient of its interior nodes: with two
+    // buffers staged through the free one and stored row by row, with one
+    // each thread its own chunk from registers.
+    if (ADJ && !ONE) {
 #pragma unroll
         for (int h = 0; h < RPB; ++h)
             jm_wr<PITCH>(W, lr0 + h, c4, ms_f4{r[h][0], r[h][1], r[h][2], r[h][3]});
@@ -328,8 +372,20 @@ __global__ __launch_bounds__(NT) void jacobi_mstep_kernel(
         const bool in_ = interior(x, k);
         if (in_) og[a] = R[(lr + 1) * PITCH + lc];
         if (ADJ) {
-            if (in_) gg[a] = W[(lr + 1) * PITCH + lc];
-            else og[a] = 0.0f;
+            if (!in_) og[a] = 0.0f;
+            else if (!ONE) gg[a] = W[(lr + 1) * PITCH + lc];
+        }
+    }
+    if (ADJ && ONE) {
+#pragma unroll
+        for (int h = 0; h < RPB; ++h) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int lr = lr0 + h, lc = lc0 + i;
+                if (lr < orow0 || lr >= orow1 || lc < ocol0 || lc >= ocol1) continue;
+                const int x = T.r0 + lr, k = T.c0 + lc;
+                if (interior(x, k)) gg[(int64_t)x * P2 + k] = r[h][i];
+            }
         }
     }
 }
@@ -351,7 +407,7 @@ static void jm_axis(int P, int cap, int halo, int *L, int *m)
 }
 
 // 64 x 64 regions: 512 threads of 2 rows each (more waves per tile than 4
-// rows per thread, less LDS traffic per node than 1)
+// rows per thread, less LDS traffic per node than 1).
 constexpr int JM_NT64 = 512;
 
 template <bool ADJ>
@@ -373,11 +429,18 @@ static int jm_launch(const float *in, const float *in2, float *out, const float 
     PL.ghost = ghost;
     PL.halo = sweeps;
     PL.stride = G.P[1] * G.P[2];
-    // 64 x 64 regions (512 threads, up to four workgroups per CU) while the halo
-    // leaves them room and the grids are small enough to want many
-    // workgroups; 128 x 128 (1024 threads) otherwise
-    const bool small = (PL.P1 <= 64 && PL.P2 <= 64) ||
-                       (2 * sweeps + 16 <= 64 && batch * PL.stride <= (int64_t)1 << 22);
+    // 64 x 64 regions (512 threads, two LDS buffers, up to four workgroups
+    // per CU) while the halo leaves them room and the grids are small enough
+    // to want many workgroups; 128 x 128 (1024 threads) otherwise -- except
+    // the forward of batches over 2^22 nodes, which keeps 64 x 64 regions in
+    // one LDS buffer (17 KB, registers for 6 waves per SIMD).  Measured at
+    // 1026^2, 10 sweeps per launch, graph-replayed cavity steps: 16 seeds
+    // batched, forward step 0.895 against 0.980 ms (128 x 128); the one-buffer
+    // form for one grid 0.119 against 0.110 ms per forward step.
+    const bool fits64 = (PL.P1 <= 64 && PL.P2 <= 64) || 2 * sweeps + 16 <= 64;
+    const bool big = batch * PL.stride > (int64_t)1 << 22;
+    const bool one = !ADJ && fits64 && big;
+    const bool small = (PL.P1 <= 64 && PL.P2 <= 64) || (fits64 && !big) || one;
     const int cap = small ? 64 : 128;
     int m1;
     jm_axis(PL.P1, cap, sweeps, &PL.L1, &m1);
@@ -385,13 +448,18 @@ static int jm_launch(const float *in, const float *in2, float *out, const float 
     const int64_t tiles = (int64_t)m1 * PL.m2;
     if (tiles > 0x7fffffff) return PDE_ESHAPE;
     const dim3 grid((unsigned)tiles, (unsigned)batch);
-    if (small)
-        hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 64, JM_NT64, 64 * 16 / JM_NT64>), grid,
-                           dim3(JM_NT64), 0, stream, in, in2, out, rhs, grhs, PL, h2, inv,
-                           sweeps, accumulate);
+    if (one) {
+        if constexpr (!ADJ)
+            hipLaunchKernelGGL((jacobi_mstep_kernel<false, 64, JM_NT64, 2, true, 6>), grid,
+                               dim3(JM_NT64), 0, stream, in, in2, out, rhs, grhs, PL, h2, inv,
+                               sweeps, accumulate);
+    } else if (small)
+        hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 64, JM_NT64, 2, false, 1>), grid,
+                           dim3(JM_NT64), 0, stream, in, in2, out, rhs, grhs, PL, h2, inv, sweeps,
+                           accumulate);
     else
-        hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 128, 1024, 4>), grid, dim3(1024), 0,
-                           stream, in, in2, out, rhs, grhs, PL, h2, inv, sweeps, accumulate);
+        hipLaunchKernelGGL((jacobi_mstep_kernel<ADJ, 128, 1024, 4, false, 1>), grid, dim3(1024),
+                           0, stream, in, in2, out, rhs, grhs, PL, h2, inv, sweeps, accumulate);
     return launch_status();
 }
 

@@ -101,9 +101,11 @@ def test_multisweep_equals_one_sweep_launches(dev, shape, g, k):
                           grhs_s.cpu().numpy().view(np.int32))
 
 
-def test_multisweep_batch_equals_instances(dev):
-    shape, g, k, B = (70, 200, 1), 1, 7, 3
-    print(f"multisweep ABI batch of {B}", flush=True)
+# (shape, ghost, sweeps, batch): a small batch, and one over 2^22 nodes (the
+# forward in one LDS buffer of 64 x 64 nodes, the transpose in 128 x 128 regions)
+@pytest.mark.parametrize("shape,g,k,B", [((70, 200, 1), 1, 7, 3), ((1026, 1026, 1), 1, 10, 4)])
+def test_multisweep_batch_equals_instances(dev, shape, g, k, B):
+    print(f"multisweep ABI batch of {B} {shape} sweeps={k}", flush=True)
     rng = np.random.default_rng(3)
     h2, inv = np.float32(0.013), np.float32(0.25)
     p = torch.from_numpy(_rand(rng, shape, B)).to(dev)
