@@ -757,16 +757,16 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic.get("ops3d_band<1, true, 4, 4>") if traffic else None,
-                     "kernel": "ops3d_band<1, true, 4, 4> (periodic Laplacian)",
+                     "traffic": traffic.get("lap3d_ring<true, 2>") if traffic else None,
+                     "kernel": "lap3d_ring<true, 2> (periodic Laplacian)",
                      "avg_launch_us": round(t_step_kernel * 1e6, 2),
                      "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
         "extra": {
             "dropin_laplacian_kernel": {
-                "kernel": "ops3d_band<1, false, 4, 4>", "us": round(t_lap * 1e6, 2),
+                "kernel": "lap3d_ring<false, 2>", "us": round(t_lap * 1e6, 2),
                 "GBs": round(lap_gbs, 1), "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
                 "glups": round(LUPS / t_lap / 1e9, 1),
-                "traffic": traffic.get("ops3d_band<1, false, 4, 4>") if traffic else None},
+                "traffic": traffic.get("lap3d_ring<false, 2>") if traffic else None},
             "dropin_step_fill_plus_laplacian": {
                 "us": round(t_dropin * 1e6, 2), "glups": round(LUPS / t_dropin / 1e9, 1)},
             "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1),

@@ -489,6 +489,155 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
     }
 }
 
+// 3D scalar Laplacian through an LDS plane ring (rows of 256 * NW points).
+//
+// The same tiles, chunks and XCD band order as the band kernels (RT_TB = 4
+// rows x 256 * NW z points per block, RT_NXC = 12 planes per chunk), but the
+// planes never pass through registers on their way in: every wave streams its
+// row segment of each plane (16 bytes per lane) straight into LDS with
+// global_load_lds_dwordx4, RT_PF planes ahead of the plane being computed.
+// A ring slot holds one plane of the tile: its TB rows, the two tile-edge halo
+// rows, and in each row the two z neighbours of the segment's ends (position 3
+// and 4 + 256 * NW; the points start at position 4, so every lane's 16-byte
+// LDS read is aligned).  The ring holds planes x-1, x, x+1 in use plus RT_PF
+// planes in flight (RT_PF + 3 slots, 62 KB for NW = 2).
+//
+// Each wave counts its own outstanding vector-memory operations: per plane it
+// issues one row load, one z-end load on the first and last wave of a row, one
+// halo-row load on the tile's first and last rows, and per computed plane one
+// store (stores count in vmcnt on this architecture).  Before computing plane
+// x it waits until everything up to plane x + 1's loads has landed, then a
+// barrier publishes the other waves' loads; the slot refilled after that
+// barrier (plane x + 1 + RT_PF) held plane x - 2, which every wave finished
+// reading before it.  The compiler is kept from moving LDS reads across the
+// barrier by memory clobbers and emits no wait of its own for the ring.
+//
+// Requires E2 a multiple of 256 * NW (every lane holds 4 points) and
+// E1 >= RT_TB; the arithmetic and its order are the band kernel's.
+constexpr int RT_TB = 4;
+constexpr int RT_NXC = 12;
+constexpr int RT_PF = 2;
+
+struct Ring3 {
+    int64_t SX, SY;
+    int32_t E0, E1, E2, g;
+    uint32_t ty, T, per;
+};
+
+template <bool PER>
+__device__ __forceinline__ int wrap_index(int i, int E)
+{
+    if (!PER) return i;
+    return i < 0 ? i + E : (i >= E ? i - E : i);
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 15 (the count is an immediate).
+__device__ __forceinline__ void wait_vm(int n)
+{
+    switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(0x0f70 | 0); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0x0f70 | 1); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0x0f70 | 2); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0x0f70 | 3); break;
+    case 4: __builtin_amdgcn_s_waitcnt(0x0f70 | 4); break;
+    case 5: __builtin_amdgcn_s_waitcnt(0x0f70 | 5); break;
+    case 6: __builtin_amdgcn_s_waitcnt(0x0f70 | 6); break;
+    case 7: __builtin_amdgcn_s_waitcnt(0x0f70 | 7); break;
+    case 8: __builtin_amdgcn_s_waitcnt(0x0f70 | 8); break;
+    case 9: __builtin_amdgcn_s_waitcnt(0x0f70 | 9); break;
+    case 10: __builtin_amdgcn_s_waitcnt(0x0f70 | 10); break;
+    case 11: __builtin_amdgcn_s_waitcnt(0x0f70 | 11); break;
+    case 12: __builtin_amdgcn_s_waitcnt(0x0f70 | 12); break;
+    case 13: __builtin_amdgcn_s_waitcnt(0x0f70 | 13); break;
+    case 14: __builtin_amdgcn_s_waitcnt(0x0f70 | 14); break;
+    default: __builtin_amdgcn_s_waitcnt(0x0f70 | 0); break;
+    }
+}
+
+typedef __attribute__((address_space(3))) void *lds_ptr;
+
+template <bool PER, int NW>
+__global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__restrict__ s,
+                                                              float *__restrict__ out, Ring3 m,
+                                                              float wm, float wc, float wp,
+                                                              float alpha)
+{
+    constexpr int R = RT_PF + 3;             // ring slots
+    constexpr int ZT = 256 * NW;             // z points per tile
+    constexpr int RW = ZT + 8;               // floats per LDS row
+    __shared__ __attribute__((aligned(16))) float ring[R][RT_TB + 2][RW];
+    const uint32_t b = blockIdx.x, slot = b >> 3;
+    const uint32_t xt = slot / m.per;
+    const uint32_t lt = (b & 7u) * m.per + slot % m.per;
+    if (lt >= m.T) return;                   // whole block: no barrier skipped
+    const uint32_t yt = lt % m.ty, zt = lt / m.ty;
+    const int lane = threadIdx.x & 63, half = threadIdx.x >> 6, w = threadIdx.y;
+    int j0 = (int)yt * RT_TB;                // the last tile ends at the last row
+    if (j0 > m.E1 - RT_TB) j0 = m.E1 - RT_TB;
+    const int j = j0 + w;
+    const int k0 = (int)zt * ZT;
+    const int x0 = (int)xt * RT_NXC;
+    const int nx = m.E0 - x0 < RT_NXC ? m.E0 - x0 : RT_NXC;
+    const bool top = w == 0, bot = w == RT_TB - 1;
+    const bool zlo = half == 0, zhi = half == NW - 1;
+    const int64_t g = m.g;
+    const int c0 = 4 + 256 * half + 4 * lane;             // the lane's LDS column
+    // offsets within a plane: the lane's 4 points, its halo row (tile edges),
+    // the row segment's z neighbours
+    const int64_t rc = (j + g) * m.SY + g + (c0 - 4 + k0);
+    const int64_t rh = (wrap_index<PER>(top ? j - 1 : j + 1, m.E1) + g) * m.SY + g + (c0 - 4 + k0);
+    const int64_t rzl = (j + g) * m.SY + g + wrap_index<PER>(k0 - 1, m.E2);
+    const int64_t rzh = (j + g) * m.SY + g + wrap_index<PER>(k0 + ZT, m.E2);
+    // loads of chunk-relative plane pl (-1 .. nx) into its slot
+    auto issue = [&](int pl) {
+        const int sl = (pl + 1) % R;
+        const int64_t pb = (wrap_index<PER>(x0 + pl, m.E0) + g) * m.SX;
+        __builtin_amdgcn_global_load_lds((const void *)(s + pb + rc),
+                                         (lds_ptr)&ring[sl][w + 1][4 + 256 * half], 16, 0, 0);
+        if (zlo && lane == 0)
+            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rzl),
+                                             (lds_ptr)&ring[sl][w + 1][3], 4, 0, 0);
+        if (zhi && lane == 0)
+            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rzh),
+                                             (lds_ptr)&ring[sl][w + 1][4 + ZT], 4, 0, 0);
+        if (top || bot)
+            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rh),
+                                             (lds_ptr)&ring[sl][top ? 0 : RT_TB + 1][4 + 256 * half],
+                                             16, 0, 0);
+    };
+    const int per = 1 + (zlo ? 1 : 0) + (zhi ? 1 : 0) + ((top || bot) ? 1 : 0);
+    const int npro = RT_PF < nx ? RT_PF : nx;
+    for (int p = -1; p <= npro; ++p) issue(p);
+    for (int x = 0; x < nx; ++x) {
+        // issued after plane x + 1's loads: the stores of planes
+        // max(x - RT_PF, 0) .. x - 1 and the loads of planes x + 2 .. last
+        const int last = x + RT_PF < nx ? x + RT_PF : nx;
+        const int ahead = last - (x + 1) > 0 ? last - (x + 1) : 0;
+        wait_vm(__builtin_amdgcn_readfirstlane((x < RT_PF ? x : RT_PF) + ahead * per));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (x + 1 + RT_PF <= nx) issue(x + 1 + RT_PF);
+        const int sc = (x + 1) % R, sm = x % R, sp = (x + 2) % R;
+        const pde_f4 xc = *reinterpret_cast<const pde_f4 *>(&ring[sc][w + 1][c0]);
+        const pde_f4 lo = *reinterpret_cast<const pde_f4 *>(&ring[sc][w][c0]);
+        const pde_f4 hi = *reinterpret_cast<const pde_f4 *>(&ring[sc][w + 2][c0]);
+        const pde_f4 xm = *reinterpret_cast<const pde_f4 *>(&ring[sm][w + 1][c0]);
+        const pde_f4 xp = *reinterpret_cast<const pde_f4 *>(&ring[sp][w + 1][c0]);
+        const float l = ring[sc][w + 1][c0 - 1];
+        const float r = ring[sc][w + 1][c0 + 4];
+        float o[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float zm = t == 0 ? l : el4(xc, t > 0 ? t - 1 : 0);
+            const float zp = t == 3 ? r : el4(xc, t < 3 ? t + 1 : 3);
+            o[t] = lap7(el4(xm, t), el4(xc, t), el4(xp, t), el4(lo, t), el4(hi, t), zm, zp, wm, wc,
+                        wp, alpha);
+        }
+        st4_nt(out + (x0 + x + g) * m.SX + rc, pde_f4{o[0], o[1], o[2], o[3]});
+    }
+}
+
 // Generic periodic Laplacian (any ncomp, 1..3 eligible axes): thread per
 // interior element; a neighbour index that leaves the interior wraps to the
 // opposite interior index.
@@ -803,10 +952,46 @@ static int ops3d_band_launch(const float *s_, const float *v, float *lap, float 
     return launch_status();
 }
 
+template <bool PER, int NW>
+static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
+                             float wc, float wp, float alpha, hipStream_t s)
+{
+    Ring3 m;
+    m.SX = G.S[0];
+    m.SY = G.S[1];
+    m.E0 = (int32_t)G.E[0];
+    m.E1 = (int32_t)G.E[1];
+    m.E2 = (int32_t)G.E[2];
+    m.g = ghost;
+    const int64_t ty = (G.E[1] + RT_TB - 1) / RT_TB;
+    const int64_t T = (G.E[2] / (256 * NW)) * ty;
+    const int64_t per = (T + 7) / 8;
+    const int64_t nb = 8 * per * ((G.E[0] + RT_NXC - 1) / RT_NXC);
+    if (nb > 0x7fffffff) return PDE_ESHAPE;
+    m.ty = (uint32_t)ty;
+    m.T = (uint32_t)T;
+    m.per = (uint32_t)per;
+    hipLaunchKernelGGL((lap3d_ring<PER, NW>), dim3((unsigned)nb), dim3(64 * NW, RT_TB), 0, s, in,
+                       out, m, wm, wc, wp, alpha);
+    return launch_status();
+}
+
+// The LDS plane ring takes 3D grids whose rows split into whole 256-point
+// wave segments and that have at least RT_TB rows; extents stay below 2^31.
+static bool lap3d_ring_fits(const Geom &G)
+{
+    return G.E[2] % 256 == 0 && G.E[1] >= RT_TB && G.E[0] < 0x7fffffffLL &&
+           G.E[1] < 0x7fffffffLL && G.E[2] < 0x7fffffffLL;
+}
+
 template <bool PER>
 static int lap3d_band_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
                              float wc, float wp, float alpha, hipStream_t s)
 {
+    if (lap3d_ring_fits(G))
+        return G.E[2] % 512 == 0
+                   ? lap3d_ring_launch<PER, 2>(in, out, G, ghost, wm, wc, wp, alpha, s)
+                   : lap3d_ring_launch<PER, 1>(in, out, G, ghost, wm, wc, wp, alpha, s);
     BandW W = {wm, wc, wp, alpha, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     return ops3d_band_launch<BAND_LAP, PER>(in, nullptr, out, nullptr, nullptr, G, ghost, W, s);
 }

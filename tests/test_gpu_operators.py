@@ -191,6 +191,37 @@ def test_periodic_laplacian_matches_fill_then_laplacian(dev, corc, shape, g, C):
     assert same_bits(ud.cpu().numpy(), u)
 
 
+# Interior extents whose rows split into whole 256-point wave segments run the
+# LDS plane-ring kernel: one and two waves per row segment, several z tiles,
+# partial and single-plane x chunks, a shifted last row tile, ghost widths 1-3.
+RING_SHAPES = [((13, 5, 256), 1), ((25, 9, 512), 1), ((2, 4, 768), 2), ((14, 7, 1024), 3),
+               ((1, 4, 256), 1), ((24, 6, 512), 2), ((3, 11, 512), 1)]
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("interior,g", RING_SHAPES)
+def test_ring_laplacian_matches_oracle(dev, corc, interior, g, periodic):
+    """The 3D scalar Laplacian on grids the plane-ring kernel takes, plain and
+    periodic, against the C restatement bit for bit; ghosts of the output and
+    the whole input left untouched."""
+    shape = tuple(e + 2 * g for e in interior)
+    print(f"ring laplacian shape={shape} g={g} periodic={periodic}", flush=True)
+    rng = np.random.default_rng(21)
+    u = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+    dx, alpha = 0.043, 0.9
+    lap = (PeriodicLaplacian if periodic else Laplacian)(1, dx, g)
+    ud = to_dev(u, dev)
+    lap.setup(ud)
+    lap.output_array.fill_(float("nan"))
+    got = lap(ud, alpha).cpu().numpy()
+    src = corc.periodic_fill(u.copy(), g) if periodic else u
+    want = corc.laplacian(src, lap.stencil, np.float32(alpha), g)
+    mask = ghost_free_mask(shape, g, (1,))
+    assert same_bits(got, want, mask)
+    assert np.all(np.isnan(got[~mask]))
+    assert same_bits(ud.cpu().numpy(), u)
+
+
 def test_full_size_periodic_laplacian_matches_oracle(dev, corc):
     """The 512^3 periodic benchmark grid (514^3 padded) against the OpenMP C
     restatement, bit for bit on all 134M interior points."""
