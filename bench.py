@@ -772,10 +772,16 @@ def main():
             "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1),
                      "traffic": traffic.get("ops3d_band<2, false, 8, 2>") if traffic else None},
             "div": {"us": round(t_div * 1e6, 1), "GBs": round(DIV_BYTES * LUPS / t_div / 1e9, 1),
-                    "traffic": traffic.get("ops3d_band<4, false, 8, 16>") if traffic else None},
+                    "kernel": "div3d_ring",
+                    "traffic": traffic.get("div3d_ring") if traffic else None},
             "fused_lap_grad_div": {"us": round(t_fused * 1e6, 1),
                                    "GBs": round(FUSED_BYTES * LUPS / t_fused / 1e9, 1),
-                                   "traffic": traffic.get("ops3d_band<7, false, 4, 4>") if traffic else None,
+                                   "kernels": ["ops3d_band<3, false, 8, 2>", "div3d_ring"],
+                                   "traffic": (traffic["ops3d_band<3, false, 8, 2>"]
+                                               + traffic["div3d_ring"]
+                                               if traffic and "div3d_ring" in traffic
+                                               and "ops3d_band<3, false, 8, 2>" in traffic
+                                               else None),
                                    "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},
             "copy_GBs": round(copy_gbs, 1),
             "forward_euler": {"us": round(t_euler * 1e6, 1),

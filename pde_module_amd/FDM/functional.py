@@ -82,8 +82,10 @@ def divergence(kernel, input_array, alpha, ghost_cells, output_array, device=Non
 
 def fused_lap_grad_div(s, v, lap_out, grad_out, div_out, dx, ghost_cells,
                        alpha_lap=1.0, alpha_grad=1.0, alpha_div=1.0):
-    """lap(s), grad(s) and div(v) of a scalar and a vec3 field on one 3D grid in
-    a single HBM pass; each output equals the corresponding module's."""
+    """lap(s), grad(s) and div(v) of a scalar and a vec3 field on one 3D grid,
+    reading each input once and writing each output once (one launch, or a
+    scalar and a vector launch on grids the Divergence ring takes); each
+    output equals the corresponding module's."""
     for name, t in (("s", s), ("v", v), ("lap", lap_out), ("grad", grad_out), ("div", div_out)):
         N.check_field(t, name)
     X, Y, Z = _grid(s)

@@ -499,8 +499,10 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
 // A ring slot holds one plane of the tile: its TB rows, the two tile-edge halo
 // rows, and in each row the two z neighbours of the segment's ends (position 3
 // and 4 + 256 * NW; the points start at position 4, so every lane's 16-byte
-// LDS read is aligned).  The ring holds planes x-1, x, x+1 in use plus RT_PF
-// planes in flight (RT_PF + 3 slots, 62 KB for NW = 2).
+// LDS read is aligned).  The plane before the current one (in march order)
+// stays in registers from the step before, so a slot is free once its plane
+// has been the current one: the ring holds planes x and x+1 in use plus RT_PF
+// planes in flight (RT_PF + 2 slots, 62 KB for NW = 2: two blocks per CU).
 //
 // Each wave counts its own outstanding vector-memory operations: per plane it
 // issues one row load, one z-end load on the first and last wave of a row, one
@@ -508,20 +510,25 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
 // store (stores count in vmcnt on this architecture).  Before computing plane
 // x it waits until everything up to plane x + 1's loads has landed, then a
 // barrier publishes the other waves' loads; the slot refilled after that
-// barrier (plane x + 1 + RT_PF) held plane x - 2, which every wave finished
+// barrier (plane x + 1 + RT_PF) held plane x - 1, which every wave finished
 // reading before it.  The compiler is kept from moving LDS reads across the
 // barrier by memory clobbers and emits no wait of its own for the ring.
+//
+// Odd chunks march down (planes in descending order, the x-1 and x+1 operands
+// swapped back so the arithmetic is unchanged): the chunks on either side of
+// a chunk boundary then load its two planes at about the same time, so the
+// second load finds them in L2 instead of reading them again from memory.
 //
 // Requires E2 a multiple of 256 * NW (every lane holds 4 points) and
 // E1 >= RT_TB; the arithmetic and its order are the band kernel's.
 constexpr int RT_TB = 4;
 constexpr int RT_NXC = 12;
-constexpr int RT_PF = 2;
+constexpr int RT_PF = 3;
 
 struct Ring3 {
     int64_t SX, SY;
     int32_t E0, E1, E2, g;
-    uint32_t ty, T, per;
+    uint32_t tz, T, per;   // z tiles, tiles per chunk, tiles per XCD band
 };
 
 template <bool PER>
@@ -562,23 +569,29 @@ __global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__res
                                                               float wm, float wc, float wp,
                                                               float alpha)
 {
-    constexpr int R = RT_PF + 3;             // ring slots
+    constexpr int TB = RT_TB, NXC = RT_NXC, PF = RT_PF;
+    constexpr int R = PF + 2;                // ring slots; march plane p in slot p % R
     constexpr int ZT = 256 * NW;             // z points per tile
     constexpr int RW = ZT + 8;               // floats per LDS row
-    __shared__ __attribute__((aligned(16))) float ring[R][RT_TB + 2][RW];
+    __shared__ __attribute__((aligned(16))) float ring[R][TB + 2][RW];
     const uint32_t b = blockIdx.x, slot = b >> 3;
     const uint32_t xt = slot / m.per;
     const uint32_t lt = (b & 7u) * m.per + slot % m.per;
     if (lt >= m.T) return;                   // whole block: no barrier skipped
-    const uint32_t yt = lt % m.ty, zt = lt / m.ty;
-    const int lane = threadIdx.x & 63, half = threadIdx.x >> 6, w = threadIdx.y;
-    int j0 = (int)yt * RT_TB;                // the last tile ends at the last row
-    if (j0 > m.E1 - RT_TB) j0 = m.E1 - RT_TB;
+    // z tiles of one row band are neighbours in the XCD band (the segments
+    // read each other's end points through one L2)
+    const uint32_t zt = lt % m.tz, yt = lt / m.tz;
+    // half and w are uniform over a wave (64 * NW threads along x): scalars
+    const int lane = threadIdx.x & 63;
+    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    int j0 = (int)yt * TB;                // the last tile ends at the last row
+    if (j0 > m.E1 - TB) j0 = m.E1 - TB;
     const int j = j0 + w;
     const int k0 = (int)zt * ZT;
-    const int x0 = (int)xt * RT_NXC;
-    const int nx = m.E0 - x0 < RT_NXC ? m.E0 - x0 : RT_NXC;
-    const bool top = w == 0, bot = w == RT_TB - 1;
+    const int x0 = (int)xt * NXC;
+    const int nx = m.E0 - x0 < NXC ? m.E0 - x0 : NXC;
+    const bool top = w == 0, bot = w == TB - 1;
     const bool zlo = half == 0, zhi = half == NW - 1;
     const int64_t g = m.g;
     const int c0 = 4 + 256 * half + 4 * lane;             // the lane's LDS column
@@ -588,44 +601,54 @@ __global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__res
     const int64_t rh = (wrap_index<PER>(top ? j - 1 : j + 1, m.E1) + g) * m.SY + g + (c0 - 4 + k0);
     const int64_t rzl = (j + g) * m.SY + g + wrap_index<PER>(k0 - 1, m.E2);
     const int64_t rzh = (j + g) * m.SY + g + wrap_index<PER>(k0 + ZT, m.E2);
-    // loads of chunk-relative plane pl (-1 .. nx) into its slot
+    const bool down = (xt & 1u) != 0;        // this chunk marches down
+    // base of the pl-th plane of the march (-1 .. nx)
+    auto plane_of = [&](int pl) -> const float * {
+        const int ap = down ? nx - 1 - pl : pl;     // chunk-relative plane
+        return s + (wrap_index<PER>(x0 + ap, m.E0) + g) * m.SX;
+    };
     auto issue = [&](int pl) {
-        const int sl = (pl + 1) % R;
-        const int64_t pb = (wrap_index<PER>(x0 + pl, m.E0) + g) * m.SX;
-        __builtin_amdgcn_global_load_lds((const void *)(s + pb + rc),
+        const int sl = pl % R;
+        const float *pb = plane_of(pl);
+        __builtin_amdgcn_global_load_lds((const void *)(pb + rc),
                                          (lds_ptr)&ring[sl][w + 1][4 + 256 * half], 16, 0, 0);
         if (zlo && lane == 0)
-            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rzl),
+            __builtin_amdgcn_global_load_lds((const void *)(pb + rzl),
                                              (lds_ptr)&ring[sl][w + 1][3], 4, 0, 0);
         if (zhi && lane == 0)
-            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rzh),
+            __builtin_amdgcn_global_load_lds((const void *)(pb + rzh),
                                              (lds_ptr)&ring[sl][w + 1][4 + ZT], 4, 0, 0);
         if (top || bot)
-            __builtin_amdgcn_global_load_lds((const void *)(s + pb + rh),
-                                             (lds_ptr)&ring[sl][top ? 0 : RT_TB + 1][4 + 256 * half],
+            __builtin_amdgcn_global_load_lds((const void *)(pb + rh),
+                                             (lds_ptr)&ring[sl][top ? 0 : TB + 1][4 + 256 * half],
                                              16, 0, 0);
     };
     const int per = 1 + (zlo ? 1 : 0) + (zhi ? 1 : 0) + ((top || bot) ? 1 : 0);
-    const int npro = RT_PF < nx ? RT_PF : nx;
-    for (int p = -1; p <= npro; ++p) issue(p);
+    const int npro = PF < nx ? PF : nx;
+    // march plane -1: a register load issued before every ring load, so the
+    // wait for it waits for nothing else
+    pde_f4 prev = ld4(plane_of(-1) + rc);
+    for (int p = 0; p <= npro; ++p) issue(p);
     for (int x = 0; x < nx; ++x) {
-        // issued after plane x + 1's loads: the stores of planes
-        // max(x - RT_PF, 0) .. x - 1 and the loads of planes x + 2 .. last
-        const int last = x + RT_PF < nx ? x + RT_PF : nx;
+        // issued after plane x + 1's ring loads: the stores of planes
+        // max(x - PF, 0) .. x - 1 and the ring loads of planes x + 2 .. last
+        const int last = x + PF < nx ? x + PF : nx;
         const int ahead = last - (x + 1) > 0 ? last - (x + 1) : 0;
-        wait_vm(__builtin_amdgcn_readfirstlane((x < RT_PF ? x : RT_PF) + ahead * per));
+        wait_vm(__builtin_amdgcn_readfirstlane((x < PF ? x : PF) + ahead * per));
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (x + 1 + RT_PF <= nx) issue(x + 1 + RT_PF);
-        const int sc = (x + 1) % R, sm = x % R, sp = (x + 2) % R;
+        // the slot refilled held plane x - 1, which every wave finished
+        // reading before the barrier
+        if (x + 1 + PF <= nx) issue(x + 1 + PF);
+        const int sc = x % R, sn = (x + 1) % R;
         const pde_f4 xc = *reinterpret_cast<const pde_f4 *>(&ring[sc][w + 1][c0]);
         const pde_f4 lo = *reinterpret_cast<const pde_f4 *>(&ring[sc][w][c0]);
         const pde_f4 hi = *reinterpret_cast<const pde_f4 *>(&ring[sc][w + 2][c0]);
-        const pde_f4 xm = *reinterpret_cast<const pde_f4 *>(&ring[sm][w + 1][c0]);
-        const pde_f4 xp = *reinterpret_cast<const pde_f4 *>(&ring[sp][w + 1][c0]);
+        const pde_f4 xn = *reinterpret_cast<const pde_f4 *>(&ring[sn][w + 1][c0]);
         const float l = ring[sc][w + 1][c0 - 1];
         const float r = ring[sc][w + 1][c0 + 4];
+        const pde_f4 xm = down ? xn : prev, xp = down ? prev : xn;
         float o[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -634,8 +657,189 @@ __global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__res
             o[t] = lap7(el4(xm, t), el4(xc, t), el4(xp, t), el4(lo, t), el4(hi, t), zm, zp, wm, wc,
                         wp, alpha);
         }
-        st4_nt(out + (x0 + x + g) * m.SX + rc, pde_f4{o[0], o[1], o[2], o[3]});
+        st4_nt(out + (x0 + (down ? nx - 1 - x : x) + g) * m.SX + rc,
+               pde_f4{o[0], o[1], o[2], o[3]});
+        prev = xc;
     }
+}
+
+template <bool PER, int NW>
+static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
+                             float wc, float wp, float alpha, hipStream_t s)
+{
+    Ring3 m;
+    m.SX = G.S[0];
+    m.SY = G.S[1];
+    m.E0 = (int32_t)G.E[0];
+    m.E1 = (int32_t)G.E[1];
+    m.E2 = (int32_t)G.E[2];
+    m.g = ghost;
+    const int64_t ty = (G.E[1] + RT_TB - 1) / RT_TB, tz = G.E[2] / (256 * NW);
+    const int64_t T = tz * ty;
+    const int64_t per = (T + 7) / 8;
+    const int64_t nb = 8 * per * ((G.E[0] + RT_NXC - 1) / RT_NXC);
+    if (nb > 0x7fffffff) return PDE_ESHAPE;
+    m.tz = (uint32_t)tz;
+    m.T = (uint32_t)T;
+    m.per = (uint32_t)per;
+    hipLaunchKernelGGL((lap3d_ring<PER, NW>), dim3((unsigned)nb), dim3(64 * NW, RT_TB), 0, s, in,
+                       out, m, wm, wc, wp, alpha);
+    return launch_status();
+}
+
+// 3D vec3 Divergence through an LDS plane ring (the Laplacian ring's scheme
+// for a vector field).  A block owns DR_TB = 4 rows x 256 points (one wave per
+// row, 4 points per lane) of DR_NXC = 16 planes; a ring slot holds the tile's
+// rows of one plane as the field stores them (3 floats per point, 3 KB per
+// row, three 16-byte direct loads per lane), its two halo rows, and the z
+// components of each row segment's two outer neighbours, placed where the
+// lane arithmetic looks for them (position 3 and 4 + 3 * 256 + 2; the points
+// start at position 4).  The x components of the plane before the current one
+// stay in registers; the plane after gives its x components, the halo rows
+// their y components, and each lane's outer z neighbours are single floats
+// beside its 12.  DR_PF = 2 planes in flight, DR_PF + 2 slots (74 KB: two
+// blocks per CU).  Per plane a wave issues three row loads and two z-end
+// loads, the edge rows' waves three more for their halo row; one store per
+// computed plane.  Odd chunks march down as in the Laplacian ring.
+// Requires E2 % 256 == 0 and E1 >= DR_TB; operation order of the band kernel.
+constexpr int DR_TB = 4;
+constexpr int DR_NXC = 16;
+constexpr int DR_PF = 2;
+
+__global__ __launch_bounds__(64 * DR_TB) void div3d_ring(const float *__restrict__ v,
+                                                         float *__restrict__ out, Ring3 m,
+                                                         float cm, float cc, float cp,
+                                                         float alpha)
+{
+    constexpr int TB = DR_TB, NXC = DR_NXC, PF = DR_PF;
+    constexpr int R = PF + 2;                // march plane p in slot p % R
+    constexpr int ZT = 256;
+    constexpr int RW = 3 * ZT + 8;
+    __shared__ __attribute__((aligned(16))) float ring[R][TB + 2][RW];
+    const uint32_t b = blockIdx.x, slot = b >> 3;
+    const uint32_t xt = slot / m.per;
+    const uint32_t lt = (b & 7u) * m.per + slot % m.per;
+    if (lt >= m.T) return;                   // whole block: no barrier skipped
+    // z tiles of one row band are neighbours in the XCD band (the segments
+    // read each other's end points through one L2)
+    const uint32_t zt = lt % m.tz, yt = lt / m.tz;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    int j0 = (int)yt * TB;
+    if (j0 > m.E1 - TB) j0 = m.E1 - TB;
+    const int j = j0 + w;
+    const int k0 = (int)zt * ZT;
+    const int x0 = (int)xt * NXC;
+    const int nx = m.E0 - x0 < NXC ? m.E0 - x0 : NXC;
+    const bool top = w == 0, bot = w == TB - 1;
+    const bool down = (xt & 1u) != 0;
+    const int64_t g = m.g;
+    const int64_t n0 = (j + g) * m.SY + g + k0;                  // first node of the segment
+    const int64_t nh = ((top ? j - 1 : j + 1) + g) * m.SY + g + k0;
+    auto plane_of = [&](int pl) -> const float * {
+        const int ap = down ? nx - 1 - pl : pl;
+        return v + 3 * ((x0 + ap + g) * m.SX);
+    };
+    auto issue = [&](int pl) {
+        const int sl = pl % R;
+        const float *pb = plane_of(pl);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            __builtin_amdgcn_global_load_lds((const void *)(pb + 3 * n0 + 256 * c + 4 * lane),
+                                             (lds_ptr)&ring[sl][w + 1][4 + 256 * c], 16, 0, 0);
+        if (lane == 0) {
+            __builtin_amdgcn_global_load_lds((const void *)(pb + 3 * (n0 - 1) + 2),
+                                             (lds_ptr)&ring[sl][w + 1][3], 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(pb + 3 * (n0 + ZT) + 2),
+                                             (lds_ptr)&ring[sl][w + 1][4 + 3 * ZT + 2], 4, 0, 0);
+        }
+        if (top || bot) {
+            const int hr = top ? 0 : TB + 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                __builtin_amdgcn_global_load_lds((const void *)(pb + 3 * nh + 256 * c + 4 * lane),
+                                                 (lds_ptr)&ring[sl][hr][4 + 256 * c], 16, 0, 0);
+        }
+    };
+    const int per = 5 + ((top || bot) ? 3 : 0);
+    const int npro = PF < nx ? PF : nx;
+    // x components of march plane -1: register loads issued before every ring
+    // load, so the wait for them waits for nothing else
+    const float *p1 = plane_of(-1) + 3 * n0 + 12 * lane;
+    pde_f4 prev = comp_x(ld4(p1), ld4(p1 + 4), ld4(p1 + 8));
+    for (int p = 0; p <= npro; ++p) issue(p);
+    const int c0 = 4 + 12 * lane;
+    for (int x = 0; x < nx; ++x) {
+        // issued after plane x + 1's ring loads: the stores of planes
+        // max(x - PF, 0) .. x - 1 and the ring loads of planes x + 2 .. last
+        const int last = x + PF < nx ? x + PF : nx;
+        const int ahead = last - (x + 1) > 0 ? last - (x + 1) : 0;
+        wait_vm(__builtin_amdgcn_readfirstlane((x < PF ? x : PF) + ahead * per));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (x + 1 + PF <= nx) issue(x + 1 + PF);
+        const int sc = x % R, sn = (x + 1) % R;
+        auto ld3 = [&](const float *p, pde_f4 &a3, pde_f4 &b3, pde_f4 &c3) {
+            a3 = *reinterpret_cast<const pde_f4 *>(p);
+            b3 = *reinterpret_cast<const pde_f4 *>(p + 4);
+            c3 = *reinterpret_cast<const pde_f4 *>(p + 8);
+        };
+        pde_f4 va, vb, vc;
+        ld3(&ring[sc][w + 1][c0], va, vb, vc);
+        const pde_f4 vxc = comp_x(va, vb, vc), vy = comp_y(va, vb, vc), vz = comp_z(va, vb, vc);
+        ld3(&ring[sn][w + 1][c0], va, vb, vc);
+        const pde_f4 vxn = comp_x(va, vb, vc);
+        ld3(&ring[sc][w][c0], va, vb, vc);
+        const pde_f4 lo = comp_y(va, vb, vc);
+        ld3(&ring[sc][w + 2][c0], va, vb, vc);
+        const pde_f4 hi = comp_y(va, vb, vc);
+        const float l = ring[sc][w + 1][c0 - 1];
+        const float r = ring[sc][w + 1][c0 + 14];
+        const pde_f4 vxl = down ? vxn : prev, vxr = down ? prev : vxn;
+        float d[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float zm = t == 0 ? l : el4(vz, t > 0 ? t - 1 : 0);
+            const float zp = t == 3 ? r : el4(vz, t < 3 ? t + 1 : 3);
+            float acc = 0.0f;
+            acc += cm * el4(vxl, t);
+            acc += cc * el4(vxc, t);
+            acc += cp * el4(vxr, t);
+            acc += cm * el4(lo, t);
+            acc += cc * el4(vy, t);
+            acc += cp * el4(hi, t);
+            acc += cm * zm;
+            acc += cc * el4(vz, t);
+            acc += cp * zp;
+            d[t] = alpha * acc;
+        }
+        const int ap = down ? nx - 1 - x : x;
+        st4_nt(out + (x0 + ap + g) * m.SX + n0 + 4 * lane, pde_f4{d[0], d[1], d[2], d[3]});
+        prev = vxc;
+    }
+}
+
+static int div3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
+                             float wc, float wp, float alpha, hipStream_t s)
+{
+    Ring3 m;
+    m.SX = G.S[0];
+    m.SY = G.S[1];
+    m.E0 = (int32_t)G.E[0];
+    m.E1 = (int32_t)G.E[1];
+    m.E2 = (int32_t)G.E[2];
+    m.g = ghost;
+    const int64_t ty = (G.E[1] + DR_TB - 1) / DR_TB, tz = G.E[2] / 256;
+    const int64_t T = tz * ty;
+    const int64_t per = (T + 7) / 8;
+    const int64_t nb = 8 * per * ((G.E[0] + DR_NXC - 1) / DR_NXC);
+    if (nb > 0x7fffffff) return PDE_ESHAPE;
+    m.tz = (uint32_t)tz;
+    m.T = (uint32_t)T;
+    m.per = (uint32_t)per;
+    hipLaunchKernelGGL(div3d_ring, dim3((unsigned)nb), dim3(64, DR_TB), 0, s, in, out, m, wm, wc,
+                       wp, alpha);
+    return launch_status();
 }
 
 // Generic periodic Laplacian (any ncomp, 1..3 eligible axes): thread per
@@ -952,35 +1156,12 @@ static int ops3d_band_launch(const float *s_, const float *v, float *lap, float 
     return launch_status();
 }
 
-template <bool PER, int NW>
-static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
-                             float wc, float wp, float alpha, hipStream_t s)
+// The LDS plane rings take 3D grids whose rows split into whole wave
+// segments (zt points) and that have at least tb rows; extents stay below
+// 2^31.
+static bool ring_fits(const Geom &G, int64_t zt, int64_t tb)
 {
-    Ring3 m;
-    m.SX = G.S[0];
-    m.SY = G.S[1];
-    m.E0 = (int32_t)G.E[0];
-    m.E1 = (int32_t)G.E[1];
-    m.E2 = (int32_t)G.E[2];
-    m.g = ghost;
-    const int64_t ty = (G.E[1] + RT_TB - 1) / RT_TB;
-    const int64_t T = (G.E[2] / (256 * NW)) * ty;
-    const int64_t per = (T + 7) / 8;
-    const int64_t nb = 8 * per * ((G.E[0] + RT_NXC - 1) / RT_NXC);
-    if (nb > 0x7fffffff) return PDE_ESHAPE;
-    m.ty = (uint32_t)ty;
-    m.T = (uint32_t)T;
-    m.per = (uint32_t)per;
-    hipLaunchKernelGGL((lap3d_ring<PER, NW>), dim3((unsigned)nb), dim3(64 * NW, RT_TB), 0, s, in,
-                       out, m, wm, wc, wp, alpha);
-    return launch_status();
-}
-
-// The LDS plane ring takes 3D grids whose rows split into whole 256-point
-// wave segments and that have at least RT_TB rows; extents stay below 2^31.
-static bool lap3d_ring_fits(const Geom &G)
-{
-    return G.E[2] % 256 == 0 && G.E[1] >= RT_TB && G.E[0] < 0x7fffffffLL &&
+    return G.E[2] % zt == 0 && G.E[1] >= tb && G.E[0] < 0x7fffffffLL &&
            G.E[1] < 0x7fffffffLL && G.E[2] < 0x7fffffffLL;
 }
 
@@ -988,7 +1169,7 @@ template <bool PER>
 static int lap3d_band_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
                              float wc, float wp, float alpha, hipStream_t s)
 {
-    if (lap3d_ring_fits(G))
+    if (ring_fits(G, 256, RT_TB))
         return G.E[2] % 512 == 0
                    ? lap3d_ring_launch<PER, 2>(in, out, G, ghost, wm, wc, wp, alpha, s)
                    : lap3d_ring_launch<PER, 1>(in, out, G, ghost, wm, wc, wp, alpha, s);
@@ -1112,6 +1293,8 @@ extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64
     if (st) return st;
     if (G.ndim == 0) return PDE_ESHAPE;
     hipStream_t s = as_stream(stream);
+    if (G.ndim == 3 && nrows == 1 && ring_fits(G, 256, DR_TB))
+        return div3d_ring_launch(in, out, G, ghost, wm, wc, wp, alpha, s);
     if (G.ndim == 3 && nrows == 1 && lap3d_band_fits(G)) {
         const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, 0.0f, alpha};
         if (G.E[1] >= 8) {   // 8-row tiles: 450 vs 456 us at 512^3
@@ -1175,8 +1358,22 @@ extern "C" int pde_fused_lap_grad_div_f32(const float *s_, const float *v, float
     if (G.ndim != 3) return PDE_EINVAL;
     if (lap3d_band_fits(G)) {
         const BandW W = {lm, lc, lp, al, cm, cc, cp, ag, ad};
-        return ops3d_band_launch<BAND_LAP | BAND_GRAD | BAND_DIV, false>(
-            s_, v, lap, grad, div, G, ghost, W, as_stream(stream));
+        hipStream_t s = as_stream(stream);
+        // Grids the Divergence ring takes: the scalar half (Laplacian and Grad
+        // of s, one band launch with Grad's 8-row 2-plane tiles) and the
+        // vector half (the ring) as two launches.  Each input is still read
+        // once and each output written once (the same 36 bytes per node), and
+        // the pair measured 872 against 961 us for the single fused band
+        // launch at 512^3, whose scalar and vector paths together need 137
+        // VGPRs (3 waves per SIMD).
+        if (G.E[1] >= 8 && ring_fits(G, 256, DR_TB)) {
+            st = ops3d_band_launch<BAND_LAP | BAND_GRAD, false, 8, 2>(s_, nullptr, lap, grad,
+                                                                    nullptr, G, ghost, W, s);
+            if (st) return st;
+            return div3d_ring_launch(v, div, G, ghost, cm, cc, cp, ad, s);
+        }
+        return ops3d_band_launch<BAND_LAP | BAND_GRAD | BAND_DIV, false>(s_, v, lap, grad, div, G,
+                                                                        ghost, W, s);
     }
     constexpr int BY = 4, NXC = 64;
     March3 m;
