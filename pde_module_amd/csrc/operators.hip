@@ -492,7 +492,8 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
 // 3D scalar Laplacian through an LDS plane ring (rows of 256 * NW points).
 //
 // The same tiles, chunks and XCD band order as the band kernels (RT_TB = 4
-// rows x 256 * NW z points per block, RT_NXC = 12 planes per chunk), but the
+// rows x 256 * NW z points per block, chunks of RT_NXC_LONG = 32 planes where
+// that still gives 2048 blocks, else RT_NXC = 12), but the
 // planes never pass through registers on their way in: every wave streams its
 // row segment of each plane (16 bytes per lane) straight into LDS with
 // global_load_lds_dwordx4, RT_PF planes ahead of the plane being computed.
@@ -523,12 +524,14 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
 // E1 >= RT_TB; the arithmetic and its order are the band kernel's.
 constexpr int RT_TB = 4;
 constexpr int RT_NXC = 12;
+constexpr int RT_NXC_LONG = 32;
 constexpr int RT_PF = 3;
 
 struct Ring3 {
     int64_t SX, SY;
     int32_t E0, E1, E2, g;
     uint32_t tz, T, per;   // z tiles, tiles per chunk, tiles per XCD band
+    int32_t nxc;           // planes per chunk
 };
 
 template <bool PER>
@@ -572,7 +575,8 @@ __global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__res
                                                               float wm, float wc, float wp,
                                                               float alpha)
 {
-    constexpr int TB = RT_TB, NXC = RT_NXC, PF = RT_PF;
+    constexpr int TB = RT_TB, PF = RT_PF;
+    const int NXC = m.nxc;
     constexpr int R = PF + 2;                // ring slots; march plane p in slot p % R
     constexpr int ZT = 256 * NW;             // z points per tile
     constexpr int RW = ZT + 8;               // floats per LDS row
@@ -688,8 +692,15 @@ static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int gho
     const int64_t ty = (G.E[1] + RT_TB - 1) / RT_TB, tz = G.E[2] / (256 * NW);
     const int64_t T = tz * ty;
     const int64_t per = (T + 7) / 8;
-    const int64_t nb = 8 * per * ((G.E[0] + RT_NXC - 1) / RT_NXC);
+    // long chunks re-read fewer chunk-boundary planes (512^3: 180-182 us
+    // with 32 planes, 178-181 with 64, 184-187 with 12, 185 with 48, which
+    // leaves a short last chunk); short ones keep small grids' blocks many
+    const int64_t nxc = 8 * per * ((G.E[0] + RT_NXC_LONG - 1) / RT_NXC_LONG) >= 2048
+                            ? RT_NXC_LONG
+                            : RT_NXC;
+    const int64_t nb = 8 * per * ((G.E[0] + nxc - 1) / nxc);
     if (nb > 0x7fffffff) return PDE_ESHAPE;
+    m.nxc = (int32_t)nxc;
     m.tz = (uint32_t)tz;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;
@@ -722,7 +733,8 @@ __global__ __launch_bounds__(64 * DR_TB) void div3d_ring(const float *__restrict
                                                          float cm, float cc, float cp,
                                                          float alpha)
 {
-    constexpr int TB = DR_TB, NXC = DR_NXC, PF = DR_PF;
+    constexpr int TB = DR_TB, PF = DR_PF;
+    const int NXC = m.nxc;
     constexpr int R = PF + 2;                // march plane p in slot p % R
     constexpr int ZT = 256;
     constexpr int RW = 3 * ZT + 8;
@@ -852,6 +864,7 @@ static int div3d_ring_launch(const float *in, float *out, const Geom &G, int gho
     const int64_t per = (T + 7) / 8;
     const int64_t nb = 8 * per * ((G.E[0] + DR_NXC - 1) / DR_NXC);
     if (nb > 0x7fffffff) return PDE_ESHAPE;
+    m.nxc = DR_NXC;
     m.tz = (uint32_t)tz;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;

@@ -269,6 +269,31 @@ def test_split_fused_pass_matches_oracle(dev, corc, interior, g):
                      ghost_free_mask(shape, g, (1,)))
 
 
+def test_ring_kernels_every_chunk_length(dev, corc):
+    """Every x-chunk length of the plane rings (1 .. 17 interior planes: single
+    and partial chunks, chunks shorter than the planes in flight, up and down
+    marches) for the plain and periodic Laplacian and the vec3 Divergence,
+    against the C restatement bit for bit."""
+    rng = np.random.default_rng(24)
+    dx = 0.039
+    for e0 in range(1, 18):
+        shape = (e0 + 2, 6, 258)
+        print(f"ring chunk sweep shape={shape}", flush=True)
+        u = rng.uniform(-1, 1, shape + (1,)).astype(np.float32)
+        ud = to_dev(u, dev)
+        for periodic in (False, True):
+            lap = (PeriodicLaplacian if periodic else Laplacian)(1, dx, 1)
+            got = lap(ud, 0.8).cpu().numpy()
+            src = corc.periodic_fill(u.copy(), 1) if periodic else u
+            want = corc.laplacian(src, lap.stencil, np.float32(0.8), 1)
+            assert same_bits(got, want, ghost_free_mask(shape, 1, (1,))), (e0, periodic)
+        v = rng.uniform(-1, 1, shape + (3,)).astype(np.float32)
+        div = Divergence(3, shape, dx, 1)
+        got = div(to_dev(v, dev), 1.2).cpu().numpy()
+        want = corc.divergence(v, div.stencil, np.float32(1.2), 1)
+        assert same_bits(got, want, ghost_free_mask(shape, 1, (1,))), e0
+
+
 def test_full_size_periodic_laplacian_matches_oracle(dev, corc):
     """The 512^3 periodic benchmark grid (514^3 padded) against the OpenMP C
     restatement, bit for bit on all 134M interior points."""
@@ -290,6 +315,26 @@ def test_full_size_periodic_laplacian_matches_oracle(dev, corc):
     u[:, :, 0].zero_()
     per = PeriodicLaplacian(1, 1.0 / n, g)(u, 1.0).cpu().numpy()
     assert np.array_equal(per[inner].view(np.uint32), want[inner].view(np.uint32))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_long_chunk_ring_laplacian_matches_oracle(dev, corc, periodic):
+    """A grid large enough for the Laplacian ring's 32-plane chunks (233 x 1024
+    x 512 interior: 8 chunks, the last of 9 planes marching down) against the
+    OpenMP C restatement, bit for bit."""
+    g = 1
+    shape = (233 + 2, 1024 + 2, 512 + 2)
+    print(f"long-chunk ring laplacian shape={shape} periodic={periodic}", flush=True)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    u = torch.rand(shape + (1,), device=dev, generator=gen) * 2 - 1
+    lap = (PeriodicLaplacian if periodic else Laplacian)(1, 0.01, g)
+    got = lap(u, 0.9).cpu().numpy()
+    uh = u.cpu().numpy()
+    nt = min(16, corc.max_threads())
+    src = corc.periodic_fill(uh.copy(), g) if periodic else uh
+    want = corc.laplacian(src, lap.stencil, np.float32(0.9), g, nthreads=nt)
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(got[inner].view(np.uint32), want[inner].view(np.uint32))
 
 
 def test_full_size_grad_div_fused_match_oracle(dev, corc):
