@@ -27,6 +27,9 @@ constexpr int HT_NT = MS_NT;
 constexpr int HT_RPB = MS_RPB;
 constexpr int HT_OPT = PDE_HEAT_TILE_MAXOPS / HT_NT;   // ops per thread
 
+// 16 bytes of global memory at 4-byte alignment (rows start anywhere)
+typedef float ht_f4u __attribute__((ext_vector_type(4), aligned(4)));
+
 // The heat update of one cell from its u2 neighbours, in heat_rows' order
 // (tile2d.hpp acc5: rows x-1, x, x+1, then columns k-1, k, k+1).
 __device__ __forceinline__ float ht_cell(float up, float mid, float dn, float zl, float zr,
@@ -59,9 +62,31 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     const int64_t P2 = G.p2;
     const float *ug = u + G.offset;
 
-    for (int c = tid; c < rh * HT_PITCH; c += HT_NT) {
-        const int lr = c / HT_PITCH, lc = c - lr * HT_PITCH;
-        if (lc < rw) A[c] = ug[(int64_t)(T.r0 + lr) * P2 + T.c0 + lc];
+    // the region, 4 columns per thread and load, every load issued before
+    // the first is stored (a row's last chunk may be partial: its cells one
+    // by one afterwards)
+    constexpr int Q = HT_PITCH / 4;
+    constexpr int LPT = HT_ROWS * Q / HT_NT;   // chunks per thread
+    {
+        ht_f4u v[LPT];
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const int c = tid + j * HT_NT, lr = c / Q, lc = 4 * (c - lr * Q);
+            if (lr < rh && lc + 4 <= rw)
+                v[j] = *reinterpret_cast<const ht_f4u *>(ug + (int64_t)(T.r0 + lr) * P2 + T.c0 + lc);
+        }
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const int c = tid + j * HT_NT, lr = c / Q, lc = 4 * (c - lr * Q);
+            if (lr >= rh || lc >= rw) continue;
+            if (lc + 4 <= rw) {
+                *reinterpret_cast<ms_f4 *>(&A[lr * HT_PITCH + lc]) =
+                    ms_f4{v[j].x, v[j].y, v[j].z, v[j].w};
+            } else {
+                const float *src = ug + (int64_t)(T.r0 + lr) * P2 + T.c0 + lc;
+                for (int i = 0; lc + i < rw; ++i) A[lr * HT_PITCH + lc + i] = src[i];
+            }
+        }
     }
 
     // this thread's boundary ops, folded for the sub-steps (one time array per
@@ -167,10 +192,21 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     const int orow0 = T.o_r0 - T.r0, orow1 = T.o_r1 - T.r0;
     const int ocol0 = T.o_c0 - T.c0, ocol1 = T.o_c1 - T.c0;
     if (orow0 < 0 || orow1 > rh || ocol0 < 0 || ocol1 > rw) return;
-    const int ow = ocol1 - ocol0;
-    for (int c = tid; c < (orow1 - orow0) * ow; c += HT_NT) {
-        const int r = c / ow, lc = ocol0 + (c - r * ow), lr = orow0 + r;
-        ung[(int64_t)(T.r0 + lr) * P2 + T.c0 + lc] = A[lr * HT_PITCH + lc];
+    // the output rectangle, 4 columns per thread and (streaming) store
+    const int ow = ocol1 - ocol0, oq = (ow + 3) / 4;
+    if (ow <= 0 || orow1 <= orow0) return;
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+        const int c = tid + j * HT_NT, r = c / oq, lc = ocol0 + 4 * (c - r * oq), lr = orow0 + r;
+        if (lr >= orow1) break;
+        const float *a = &A[lr * HT_PITCH + lc];
+        float *dst = ung + (int64_t)(T.r0 + lr) * P2 + T.c0 + lc;
+        if (lc + 4 <= ocol1) {
+            __builtin_nontemporal_store(ht_f4u{a[0], a[1], a[2], a[3]},
+                                        reinterpret_cast<ht_f4u *>(dst));
+        } else {
+            for (int i = 0; lc + i < ocol1; ++i) dst[i] = a[i];
+        }
     }
 }
 
