@@ -29,6 +29,7 @@ constexpr int HT_OPT = PDE_HEAT_TILE_MAXOPS / HT_NT;   // ops per thread
 
 // 16 bytes of global memory at 4-byte alignment (rows start anywhere)
 typedef float ht_f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float ht_f4a __attribute__((ext_vector_type(4)));   // 16-byte aligned
 
 // The heat update of one cell from its u2 neighbours, in heat_rows' order
 // (tile2d.hpp acc5: rows x-1, x, x+1, then columns k-1, k, k+1).
@@ -192,20 +193,26 @@ __global__ __launch_bounds__(HT_NT) void heat_mstep_kernel(
     const int orow0 = T.o_r0 - T.r0, orow1 = T.o_r1 - T.r0;
     const int ocol0 = T.o_c0 - T.c0, ocol1 = T.o_c1 - T.c0;
     if (orow0 < 0 || orow1 > rh || ocol0 < 0 || ocol1 > rw) return;
-    // the output rectangle, 4 columns per thread and (streaming) store
-    const int ow = ocol1 - ocol0, oq = (ow + 3) / 4;
+    // the output rectangle, row by row in 16-byte-aligned chunks of 4
+    // columns (streaming stores; unaligned ones would write partial 32-byte
+    // sectors twice): chunk 0 of a row is its head up to the first aligned
+    // column, the rest aligned chunks, the last one possibly partial
+    const int ow = ocol1 - ocol0, oq = 1 + (ow + 3) / 4;
     if (ow <= 0 || orow1 <= orow0) return;
-#pragma unroll
-    for (int j = 0; j < LPT; ++j) {
-        const int c = tid + j * HT_NT, r = c / oq, lc = ocol0 + 4 * (c - r * oq), lr = orow0 + r;
-        if (lr >= orow1) break;
+    for (int c = tid; c < (orow1 - orow0) * oq; c += HT_NT) {
+        const int r = c / oq, q = c - r * oq, lr = orow0 + r;
+        float *row = ung + (int64_t)(T.r0 + lr) * P2 + T.c0;
+        const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(row + ocol0) >> 2) & 3)) & 3);
+        const int h = head < ow ? head : ow;
+        const int lc = q == 0 ? ocol0 : ocol0 + h + 4 * (q - 1);
+        const int n = q == 0 ? h : (ocol1 - lc < 4 ? ocol1 - lc : 4);
+        if (n <= 0) continue;
         const float *a = &A[lr * HT_PITCH + lc];
-        float *dst = ung + (int64_t)(T.r0 + lr) * P2 + T.c0 + lc;
-        if (lc + 4 <= ocol1) {
-            __builtin_nontemporal_store(ht_f4u{a[0], a[1], a[2], a[3]},
-                                        reinterpret_cast<ht_f4u *>(dst));
+        if (n == 4) {
+            __builtin_nontemporal_store(ht_f4a{a[0], a[1], a[2], a[3]},
+                                        reinterpret_cast<ht_f4a *>(row + lc));
         } else {
-            for (int i = 0; lc + i < ocol1; ++i) dst[i] = a[i];
+            for (int i = 0; i < n; ++i) row[lc + i] = a[i];
         }
     }
 }
