@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--only-heat", action="store_true",
                    help="profiling runs: only the heat workloads (512-grid ensemble, 256^2 "
                         "loop), their JSON on stdout")
+    p.add_argument("--only-ldc", action="store_true",
+                   help="only the module-path lid-driven cavity (the reference's benchmark harness)")
     p.add_argument("--only-cavity", action="store_true",
                    help="profiling runs: only the 1024^2 projection cavity, its JSON on stdout")
     p.add_argument("--cavity-steps", type=int, default=2000,
@@ -698,6 +700,9 @@ def main():
     if args.only_heat:
         print(json.dumps({"heat_ensemble_512": heat_ensemble_timing(dev),
                           "heat_256": heat_config_timing(dev)}))
+        return
+    if args.only_ldc:
+        print(json.dumps({"ldc_201_modules": ldc_module_timing(dev)}))
         return
     if args.only_cavity:
         print(json.dumps({"projection_cavity_1024":
