@@ -836,29 +836,29 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic.get("lap3d_ring<true, 2>") if traffic else None,
-                     "kernel": "lap3d_ring<true, 2> (periodic Laplacian)",
+                     "traffic": traffic.get("lap3d_ring<true, 2, 3, 4>") if traffic else None,
+                     "kernel": "lap3d_ring<true, 2, 3, 4> (periodic Laplacian)",
                      "avg_launch_us": round(t_step_kernel * 1e6, 2),
                      "algorithmic_bytes_per_launch": LAP_BYTES * LUPS},
         "extra": {
             "dropin_laplacian_kernel": {
-                "kernel": "lap3d_ring<false, 2>", "us": round(t_lap * 1e6, 2),
+                "kernel": "lap3d_ring<false, 2, 3, 4>", "us": round(t_lap * 1e6, 2),
                 "GBs": round(lap_gbs, 1), "frac": round(lap_gbs / HBM_PEAK_GBS, 4),
                 "glups": round(LUPS / t_lap / 1e9, 1),
-                "traffic": traffic.get("lap3d_ring<false, 2>") if traffic else None},
+                "traffic": traffic.get("lap3d_ring<false, 2, 3, 4>") if traffic else None},
             "dropin_step_fill_plus_laplacian": {
                 "us": round(t_dropin * 1e6, 2), "glups": round(LUPS / t_dropin / 1e9, 1)},
             "grad": {"us": round(t_grad * 1e6, 1), "GBs": round(GRAD_BYTES * LUPS / t_grad / 1e9, 1),
                      "traffic": traffic.get("ops3d_band<2, false, 8, 2>") if traffic else None},
             "div": {"us": round(t_div * 1e6, 1), "GBs": round(DIV_BYTES * LUPS / t_div / 1e9, 1),
-                    "kernel": "div3d_ring",
-                    "traffic": traffic.get("div3d_ring") if traffic else None},
+                    "kernel": "div3d_ring<2, 8>",
+                    "traffic": traffic.get("div3d_ring<2, 8>") if traffic else None},
             "fused_lap_grad_div": {"us": round(t_fused * 1e6, 1),
                                    "GBs": round(FUSED_BYTES * LUPS / t_fused / 1e9, 1),
-                                   "kernels": ["ops3d_band<3, false, 8, 2>", "div3d_ring"],
+                                   "kernels": ["ops3d_band<3, false, 8, 2>", "div3d_ring<2, 8>"],
                                    "traffic": (traffic["ops3d_band<3, false, 8, 2>"]
-                                               + traffic["div3d_ring"]
-                                               if traffic and "div3d_ring" in traffic
+                                               + traffic["div3d_ring<2, 8>"]
+                                               if traffic and "div3d_ring<2, 8>" in traffic
                                                and "ops3d_band<3, false, 8, 2>" in traffic
                                                else None),
                                    "vs_separate": round((t_lap + t_grad + t_div) / t_fused, 2)},

@@ -522,6 +522,13 @@ __global__ __launch_bounds__(128 * TB) void ops3d_band(const float *__restrict__
 //
 // Requires E2 a multiple of 256 * NW (every lane holds 4 points) and
 // E1 >= RT_TB; the arithmetic and its order are the band kernel's.
+//
+// Prefetch depth and rows per block are template parameters with these
+// defaults.  At 512^3 periodic (NW = 2, 32-plane chunks): PF 3 / TB 4
+// 180-181 us; PF 4 (75 KB, still two blocks per CU) 181; PF 2 184; TB 2
+// (PF 3..6) 194-201, its extra halo rows costing more than the third block
+// per CU gains; TB 8 / PF 2 (one block per CU) 185; PF 4 with 64-plane
+// chunks 187.
 constexpr int RT_TB = 4;
 constexpr int RT_NXC = 12;
 constexpr int RT_NXC_LONG = 32;
@@ -541,43 +548,42 @@ __device__ __forceinline__ int wrap_index(int i, int E)
     return i < 0 ? i + E : (i >= E ? i - E : i);
 }
 
-// s_waitcnt vmcnt(n) for a wave-uniform n <= 15 (the count is an immediate).
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 31 (the count is an immediate:
+// vmcnt bits 3:0 and 5:4 sit at 3:0 and 15:14, expcnt and lgkmcnt stay at
+// their maxima).  Larger counts wait for everything.
+#define PDE_VMCNT(n) (0x0f70 | ((n) & 15) | (((n) >> 4) << 14))
+#define PDE_WAIT_CASE(n) \
+    case n: __builtin_amdgcn_s_waitcnt(PDE_VMCNT(n)); break;
 __device__ __forceinline__ void wait_vm(int n)
 {
     switch (n) {
-    case 0: __builtin_amdgcn_s_waitcnt(0x0f70 | 0); break;
-    case 1: __builtin_amdgcn_s_waitcnt(0x0f70 | 1); break;
-    case 2: __builtin_amdgcn_s_waitcnt(0x0f70 | 2); break;
-    case 3: __builtin_amdgcn_s_waitcnt(0x0f70 | 3); break;
-    case 4: __builtin_amdgcn_s_waitcnt(0x0f70 | 4); break;
-    case 5: __builtin_amdgcn_s_waitcnt(0x0f70 | 5); break;
-    case 6: __builtin_amdgcn_s_waitcnt(0x0f70 | 6); break;
-    case 7: __builtin_amdgcn_s_waitcnt(0x0f70 | 7); break;
-    case 8: __builtin_amdgcn_s_waitcnt(0x0f70 | 8); break;
-    case 9: __builtin_amdgcn_s_waitcnt(0x0f70 | 9); break;
-    case 10: __builtin_amdgcn_s_waitcnt(0x0f70 | 10); break;
-    case 11: __builtin_amdgcn_s_waitcnt(0x0f70 | 11); break;
-    case 12: __builtin_amdgcn_s_waitcnt(0x0f70 | 12); break;
-    case 13: __builtin_amdgcn_s_waitcnt(0x0f70 | 13); break;
-    case 14: __builtin_amdgcn_s_waitcnt(0x0f70 | 14); break;
-    default: __builtin_amdgcn_s_waitcnt(0x0f70 | 0); break;
+    PDE_WAIT_CASE(0) PDE_WAIT_CASE(1) PDE_WAIT_CASE(2) PDE_WAIT_CASE(3)
+    PDE_WAIT_CASE(4) PDE_WAIT_CASE(5) PDE_WAIT_CASE(6) PDE_WAIT_CASE(7)
+    PDE_WAIT_CASE(8) PDE_WAIT_CASE(9) PDE_WAIT_CASE(10) PDE_WAIT_CASE(11)
+    PDE_WAIT_CASE(12) PDE_WAIT_CASE(13) PDE_WAIT_CASE(14) PDE_WAIT_CASE(15)
+    PDE_WAIT_CASE(16) PDE_WAIT_CASE(17) PDE_WAIT_CASE(18) PDE_WAIT_CASE(19)
+    PDE_WAIT_CASE(20) PDE_WAIT_CASE(21) PDE_WAIT_CASE(22) PDE_WAIT_CASE(23)
+    PDE_WAIT_CASE(24) PDE_WAIT_CASE(25) PDE_WAIT_CASE(26) PDE_WAIT_CASE(27)
+    PDE_WAIT_CASE(28) PDE_WAIT_CASE(29) PDE_WAIT_CASE(30) PDE_WAIT_CASE(31)
+    default: __builtin_amdgcn_s_waitcnt(PDE_VMCNT(0)); break;
     }
 }
+#undef PDE_WAIT_CASE
 
 typedef __attribute__((address_space(3))) void *lds_ptr;
 
 // Cache policy operand of the direct-to-LDS loads: non-temporal (streaming).
 constexpr int CPOL_NT = 2;
 
-template <bool PER, int NW>
-__global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__restrict__ s,
-                                                              float *__restrict__ out, Ring3 m,
-                                                              float wm, float wc, float wp,
-                                                              float alpha)
+template <bool PER, int NW, int PF = RT_PF, int TB = RT_TB>
+__global__ __launch_bounds__(64 * NW * TB) void lap3d_ring(const float *__restrict__ s,
+                                                           float *__restrict__ out, Ring3 m,
+                                                           float wm, float wc, float wp,
+                                                           float alpha)
 {
-    constexpr int TB = RT_TB, PF = RT_PF;
     const int NXC = m.nxc;
     constexpr int R = PF + 2;                // ring slots; march plane p in slot p % R
+    static_assert(PF >= 1 && PF + (PF - 1) * 4 <= 31, "wait counts must fit wait_vm");
     constexpr int ZT = 256 * NW;             // z points per tile
     constexpr int RW = ZT + 8;               // floats per LDS row
     __shared__ __attribute__((aligned(16))) float ring[R][TB + 2][RW];
@@ -678,9 +684,10 @@ __global__ __launch_bounds__(64 * NW * RT_TB) void lap3d_ring(const float *__res
     }
 }
 
-template <bool PER, int NW>
+template <bool PER, int NW, int PF = RT_PF, int TB = RT_TB>
 static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
-                             float wc, float wp, float alpha, hipStream_t s)
+                             float wc, float wp, float alpha, hipStream_t s,
+                             int nxc_long = RT_NXC_LONG)
 {
     Ring3 m;
     m.SX = G.S[0];
@@ -689,14 +696,14 @@ static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int gho
     m.E1 = (int32_t)G.E[1];
     m.E2 = (int32_t)G.E[2];
     m.g = ghost;
-    const int64_t ty = (G.E[1] + RT_TB - 1) / RT_TB, tz = G.E[2] / (256 * NW);
+    const int64_t ty = (G.E[1] + TB - 1) / TB, tz = G.E[2] / (256 * NW);
     const int64_t T = tz * ty;
     const int64_t per = (T + 7) / 8;
     // long chunks re-read fewer chunk-boundary planes (512^3: 180-182 us
     // with 32 planes, 178-181 with 64, 184-187 with 12, 185 with 48, which
     // leaves a short last chunk); short ones keep small grids' blocks many
-    const int64_t nxc = 8 * per * ((G.E[0] + RT_NXC_LONG - 1) / RT_NXC_LONG) >= 2048
-                            ? RT_NXC_LONG
+    const int64_t nxc = 8 * per * ((G.E[0] + nxc_long - 1) / nxc_long) >= 2048
+                            ? nxc_long
                             : RT_NXC;
     const int64_t nb = 8 * per * ((G.E[0] + nxc - 1) / nxc);
     if (nb > 0x7fffffff) return PDE_ESHAPE;
@@ -704,14 +711,15 @@ static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int gho
     m.tz = (uint32_t)tz;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;
-    hipLaunchKernelGGL((lap3d_ring<PER, NW>), dim3((unsigned)nb), dim3(64 * NW, RT_TB), 0, s, in,
-                       out, m, wm, wc, wp, alpha);
+    hipLaunchKernelGGL((lap3d_ring<PER, NW, PF, TB>), dim3((unsigned)nb), dim3(64 * NW, TB), 0,
+                       s, in, out, m, wm, wc, wp, alpha);
     return launch_status();
 }
 
 // 3D vec3 Divergence through an LDS plane ring (the Laplacian ring's scheme
-// for a vector field).  A block owns DR_TB = 4 rows x 256 points (one wave per
-// row, 4 points per lane) of DR_NXC = 16 planes; a ring slot holds the tile's
+// for a vector field).  A block owns TB rows x 256 points (one wave per row,
+// 4 points per lane) of a chunk of planes (DR_TB = 4 rows of DR_NXC = 16
+// planes by default; div3d_ring_run picks the shape); a ring slot holds the tile's
 // rows of one plane as the field stores them (3 floats per point, 3 KB per
 // row, three 16-byte direct loads per lane), its two halo rows, and the z
 // components of each row segment's two outer neighbours, placed where the
@@ -719,23 +727,23 @@ static int lap3d_ring_launch(const float *in, float *out, const Geom &G, int gho
 // start at position 4).  The x components of the plane before the current one
 // stay in registers; the plane after gives its x components, the halo rows
 // their y components, and each lane's outer z neighbours are single floats
-// beside its 12.  DR_PF = 2 planes in flight, DR_PF + 2 slots (74 KB: two
-// blocks per CU).  Per plane a wave issues three row loads and two z-end
+// beside its 12.  DR_PF = 2 planes in flight, DR_PF + 2 slots (74 KB at 4
+// rows: two blocks per CU).  Per plane a wave issues three row loads and two z-end
 // loads, the edge rows' waves three more for their halo row; one store per
 // computed plane.  Odd chunks march down as in the Laplacian ring.
-// Requires E2 % 256 == 0 and E1 >= DR_TB; operation order of the band kernel.
+// Requires E2 % 256 == 0 and E1 >= TB; operation order of the band kernel.
 constexpr int DR_TB = 4;
 constexpr int DR_NXC = 16;
 constexpr int DR_PF = 2;
 
-__global__ __launch_bounds__(64 * DR_TB) void div3d_ring(const float *__restrict__ v,
-                                                         float *__restrict__ out, Ring3 m,
-                                                         float cm, float cc, float cp,
-                                                         float alpha)
+template <int PF = DR_PF, int TB = DR_TB>
+__global__ __launch_bounds__(64 * TB) void div3d_ring(const float *__restrict__ v,
+                                                      float *__restrict__ out, Ring3 m,
+                                                      float cm, float cc, float cp, float alpha)
 {
-    constexpr int TB = DR_TB, PF = DR_PF;
     const int NXC = m.nxc;
     constexpr int R = PF + 2;                // march plane p in slot p % R
+    static_assert(PF >= 1 && PF + (PF - 1) * 8 <= 31, "wait counts must fit wait_vm");
     constexpr int ZT = 256;
     constexpr int RW = 3 * ZT + 8;
     __shared__ __attribute__((aligned(16))) float ring[R][TB + 2][RW];
@@ -849,8 +857,9 @@ __global__ __launch_bounds__(64 * DR_TB) void div3d_ring(const float *__restrict
     }
 }
 
+template <int PF = DR_PF, int TB = DR_TB>
 static int div3d_ring_launch(const float *in, float *out, const Geom &G, int ghost, float wm,
-                             float wc, float wp, float alpha, hipStream_t s)
+                             float wc, float wp, float alpha, hipStream_t s, int nxc = DR_NXC)
 {
     Ring3 m;
     m.SX = G.S[0];
@@ -859,18 +868,32 @@ static int div3d_ring_launch(const float *in, float *out, const Geom &G, int gho
     m.E1 = (int32_t)G.E[1];
     m.E2 = (int32_t)G.E[2];
     m.g = ghost;
-    const int64_t ty = (G.E[1] + DR_TB - 1) / DR_TB, tz = G.E[2] / 256;
+    const int64_t ty = (G.E[1] + TB - 1) / TB, tz = G.E[2] / 256;
     const int64_t T = tz * ty;
     const int64_t per = (T + 7) / 8;
-    const int64_t nb = 8 * per * ((G.E[0] + DR_NXC - 1) / DR_NXC);
+    const int64_t nb = 8 * per * ((G.E[0] + nxc - 1) / nxc);
     if (nb > 0x7fffffff) return PDE_ESHAPE;
-    m.nxc = DR_NXC;
+    m.nxc = nxc;
     m.tz = (uint32_t)tz;
     m.T = (uint32_t)T;
     m.per = (uint32_t)per;
-    hipLaunchKernelGGL(div3d_ring, dim3((unsigned)nb), dim3(64, DR_TB), 0, s, in, out, m, wm, wc,
-                       wp, alpha);
+    hipLaunchKernelGGL((div3d_ring<PF, TB>), dim3((unsigned)nb), dim3(64, TB), 0, s, in, out, m,
+                       wm, wc, wp, alpha);
     return launch_status();
+}
+
+// The Divergence ring's tile shape: 8-row tiles of 32-plane chunks (one
+// block per CU) where those still give 2048 blocks, else 4-row tiles of 16
+// planes.  At 512^3: 366 us against 371-372 for 4 rows with 16, 24 or 32
+// planes and 371 for 8 rows with 16; a third plane in flight (one block per
+// CU at 4 rows) 458-480.
+static int div3d_ring_run(const float *in, float *out, const Geom &G, int ghost, float wm,
+                          float wc, float wp, float alpha, hipStream_t s)
+{
+    const int64_t per8 = ((G.E[2] / 256) * ((G.E[1] + 7) / 8) + 7) / 8;
+    if (G.E[1] >= 8 && 8 * per8 * ((G.E[0] + 31) / 32) >= 2048)
+        return div3d_ring_launch<DR_PF, 8>(in, out, G, ghost, wm, wc, wp, alpha, s, 32);
+    return div3d_ring_launch(in, out, G, ghost, wm, wc, wp, alpha, s);
 }
 
 // Generic periodic Laplacian (any ncomp, 1..3 eligible axes): thread per
@@ -1325,7 +1348,7 @@ extern "C" int pde_divergence_f32(const float *in, float *out, int64_t nx, int64
     if (G.ndim == 0) return PDE_ESHAPE;
     hipStream_t s = as_stream(stream);
     if (G.ndim == 3 && nrows == 1 && ring_fits(G, 256, DR_TB))
-        return div3d_ring_launch(in, out, G, ghost, wm, wc, wp, alpha, s);
+        return div3d_ring_run(in, out, G, ghost, wm, wc, wp, alpha, s);
     if (G.ndim == 3 && nrows == 1 && lap3d_band_fits(G)) {
         const BandW W = {0.0f, 0.0f, 0.0f, 0.0f, wm, wc, wp, 0.0f, alpha};
         if (G.E[1] >= 8) {   // 8-row tiles: 450 vs 456 us at 512^3
@@ -1401,7 +1424,7 @@ extern "C" int pde_fused_lap_grad_div_f32(const float *s_, const float *v, float
             st = ops3d_band_launch<BAND_LAP | BAND_GRAD, false, 8, 2>(s_, nullptr, lap, grad,
                                                                     nullptr, G, ghost, W, s);
             if (st) return st;
-            return div3d_ring_launch(v, div, G, ghost, cm, cc, cp, ad, s);
+            return div3d_ring_run(v, div, G, ghost, cm, cc, cp, ad, s);
         }
         return ops3d_band_launch<BAND_LAP | BAND_GRAD | BAND_DIV, false>(s_, v, lap, grad, div, G,
                                                                         ghost, W, s);
