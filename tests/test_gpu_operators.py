@@ -119,7 +119,8 @@ def test_fused_pass_equals_separate_modules(dev):
         assert torch.equal(got[m].view(torch.int32), want[m].view(torch.int32))
 
 
-@pytest.mark.parametrize("shape,g", [((10, 11, 12), 1), ((9, 10, 1), 2), ((8, 1, 1), 1)])
+@pytest.mark.parametrize("shape,g", [((10, 11, 12), 1), ((9, 10, 1), 2), ((8, 1, 1), 1),
+                                     ((12, 9, 13), 3), ((7, 300, 5), 1)])
 def test_periodic_fill_matches_oracle(dev, corc, shape, g):
     rng = np.random.default_rng(4)
     u = rng.uniform(-1, 1, shape + (2,)).astype(np.float32)
