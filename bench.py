@@ -6,7 +6,7 @@ one ghost layer), one MI355X.
 A step is one periodic Laplacian application over the whole grid: one launch
 of the wrapped-index Laplacian (PeriodicLaplacian), which gives bit for bit the
 values of a periodic ghost fill followed by the drop-in Laplacian module.  The
-drop-in path itself (PeriodicBoundary + Laplacian, two modules, four launches)
+drop-in path itself (PeriodicBoundary + Laplacian, two modules, two launches)
 is timed beside it.  The inputs (s ~ U[-1, 1], seeded) are resident in HBM
 before timing starts.  Printed: one JSON line with GLUPS (interior lattice
 updates per second / 1e9), the step kernel's achieved bandwidth against the
